@@ -1,0 +1,302 @@
+/*
+ * shb.h — C ABI of the MI355X SH-lightmap baker (libshb_hip.so).
+ *
+ * Plain C: POD structs, plain pointers and sizes, integer status codes.
+ * Every entry point returns SHB_OK (0) or a negative SHB_ERR_* code and never
+ * aborts across the boundary; the message of the last failure on a context is
+ * returned by shb_last_error().
+ *
+ * Threading: one context per host thread. A context is not safe to use from
+ * several threads at once. All GPU work of a context runs on its own stream.
+ * Results are deterministic for a given (scene, surface, config, seed) when no
+ * material has alpha < 1 (alpha splits add into shared sample accumulators).
+ *
+ * Each entry point names the reference interface it replaces
+ * (DaviesX/sh-baker, paths relative to the repository root). Host buffers are
+ * caller-allocated; the library copies inputs and keeps no host pointers.
+ */
+#ifndef SHB_H_
+#define SHB_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+enum {
+  SHB_OK = 0,
+  SHB_ERR_INVALID_ARG = -1,  /* bad pointer, size or config value          */
+  SHB_ERR_NO_SCENE = -2,     /* no scene uploaded on this context           */
+  SHB_ERR_DEVICE = -3,       /* HIP runtime error                           */
+  SHB_ERR_OOM = -4,          /* device or host allocation failed            */
+  SHB_ERR_LIMIT = -5,        /* a documented size limit was exceeded        */
+  SHB_ERR_NO_SURFACE = -6,   /* bake called before rasterize / set_surface  */
+  SHB_ERR_UNSUPPORTED = -7   /* a reference feature this build rejects      */
+};
+
+/* ----------------------------------------------------------------- scene */
+enum {
+  SHB_LIGHT_POINT = 0,
+  SHB_LIGHT_DIRECTIONAL = 1,
+  SHB_LIGHT_SPOT = 2,
+  SHB_LIGHT_AREA = 3
+};
+
+enum {
+  SHB_GEOM_HAS_NORMALS = 1u << 0,
+  SHB_GEOM_HAS_TEXTURE_UVS = 1u << 1,
+  SHB_GEOM_HAS_LIGHTMAP_UVS = 1u << 2,
+  SHB_GEOM_HAS_TANGENTS = 1u << 3
+};
+
+/* One reference `Geometry` (src/scene.h:56-67). Vertex attributes live in the
+ * flat arrays of shb_scene_desc at [vertex_offset, vertex_offset+vertex_count);
+ * indices at [index_offset, index_offset+index_count) are local to the
+ * geometry, three per triangle. */
+typedef struct shb_geometry {
+  uint32_t vertex_offset;
+  uint32_t vertex_count;
+  uint32_t index_offset;
+  uint32_t index_count;
+  int32_t material_id;  /* index into materials; must be valid              */
+  uint32_t flags;       /* SHB_GEOM_HAS_*                                    */
+  float transform[12];  /* row-major 3x4 affine (object -> world)            */
+} shb_geometry;
+
+/* One reference `Material` (src/scene.h:43-53): the albedo texture (8-bit
+ * sRGB, row-major, 3 or 4 channels) and the emission intensity. */
+typedef struct shb_material {
+  uint32_t albedo_width;
+  uint32_t albedo_height;
+  uint32_t albedo_channels; /* 0: no texture (albedo 0.8 grey), 3 or 4      */
+  uint32_t reserved;
+  uint64_t albedo_offset;   /* byte offset into texel_pool                  */
+  float emission_intensity;
+  float reserved2;
+} shb_material;
+
+/* One reference `Light` (src/scene.h:70-87). Area lights name the geometry
+ * whose triangles emit and the material that gives their emission. */
+typedef struct shb_light {
+  int32_t type;            /* SHB_LIGHT_*                                    */
+  float position[3];
+  float direction[3];
+  float color[3];
+  float intensity;
+  float cos_inner_cone;
+  float cos_outer_cone;
+  int32_t geometry_index;  /* area lights only, else -1                      */
+  int32_t material_index;  /* area lights only, else -1                      */
+} shb_light;
+
+/* The whole reference `Scene` (src/scene.h:110-115) as flat arrays.
+ * Optional per-vertex arrays may be NULL when no geometry sets the flag. */
+typedef struct shb_scene_desc {
+  uint32_t num_vertices;
+  const float* positions;     /* 3 * num_vertices, object space */
+  const float* normals;       /* 3 * num_vertices or NULL       */
+  const float* texture_uvs;   /* 2 * num_vertices or NULL       */
+  const float* lightmap_uvs;  /* 2 * num_vertices or NULL       */
+  const float* tangents;      /* 4 * num_vertices or NULL       */
+  uint32_t num_indices;
+  const uint32_t* indices;
+  uint32_t num_geometries;
+  const shb_geometry* geometries;
+  uint32_t num_materials;
+  const shb_material* materials;
+  uint64_t texel_pool_size;
+  const uint8_t* texel_pool;
+  uint32_t num_lights;
+  const shb_light* lights;
+} shb_scene_desc;
+
+/* ------------------------------------------------------ raster / surface */
+/* Reference `RasterConfig` (src/rasterizer.h:12-16). */
+typedef struct shb_raster_cfg {
+  int32_t width;
+  int32_t height;
+  int32_t supersample_scale;
+} shb_raster_cfg;
+
+/* Reference `SurfacePoint` (src/rasterizer.h:18-23), packed to 48 bytes. */
+typedef struct shb_surface_point {
+  float position[3];
+  float normal[3];
+  float tangent[4];
+  int32_t material_id;  /* < 0: texel not covered */
+  int32_t reserved;
+} shb_surface_point;
+
+/* ------------------------------------------------------------------ bake */
+enum {
+  SHB_BAKE_COUNT_VISITS = 1u << 0  /* also count BVH node visits / triangle
+                                      tests (slower; for measurement only) */
+};
+
+/* Reference `BakeConfig` (src/baker.h:10-14) plus the seed of the
+ * counter-based random stream. */
+typedef struct shb_bake_cfg {
+  int32_t samples;           /* rays per texel                              */
+  int32_t bounces;           /* max path depth                              */
+  int32_t num_light_samples; /* must be 1 (the reference CLI always uses 1) */
+  uint32_t flags;            /* SHB_BAKE_*                                  */
+  uint64_t seed;
+} shb_bake_cfg;
+
+typedef struct shb_bake_stats {
+  uint64_t valid_texels;    /* texels with material_id >= 0                 */
+  uint64_t paths;           /* valid_texels * samples                       */
+  uint64_t rays_extension;  /* closest-hit queries traced                   */
+  uint64_t rays_shadow;     /* any-hit queries traced                       */
+  uint64_t node_visits;     /* only with SHB_BAKE_COUNT_VISITS              */
+  uint64_t tri_tests;       /* only with SHB_BAKE_COUNT_VISITS              */
+  double seconds_total;     /* wall time of the call                        */
+  double seconds_device;    /* GPU time between the first and last kernel   */
+  double seconds_trace;     /* GPU time inside the traversal kernels        */
+} shb_bake_stats;
+
+/* Launch parameters of the wavefront kernels (chosen by shb_autotune). */
+typedef struct shb_tuning {
+  int32_t block_size;       /* threads per block of the traversal kernels   */
+  int32_t batch_paths;      /* path slots resident per wavefront batch      */
+  int32_t refill_threshold; /* refill a wave when fewer lanes are active    */
+  int32_t bvh_width;        /* children per BVH node, 2..8 (rebuilds)       */
+} shb_tuning;
+
+/* ------------------------------------------------------ fixed-ray queries */
+typedef struct shb_ray {
+  float origin[3];
+  float tnear;
+  float direction[3];  /* need not be unit length; t is in units of |d| */
+  float tfar;
+} shb_ray;
+
+/* Closest hit in the reference's convention (src/occlusion.cpp:9-107):
+ * prim_id < 0 means miss. u, v are Embree barycentrics (weights of vertices 1
+ * and 2); position = o + d*t; normal is the interpolated object-space vertex
+ * normal (the world geometric normal without vertex normals); uv the texture
+ * uv. Equal-t candidates resolve to the lowest prim_id. */
+typedef struct shb_hit {
+  float t;
+  float u;
+  float v;
+  int32_t prim_id;      /* global triangle index (geometry-major order)    */
+  int32_t geom_id;
+  int32_t material_id;
+  float position[3];
+  float normal[3];
+  float uv[2];
+} shb_hit;
+
+/* ------------------------------------------------------------- lifecycle */
+typedef struct shb_ctx shb_ctx;
+
+/* Creates a context on HIP device `device` with its own stream. */
+int shb_create(shb_ctx** out, int device);
+void shb_destroy(shb_ctx* ctx);
+/* Message of the last failure on ctx ("" if none); valid until the next call. */
+const char* shb_last_error(const shb_ctx* ctx);
+const char* shb_version(void);
+
+/* Uploads the scene and builds the BVH on the host (binned SAH, collapsed to
+ * the compressed 8-wide node layout).
+ * Replaces BuildBVH / ReleaseBVH (src/scene.cpp:209-264, src/scene.h:129-132)
+ * and the scene arguments of BakeSHLightMap (src/baker.h:25-28). */
+int shb_upload_scene(shb_ctx* ctx, const shb_scene_desc* scene);
+
+/* Rasterizes the lightmap uvs into (width*ss) x (height*ss) surface points,
+ * kept on the device for shb_bake; copies them to `out` when not NULL.
+ * Replaces RasterizeScene (src/rasterizer.cpp:68-211). */
+int shb_rasterize(shb_ctx* ctx, const shb_raster_cfg* cfg,
+                  shb_surface_point* out);
+
+/* Uploads caller-made surface points (the `surface_points` argument of
+ * BakeSHLightMap, src/baker.h:25-28); count = width*ss * height*ss. */
+int shb_set_surface(shb_ctx* ctx, const shb_raster_cfg* cfg,
+                    const shb_surface_point* points);
+
+/* Validity mask of the current surface: 1 where material_id >= 0.
+ * Replaces CreateValidityMask (src/rasterizer.cpp:213-220). */
+int shb_validity_mask(shb_ctx* ctx, uint8_t* mask_out);
+
+/* Bakes the SH9 lightmap at the supersampled resolution:
+ * sh_out[texel*27 + coeff*3 + channel], vis_out[texel] (either may be NULL).
+ * Replaces BakeSHLightMap (src/baker.cpp:186-304). */
+int shb_bake(shb_ctx* ctx, const shb_bake_cfg* cfg, float* sh_out,
+             float* vis_out, shb_bake_stats* stats);
+
+/* Bake followed by the post-bake steps of the reference CLI, all on the
+ * device: validity mask, ss x ss downsample of SH, visibility and mask,
+ * then `dilation` passes on SH and on visibility with the mask the SH pass
+ * grew. Outputs are at the base width x height.
+ * Replaces main.cpp:131-163 (BakeSHLightMap, CreateValidityMask,
+ * DownsampleSHTexture, DownsampleEnvironmentVisibilityTexture,
+ * DownsampleValidityMask, Dilate x2). */
+int shb_bake_lightmap(shb_ctx* ctx, const shb_bake_cfg* cfg,
+                      int32_t dilation, float* sh_out, float* vis_out,
+                      uint8_t* mask_out, shb_bake_stats* stats);
+
+/* ss x ss box average of a (width*scale) x (height*scale) image of
+ * `channels` floats per texel into width x height. Replaces DownsampleTexture
+ * / DownsampleSHTexture / DownsampleEnvironmentVisibilityTexture
+ * (src/baker.cpp:151-182, 306-323). */
+int shb_downsample(shb_ctx* ctx, int32_t width, int32_t height, int32_t scale,
+                   int32_t channels, const float* in, float* out);
+
+/* Any-valid downsample of a mask. Replaces DownsampleValidityMask
+ * (src/rasterizer.cpp:222-241). */
+int shb_downsample_mask(shb_ctx* ctx, int32_t width, int32_t height,
+                        int32_t scale, const uint8_t* in, uint8_t* out);
+
+/* In-place dilation of `channels` floats per texel and of the mask.
+ * Replaces Dilate<T> (src/dilation.cpp:7-73). */
+int shb_dilate(shb_ctx* ctx, int32_t width, int32_t height, int32_t channels,
+               float* data, uint8_t* mask, int32_t passes);
+
+/* Traces a list of rays against the uploaded scene (closest hit, or any hit
+ * when any_hit != 0: then only prim_id >= 0 / < 0 is meaningful).
+ * Replaces FindOcclusion (src/occlusion.cpp:9-107). node_visits / tri_tests,
+ * when not NULL, receive per-ray traversal counts. */
+int shb_trace_rays(shb_ctx* ctx, const shb_ray* rays, uint32_t n,
+                   int32_t any_hit, shb_hit* hits, uint32_t* node_visits,
+                   uint32_t* tri_tests);
+
+/* ------------------------------------------------ batched (no reference) */
+/* Bakes several contexts (each with its own scene and surface; all on the
+ * same device) in one persistent launch per bounce over a shared ray queue,
+ * with the bounce loop captured in a hipGraph. sh_outs[i] / vis_outs[i] as in
+ * shb_bake. No reference counterpart: the reference bakes one lightmap per
+ * process (src/main.cpp:40-193). */
+int shb_bake_batched(shb_ctx* const* ctxs, int32_t num_jobs,
+                     const shb_bake_cfg* cfgs, float* const* sh_outs,
+                     float* const* vis_outs, shb_bake_stats* stats);
+
+/* Chooses block size, batch size, refill threshold and BVH width by timing
+ * short bakes of the current scene and surface; applies and returns them. */
+int shb_autotune(shb_ctx* ctx, const shb_bake_cfg* cfg, shb_tuning* best);
+int shb_set_tuning(shb_ctx* ctx, const shb_tuning* tuning);
+int shb_get_tuning(const shb_ctx* ctx, shb_tuning* tuning);
+
+/* ------------------------------------------- differentiation (no reference) */
+/* Per-light RGB radiance scales and per-material RGB albedo scales (all 1 by
+ * default, which reproduces the reference bake). Either pointer may be NULL. */
+int shb_set_params(shb_ctx* ctx, const float* light_scale,
+                   const float* albedo_scale);
+
+/* Path-replay adjoint: given dL/dSH at the supersampled bake resolution
+ * (same layout as shb_bake's sh_out), writes dL/d(light_scale) (3 per light)
+ * and dL/d(albedo_scale) (3 per material). Light-selection probabilities are
+ * held fixed (detached); every path is re-traced from its counter-based seed,
+ * so memory stays O(texels). No reference counterpart. */
+int shb_bake_adjoint(shb_ctx* ctx, const shb_bake_cfg* cfg,
+                     const float* dloss_dsh, float* dloss_dlight,
+                     float* dloss_dalbedo, shb_bake_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SHB_H_ */

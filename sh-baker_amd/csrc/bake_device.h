@@ -1,0 +1,220 @@
+// Device-side shading of the bake: materials, lights, SH9, sampling.
+// Restates the reference's per-vertex arithmetic in the same operation order
+// (compiled with -ffp-contract=off); references are to DaviesX/sh-baker.
+#ifndef SHB_BAKE_DEVICE_H_
+#define SHB_BAKE_DEVICE_H_
+
+#include "shb_common.h"
+
+namespace shb {
+
+constexpr double kPi = 3.14159265358979323846;
+
+// src/material.cpp:69-85 GetAlbedo: nearest texel, clamp, per-fetch sRGB
+// decode (256-entry table equal to SRGBToLinear, colorspace.h:10-14).
+__device__ __forceinline__ f3 get_albedo(const SceneView& sc, int m, float u, float v) {
+  const MaterialDev& M = sc.mats[m];
+  if (M.tex_c == 0) return {0.8f, 0.8f, 0.8f};
+  int tx = min(max((int)(u * (float)M.tex_w), 0), (int)M.tex_w - 1);
+  int ty = min(max((int)(v * (float)M.tex_h), 0), (int)M.tex_h - 1);
+  const uint8_t* px = sc.texels + M.tex_off + ((size_t)ty * M.tex_w + tx) * M.tex_c;
+  return {sc.srgb_lut[px[0]], sc.srgb_lut[px[1]], sc.srgb_lut[px[2]]};
+}
+
+// src/material.cpp:57-67 GetAlpha.
+__device__ __forceinline__ float get_alpha(const SceneView& sc, int m, float u, float v) {
+  const MaterialDev& M = sc.mats[m];
+  if (M.tex_c < 4) return 1.0f;
+  int tx = min(max((int)(u * (float)M.tex_w), 0), (int)M.tex_w - 1);
+  int ty = min(max((int)(v * (float)M.tex_h), 0), (int)M.tex_h - 1);
+  const uint8_t* px = sc.texels + M.tex_off + ((size_t)ty * M.tex_w + tx) * M.tex_c;
+  return px[3] / 255.0f;
+}
+
+// src/material.cpp:87-92 GetEmission (unscaled albedo x intensity).
+__device__ __forceinline__ f3 get_emission(const SceneView& sc, int m, float u, float v) {
+  float e = sc.mats[m].emission;
+  if (e > 0.0f) return get_albedo(sc, m, u, v) * e;
+  return {0.f, 0.f, 0.f};
+}
+
+// Lambert BRDF value albedo/pi (material.cpp:160-161) with the material's
+// albedo scale (1 reproduces the reference).
+__device__ __forceinline__ f3 lambert(const SceneView& sc, int m, float u, float v) {
+  f3 a = get_albedo(sc, m, u, v);
+  const float* s = sc.albedo_scale + 3 * m;
+  a = cmul(a, f3{s[0], s[1], s[2]});
+  return a / (float)kPi;
+}
+
+// src/baker.cpp:38-46 SampleHemisphereUniform given its uniforms.
+__device__ __forceinline__ f3 hemisphere(float u1, float u2) {
+  float r = sqrtf(1.0f - u1 * u1);
+  float phi = (float)(2.0 * kPi * (double)u2);
+  return {r * cosf(phi), r * sinf(phi), u1};
+}
+
+// src/material.cpp:113-146 SampleMaterial (Lambert, cosine-weighted).
+__device__ __forceinline__ void sample_lambert(f3 n, float r1, float r2, f3& dir, float& pdf) {
+  f3 t;
+  if (fabsf(n.x) > fabsf(n.z))
+    t = {-n.y, n.x, 0.0f};
+  else
+    t = {0.0f, -n.z, n.y};
+  t = normalized(t);
+  f3 b = cross(n, t);
+  float phi = (float)(2.0 * kPi * (double)r1);
+  float theta = acosf(sqrtf(r2));
+  float sin_theta = sinf(theta);
+  f3 l = {cosf(phi) * sin_theta, sinf(phi) * sin_theta, cosf(theta)};
+  f3 w = t * l.x + b * l.y + n * l.z;
+  dir = normalized(w);
+  pdf = (float)((double)l.z / kPi);
+}
+
+// src/sh_coeffs.cpp:50-78 basis (direction not normalised, constants :27-31).
+__device__ __forceinline__ void sh_basis(f3 d, float Y[9]) {
+  const float x = d.x, y = d.y, z = d.z;
+  Y[0] = 0.282095f;
+  Y[1] = 0.488603f * y;
+  Y[2] = 0.488603f * z;
+  Y[3] = 0.488603f * x;
+  Y[4] = 1.092548f * x * y;
+  Y[5] = 1.092548f * y * z;
+  Y[6] = 0.315392f * (3.0f * z * z - 1.0f);
+  Y[7] = 1.092548f * x * z;
+  Y[8] = 0.546274f * (x * x - y * y);
+}
+
+struct NeeResult {
+  f3 L;        // radiance / joint pdf (light.cpp:185, 287-288)
+  f3 dir;      // unnormalised shadow direction
+  f3 origin;
+  float tfar;
+  int light;
+};
+
+// One NEE sample over all lights (light.cpp:82-189 at a path vertex when
+// VERTEX, light.cpp:191-293 at a texel otherwise). Unshadowed radiance and a
+// pending shadow ray per light (light.h:34-203; area lights sampled as in
+// light.cpp:16-78), weight = maxCoeff, selection by a streaming reservoir:
+// for each light draw u, W += w, keep light i when u*W < w -- the same
+// w_i / sum(w) selection law as std::discrete_distribution. Returns false
+// when nothing is selected, sum(w) < 1e-6 or joint pdf < 1e-3.
+template <bool VERTEX>
+__device__ __forceinline__ bool nee_sample(const SceneView& sc, f3 P, f3 N,
+                                           f3 brdf, bool front, Rng& rng,
+                                           NeeResult& out) {
+  float W = 0.f;
+  int pick = -1;
+  float pw = 0.f, pap = 1.f, ptfar = 0.f;
+  f3 prad = {0.f, 0.f, 0.f}, pdir = {0.f, 0.f, 0.f};
+  const int nl = sc.num_lights;
+  for (int l = 0; l < nl; ++l) {
+    const LightDev& L = sc.lights[l];
+    f3 rad = {0.f, 0.f, 0.f}, sdir;
+    float tfar, ap = 1.0f;
+    const int type = L.type;
+    if (type == kLightDirectional) {
+      f3 ld = {L.dir[0], L.dir[1], L.dir[2]};
+      sdir = -ld;
+      tfar = 1.0e10f;
+      float cos_n = dot(N, -ld);
+      if (!(cos_n < 0.f)) {
+        rad = f3{L.ic[0], L.ic[1], L.ic[2]};
+        if (VERTEX) {
+          bool ok = front && dot(N, -ld) > 0.0f;
+          f3 bv = ok ? brdf : f3{0.f, 0.f, 0.f};
+          rad = cmul(rad, cos_n * bv);
+        }
+      }
+    } else {
+      f3 target;
+      f3 emit = {0.f, 0.f, 0.f};
+      bool area = type == kLightArea;
+      if (area) {
+        if (L.num_tris == 0) {
+          ap = 0.f;
+          target = f3{0.f, 0.f, 0.f};
+        } else {
+          uint32_t tri = rng.index(L.num_tris);
+          float u1 = rng.uniform();
+          float u2 = rng.uniform();
+          const AreaTri& at = sc.area_tris[L.first_tri + tri];
+          if (u1 + u2 > 1.0f) {
+            u1 = 1.0f - u1;
+            u2 = 1.0f - u2;
+          }
+          float w = 1.0f - u1 - u2;
+          f3 v0 = {at.v0[0], at.v0[1], at.v0[2]};
+          f3 v1 = {at.v1[0], at.v1[1], at.v1[2]};
+          f3 v2 = {at.v2[0], at.v2[1], at.v2[2]};
+          target = w * v0 + u1 * v1 + u2 * v2;
+          ap = at.pdf;
+          if (L.emit_const) {
+            emit = f3{L.emit[0], L.emit[1], L.emit[2]};
+          } else {
+            float tu = 0.f, tv = 0.f;
+            if (at.has_uv != 0.f) {
+              tu = w * at.uv[0] + u1 * at.uv[2] + u2 * at.uv[4];
+              tv = w * at.uv[1] + u1 * at.uv[3] + u2 * at.uv[5];
+            }
+            emit = get_emission(sc, L.material, tu, tv);
+          }
+        }
+      } else {
+        target = f3{L.pos[0], L.pos[1], L.pos[2]};
+      }
+      f3 Lv = target - P;
+      float dist_sq = sqnorm(Lv);
+      float dist = sqrtf(dist_sq);
+      sdir = target - P;
+      tfar = dist - 0.001f;
+      Lv = Lv / dist;
+      float cos_n = dot(N, Lv);
+      if (type == kLightSpot) {
+        f3 ld = {L.dir[0], L.dir[1], L.dir[2]};
+        float cos_l = dot(ld, -Lv);
+        float falloff = (cos_l - L.cos_outer) / (L.cos_inner - L.cos_outer);
+        falloff = fminf(fmaxf(falloff, 0.0f), 1.0f);
+        if (!(cos_n < 0.f)) rad = f3{L.ic[0], L.ic[1], L.ic[2]} / (dist * dist) * falloff;
+      } else if (type == kLightPoint) {
+        if (!(cos_n < 0.f)) rad = f3{L.ic[0], L.ic[1], L.ic[2]} / (dist * dist);
+      } else {
+        if (!(cos_n < 0.f)) rad = emit / (dist * dist);
+      }
+      if (VERTEX && !(cos_n < 0.f)) {
+        bool ok = front && cos_n > 0.0f;
+        f3 bv = ok ? brdf : f3{0.f, 0.f, 0.f};
+        rad = cmul(rad, cos_n * bv);
+      }
+    }
+    const float* ls = sc.light_scale + 3 * l;
+    rad = cmul(rad, f3{ls[0], ls[1], ls[2]});
+    float w = maxc(rad);
+    float u = rng.uniform();
+    W += w;
+    if (w > 0.f && u * W < w) {
+      pick = l;
+      pw = w;
+      pap = ap;
+      prad = rad;
+      pdir = sdir;
+      ptfar = tfar;
+    }
+  }
+  if (W < 1e-6f || pick < 0) return false;
+  float pdf = pw / W;
+  float joint = pdf * pap;
+  if (joint < 1e-3f) return false;
+  out.L = prad / joint;
+  out.dir = pdir;
+  out.origin = P;
+  out.tfar = ptfar;
+  out.light = pick;
+  return true;
+}
+
+}  // namespace shb
+
+#endif  // SHB_BAKE_DEVICE_H_

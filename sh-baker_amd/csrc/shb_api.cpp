@@ -1,0 +1,968 @@
+// C ABI of the baker (include/shb.h): context, scene upload, raster, bake
+// orchestration, post-bake image steps, fixed-ray queries.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "shb_ctx.h"
+
+using namespace shb;
+
+namespace {
+
+#define SHB_TRY(ctx, expr)                                                   \
+  do {                                                                       \
+    hipError_t _e = (expr);                                                  \
+    if (_e != hipSuccess) {                                                  \
+      (ctx).err = std::string(#expr) + ": " + hipGetErrorString(_e);         \
+      return _e == hipErrorOutOfMemory ? SHB_ERR_OOM : SHB_ERR_DEVICE;       \
+    }                                                                        \
+  } while (0)
+
+int fail(Ctx& c, int code, const std::string& msg) {
+  c.err = msg;
+  return code;
+}
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+// ------------------------------------------------------ host float math
+// Same evaluation order as the kernels (e0 + (e1 + e2) reductions); this
+// file is compiled with -ffp-contract=off.
+struct H3 {
+  float x, y, z;
+};
+inline H3 h3sub(H3 a, H3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline float h3sq(H3 a) { return a.x * a.x + (a.y * a.y + a.z * a.z); }
+inline H3 h3norm(H3 a) {
+  float z = h3sq(a);
+  if (z > 0.f) {
+    float s = std::sqrt(z);
+    return {a.x / s, a.y / s, a.z / s};
+  }
+  return a;
+}
+inline H3 h3cross(H3 a, H3 b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+inline float rowdot(const float* r, H3 v) { return r[0] * v.x + (r[1] * v.y + r[2] * v.z); }
+
+// Unitary factor of the polar decomposition of the 3x3 part (what Eigen's
+// Transform::rotation() returns), by Newton iteration R <- (R + R^-T)/2.
+void rotation_of(const float* xf, float R[9]) {
+  double m[9] = {xf[0], xf[1], xf[2], xf[4], xf[5], xf[6], xf[8], xf[9], xf[10]};
+  for (int it = 0; it < 64; ++it) {
+    double a = m[0], b = m[1], c = m[2], d = m[3], e = m[4], f = m[5], g = m[6], h = m[7],
+           i = m[8];
+    double A = e * i - f * h, B = -(d * i - f * g), C = d * h - e * g;
+    double det = a * A + b * B + c * C;
+    if (det == 0.0) break;
+    double it9[9] = {A / det,
+                     B / det,
+                     C / det,
+                     -(b * i - c * h) / det,
+                     (a * i - c * g) / det,
+                     -(a * h - b * g) / det,
+                     (b * f - c * e) / det,
+                     -(a * f - c * d) / det,
+                     (a * e - b * d) / det};
+    double diff = 0;
+    for (int k = 0; k < 9; ++k) {
+      double nv = 0.5 * (m[k] + it9[k]);
+      diff = std::max(diff, std::fabs(nv - m[k]));
+      m[k] = nv;
+    }
+    if (diff < 1e-15) break;
+  }
+  for (int k = 0; k < 9; ++k) R[k] = (float)m[k];
+}
+
+// src/colorspace.h:10-14
+float srgb_to_linear(uint8_t x) {
+  float v = x / 255.0f;
+  if (v <= 0.04045f) return v / 12.92f;
+  return std::pow((v + 0.055f) / 1.055f, 2.4f);
+}
+
+template <class T>
+hipError_t upload(DevBuf& b, const std::vector<T>& v, hipStream_t st) {
+  size_t n = std::max<size_t>(v.size() * sizeof(T), 16);
+  hipError_t e = b.reserve(n);
+  if (e != hipSuccess) return e;
+  if (!v.empty()) return hipMemcpyAsync(b.ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st);
+  return hipSuccess;
+}
+
+int build_scene(Ctx& c, const shb_scene_desc* d) {
+  HostScene hs;
+  if (!d) return fail(c, SHB_ERR_INVALID_ARG, "scene is NULL");
+  if (d->num_vertices && !d->positions) return fail(c, SHB_ERR_INVALID_ARG, "positions is NULL");
+  if (d->num_indices && !d->indices) return fail(c, SHB_ERR_INVALID_ARG, "indices is NULL");
+  if (d->num_geometries && !d->geometries)
+    return fail(c, SHB_ERR_INVALID_ARG, "geometries is NULL");
+  if (d->num_materials && !d->materials) return fail(c, SHB_ERR_INVALID_ARG, "materials is NULL");
+  if (d->num_lights && !d->lights) return fail(c, SHB_ERR_INVALID_ARG, "lights is NULL");
+  hs.num_vertices = d->num_vertices;
+  // materials (src/scene.h:43-53)
+  for (uint32_t m = 0; m < d->num_materials; ++m) {
+    const shb_material& sm = d->materials[m];
+    MaterialDev md{};
+    md.emission = sm.emission_intensity;
+    if (sm.albedo_channels != 0) {
+      if (sm.albedo_channels < 3 || sm.albedo_channels > 4 || sm.albedo_width == 0 ||
+          sm.albedo_height == 0)
+        return fail(c, SHB_ERR_INVALID_ARG, "material " + std::to_string(m) + ": bad texture");
+      uint64_t nb = (uint64_t)sm.albedo_width * sm.albedo_height * sm.albedo_channels;
+      if (!d->texel_pool || sm.albedo_offset + nb > d->texel_pool_size)
+        return fail(c, SHB_ERR_INVALID_ARG,
+                    "material " + std::to_string(m) + ": texture outside the texel pool");
+      md.tex_w = sm.albedo_width;
+      md.tex_h = sm.albedo_height;
+      md.tex_c = sm.albedo_channels;
+      md.tex_off = sm.albedo_offset;
+      if (sm.albedo_channels == 4) {
+        const uint8_t* px = d->texel_pool + sm.albedo_offset;
+        for (uint64_t k = 3; k < nb; k += 4)
+          if (px[k] < 255) hs.has_transparency = true;
+      }
+    }
+    hs.mats.push_back(md);
+  }
+  if (d->texel_pool_size) hs.texels.assign(d->texel_pool, d->texel_pool + d->texel_pool_size);
+  // geometries -> global triangles
+  std::vector<int> geom_first_tri(d->num_geometries, 0);
+  hs.wpos.assign(3 * (size_t)d->num_vertices, 0.f);
+  hs.wnrm.assign(3 * (size_t)d->num_vertices, 0.f);
+  hs.wtan.assign(4 * (size_t)d->num_vertices, 0.f);
+  hs.luv.assign(2 * (size_t)d->num_vertices, 0.f);
+  for (uint32_t gi = 0; gi < d->num_geometries; ++gi) {
+    const shb_geometry& g = d->geometries[gi];
+    std::string gs = "geometry " + std::to_string(gi);
+    if ((uint64_t)g.vertex_offset + g.vertex_count > d->num_vertices)
+      return fail(c, SHB_ERR_INVALID_ARG, gs + ": vertices out of range");
+    if ((uint64_t)g.index_offset + g.index_count > d->num_indices)
+      return fail(c, SHB_ERR_INVALID_ARG, gs + ": indices out of range");
+    if (g.material_id < 0 || (uint32_t)g.material_id >= d->num_materials)
+      return fail(c, SHB_ERR_INVALID_ARG, gs + ": material_id out of range");
+    bool hasN = (g.flags & SHB_GEOM_HAS_NORMALS) && d->normals;
+    bool hasUV = (g.flags & SHB_GEOM_HAS_TEXTURE_UVS) && d->texture_uvs;
+    bool hasL = (g.flags & SHB_GEOM_HAS_LIGHTMAP_UVS) && d->lightmap_uvs;
+    bool hasT = (g.flags & SHB_GEOM_HAS_TANGENTS) && d->tangents;
+    if (!hasL || !hasN || !hasT) hs.raster_ready = false;
+    float R[9];
+    rotation_of(g.transform, R);
+    // world attributes (src/scene.cpp:162-195)
+    for (uint32_t k = 0; k < g.vertex_count; ++k) {
+      uint32_t vi = g.vertex_offset + k;
+      H3 p = {d->positions[3 * vi], d->positions[3 * vi + 1], d->positions[3 * vi + 2]};
+      hs.wpos[3 * vi] = rowdot(g.transform, p) + g.transform[3];
+      hs.wpos[3 * vi + 1] = rowdot(g.transform + 4, p) + g.transform[7];
+      hs.wpos[3 * vi + 2] = rowdot(g.transform + 8, p) + g.transform[11];
+      if (hasN) {
+        H3 n = {d->normals[3 * vi], d->normals[3 * vi + 1], d->normals[3 * vi + 2]};
+        H3 wn = h3norm(H3{rowdot(R, n), rowdot(R + 3, n), rowdot(R + 6, n)});
+        hs.wnrm[3 * vi] = wn.x;
+        hs.wnrm[3 * vi + 1] = wn.y;
+        hs.wnrm[3 * vi + 2] = wn.z;
+      }
+      if (hasT) {
+        H3 t = {d->tangents[4 * vi], d->tangents[4 * vi + 1], d->tangents[4 * vi + 2]};
+        H3 wt = h3norm(H3{rowdot(R, t), rowdot(R + 3, t), rowdot(R + 6, t)});
+        hs.wtan[4 * vi] = wt.x;
+        hs.wtan[4 * vi + 1] = wt.y;
+        hs.wtan[4 * vi + 2] = wt.z;
+        hs.wtan[4 * vi + 3] = d->tangents[4 * vi + 3];
+      }
+      if (hasL) {
+        hs.luv[2 * vi] = d->lightmap_uvs[2 * vi];
+        hs.luv[2 * vi + 1] = d->lightmap_uvs[2 * vi + 1];
+      }
+    }
+    uint32_t nt = g.index_count / 3;
+    geom_first_tri[gi] = (int)hs.num_tris;
+    for (uint32_t t = 0; t < nt; ++t) {
+      uint32_t li[3];
+      for (int k = 0; k < 3; ++k) {
+        li[k] = d->indices[g.index_offset + 3 * t + k];
+        if (li[k] >= g.vertex_count)
+          return fail(c, SHB_ERR_INVALID_ARG, gs + ": index out of range");
+      }
+      uint32_t gv[3] = {g.vertex_offset + li[0], g.vertex_offset + li[1], g.vertex_offset + li[2]};
+      H3 w[3];
+      for (int k = 0; k < 3; ++k) {
+        w[k] = {hs.wpos[3 * gv[k]], hs.wpos[3 * gv[k] + 1], hs.wpos[3 * gv[k] + 2]};
+        hs.world_xyz.push_back(w[k].x);
+        hs.world_xyz.push_back(w[k].y);
+        hs.world_xyz.push_back(w[k].z);
+        hs.tri_vidx.push_back(gv[k]);
+      }
+      hs.tri_mat.push_back(g.material_id);
+      // shading record (src/occlusion.cpp:63-93: object-space normals)
+      float n[9] = {0}, uv[6] = {0};
+      uint32_t flags = 0;
+      if (hasN) {
+        flags |= 1u;
+        for (int k = 0; k < 3; ++k)
+          for (int a = 0; a < 3; ++a) n[3 * k + a] = d->normals[3 * gv[k] + a];
+      } else {
+        H3 ng = h3norm(h3cross(h3sub(w[1], w[0]), h3sub(w[2], w[0])));
+        n[0] = ng.x;
+        n[1] = ng.y;
+        n[2] = ng.z;
+      }
+      if (hasUV) {
+        flags |= 2u;
+        for (int k = 0; k < 3; ++k) {
+          uv[2 * k] = d->texture_uvs[2 * gv[k]];
+          uv[2 * k + 1] = d->texture_uvs[2 * gv[k] + 1];
+        }
+      }
+      hs.shade.push_back(make_float4(n[0], n[1], n[2], uv[0]));
+      hs.shade.push_back(make_float4(n[3], n[4], n[5], uv[1]));
+      hs.shade.push_back(make_float4(n[6], n[7], n[8], uv[2]));
+      hs.shade.push_back(make_float4(uv[3], uv[4], uv[5], as_float(flags)));
+      hs.tri_info.push_back(make_int4(g.material_id, -1, (int)gi, (int)t));
+      hs.num_tris++;
+    }
+  }
+  // lights (src/scene.h:70-87, src/light.h, src/light.cpp:16-78)
+  for (uint32_t li = 0; li < d->num_lights; ++li) {
+    const shb_light& sl = d->lights[li];
+    LightDev L{};
+    L.type = sl.type;
+    if (sl.type < SHB_LIGHT_POINT || sl.type > SHB_LIGHT_AREA)
+      return fail(c, SHB_ERR_INVALID_ARG, "light " + std::to_string(li) + ": bad type");
+    for (int a = 0; a < 3; ++a) {
+      L.ic[a] = sl.intensity * sl.color[a];
+      L.pos[a] = sl.position[a];
+      L.dir[a] = sl.direction[a];
+    }
+    L.cos_inner = sl.cos_inner_cone;
+    L.cos_outer = sl.cos_outer_cone;
+    L.material = -1;
+    if (sl.type == SHB_LIGHT_AREA) {
+      int gi = sl.geometry_index, mi = sl.material_index;
+      if (gi >= 0 && (uint32_t)gi < d->num_geometries && mi >= 0 &&
+          (uint32_t)mi < d->num_materials && d->geometries[gi].index_count >= 3) {
+        const shb_geometry& g = d->geometries[gi];
+        uint32_t nt = g.index_count / 3;
+        L.first_tri = (uint32_t)hs.area_tris.size();
+        L.num_tris = nt;
+        L.material = mi;
+        bool hasUV = (g.flags & SHB_GEOM_HAS_TEXTURE_UVS) && d->texture_uvs;
+        for (uint32_t t = 0; t < nt; ++t) {
+          AreaTri at{};
+          H3 v[3];
+          for (int k = 0; k < 3; ++k) {
+            uint32_t vi = g.vertex_offset + d->indices[g.index_offset + 3 * t + k];
+            v[k] = {d->positions[3 * vi], d->positions[3 * vi + 1], d->positions[3 * vi + 2]};
+            if (hasUV) {
+              at.uv[2 * k] = d->texture_uvs[2 * vi];
+              at.uv[2 * k + 1] = d->texture_uvs[2 * vi + 1];
+            }
+          }
+          at.v0[0] = v[0].x, at.v0[1] = v[0].y, at.v0[2] = v[0].z;
+          at.v1[0] = v[1].x, at.v1[1] = v[1].y, at.v1[2] = v[1].z;
+          at.v2[0] = v[2].x, at.v2[1] = v[2].y, at.v2[2] = v[2].z;
+          float area = std::sqrt(h3sq(h3cross(h3sub(v[0], v[1]), h3sub(v[0], v[2])))) / 2.f;
+          at.pdf = std::max(1e-6f, 1.f / (float)nt * 1.f / area);
+          at.has_uv = hasUV ? 1.f : 0.f;
+          hs.area_tris.push_back(at);
+        }
+        const MaterialDev& md = hs.mats[mi];
+        if (md.tex_c == 0 || (md.tex_w == 1 && md.tex_h == 1)) {
+          float alb[3] = {0.8f, 0.8f, 0.8f};
+          if (md.tex_c != 0)
+            for (int a = 0; a < 3; ++a) alb[a] = srgb_to_linear(hs.texels[md.tex_off + a]);
+          L.emit_const = 1;
+          for (int a = 0; a < 3; ++a)
+            L.emit[a] = md.emission > 0.0f ? alb[a] * md.emission : 0.f;
+        }
+        // triangles of the emitting geometry carry the light for the
+        // emission they show when hit directly
+        for (uint32_t t = 0; t < nt; ++t) hs.tri_info[geom_first_tri[gi] + t].y = (int)li;
+      }
+    }
+    hs.lights.push_back(L);
+  }
+  c.hs = std::move(hs);
+  return SHB_OK;
+}
+
+int upload_bvh(Ctx& c) {
+  build_cwbvh(c.hs.world_xyz.data(), c.hs.num_tris, c.tuning.bvh_width, &c.bvh);
+  if (c.bvh.max_depth - 1 > kStackTotal)
+    return fail(c, SHB_ERR_LIMIT,
+                "BVH depth " + std::to_string(c.bvh.max_depth) + " exceeds the traversal stack");
+  SHB_TRY(c, upload(c.d_nodes, c.bvh.nodes, c.stream));
+  SHB_TRY(c, upload(c.d_tris, c.bvh.tris, c.stream));
+  return SHB_OK;
+}
+
+int spill_cap(const Ctx& c) { return std::max(0, c.bvh.max_depth - 1 - kStackLds); }
+
+int ensure_trace_occupancy(Ctx& c) {
+  if (c.trace_bpc_block != c.tuning.block_size) {
+    c.trace_bpc[0] = trace_blocks_per_cu(false, c.tuning.block_size);
+    c.trace_bpc[1] = trace_blocks_per_cu(true, c.tuning.block_size);
+    c.trace_bpc_block = c.tuning.block_size;
+  }
+  return SHB_OK;
+}
+
+// Grid of the persistent traversal kernels and its stack spill area.
+int trace_grid(Ctx& c, bool any, int* grid) {
+  ensure_trace_occupancy(c);
+  int g = c.num_cus * std::max(c.trace_bpc[0], c.trace_bpc[1]);
+  *grid = std::max(1, g);
+  (void)any;
+  int cap = spill_cap(c);
+  if (cap > 0) {
+    size_t need = (size_t)(*grid) * c.tuning.block_size * cap * sizeof(uint2);
+    SHB_TRY(c, c.d_spill.reserve(need));
+  }
+  return SHB_OK;
+}
+
+TraceArgs trace_args(Ctx& c) {
+  TraceArgs a{};
+  a.scene = c.view();
+  a.spill = spill_cap(c) > 0 ? c.d_spill.as<uint2>() : nullptr;
+  a.spill_cap = spill_cap(c);
+  a.refill = c.tuning.refill_threshold;
+  a.iters = c.iters;
+  a.has_nodes = c.bvh.nodes.empty() ? 0 : 1;
+  a.error = c.d_work.as<uint32_t>();
+  return a;
+}
+
+int check_overflow(Ctx& c) {
+  uint32_t e = 0;
+  SHB_TRY(c, hipMemcpyAsync(&e, c.d_work.ptr, 4, hipMemcpyDeviceToHost, c.stream));
+  SHB_TRY(c, hipStreamSynchronize(c.stream));
+  if (e) return fail(c, SHB_ERR_LIMIT, "traversal stack overflow");
+  return SHB_OK;
+}
+
+// Counter layout per batch (uint32): extension-queue counts by depth, shadow
+// counts by depth, work counters of the extension and shadow traversals.
+constexpr int kCntExt = 0, kCntSh = 20, kCntWExt = 40, kCntWSh = 60, kCntStride = 80;
+
+struct BakeRun {
+  uint64_t rays_ext = 0, rays_sh = 0, valid = 0, node_visits = 0, tri_tests = 0;
+  double dev_s = 0, trace_s = 0;
+};
+
+// The bake into c.d_sh / c.d_vis (supersampled), with per-batch kernels:
+// generate, then per depth: extend, shade, shadow; then project.
+int bake_device(Ctx& c, const shb_bake_cfg* cfg, uint32_t job, BakeRun* run) {
+  if (!c.has_scene) return fail(c, SHB_ERR_NO_SCENE, "no scene uploaded");
+  if (!c.has_surface) return fail(c, SHB_ERR_NO_SURFACE, "no surface: call shb_rasterize or shb_set_surface");
+  if (!cfg) return fail(c, SHB_ERR_INVALID_ARG, "cfg is NULL");
+  if (cfg->samples < 1 || cfg->samples > kMaxSamples)
+    return fail(c, SHB_ERR_INVALID_ARG, "samples must be in [1, 65535]");
+  if (cfg->bounces < 0 || cfg->bounces > kMaxBounces)
+    return fail(c, SHB_ERR_INVALID_ARG, "bounces must be in [0, 15]");
+  if (cfg->num_light_samples != 1)
+    return fail(c, SHB_ERR_UNSUPPORTED, "num_light_samples must be 1");
+  if (c.hs.has_transparency)
+    return fail(c, SHB_ERR_UNSUPPORTED,
+                "materials with alpha < 1 (path splitting) are not supported by the GPU bake");
+  const uint32_t W = (uint32_t)(c.width * c.ss), H = (uint32_t)(c.height * c.ss);
+  const size_t npx = (size_t)W * H;
+  const uint32_t spp = (uint32_t)cfg->samples, B = (uint32_t)cfg->bounces;
+  hipStream_t st = c.stream;
+  hipEvent_t e0, e1;
+  SHB_TRY(c, hipEventCreate(&e0));
+  SHB_TRY(c, hipEventCreate(&e1));
+  SHB_TRY(c, hipEventRecord(e0, st));
+  // outputs
+  SHB_TRY(c, c.d_sh.reserve(std::max<size_t>(npx * 27 * sizeof(float), 16)));
+  SHB_TRY(c, c.d_vis.reserve(std::max<size_t>(npx * sizeof(float), 16)));
+  SHB_TRY(c, launch_fill_f32(c.d_sh.as<float>(), 0.f, npx * 27, st));
+  SHB_TRY(c, launch_fill_f32(c.d_vis.as<float>(), 0.f, npx, st));
+  // valid texel list
+  SHB_TRY(c, c.d_mask.reserve(std::max<size_t>(npx, 16)));
+  SHB_TRY(c, c.d_list.reserve(std::max<size_t>(npx * 4 + 16, 64)));
+  size_t tmpb = compact_tmp_bytes((uint32_t)npx);
+  SHB_TRY(c, c.d_tmp.reserve(std::max<size_t>(tmpb, 16)));
+  SHB_TRY(c, launch_validity_mask(c.d_surface.as<float4>(), npx, c.d_mask.as<uint8_t>(), st));
+  uint32_t* d_nvalid = c.d_list.as<uint32_t>() + npx;
+  SHB_TRY(c, launch_compact_valid(c.d_mask.as<uint8_t>(), (uint32_t)npx, c.d_list.as<uint32_t>(),
+                                  d_nvalid, c.d_tmp.ptr, tmpb, st));
+  uint32_t V = 0;
+  SHB_TRY(c, hipMemcpyAsync(&V, d_nvalid, 4, hipMemcpyDeviceToHost, st));
+  SHB_TRY(c, hipStreamSynchronize(st));
+  run->valid = V;
+  const bool count = (cfg->flags & SHB_BAKE_COUNT_VISITS) != 0;
+  // batches
+  uint32_t T = (uint32_t)std::max<int64_t>(1, (int64_t)c.tuning.batch_paths / spp);
+  T = std::min<uint32_t>(T, std::max<uint32_t>(V, 1));
+  const uint32_t P = T * spp;
+  const uint32_t nb = V == 0 ? 0 : (V + T - 1) / T;
+  // path state: 13 float4 per slot; queues 4 uint32 per slot
+  SHB_TRY(c, c.d_paths.reserve((size_t)P * 13 * sizeof(float4)));
+  SHB_TRY(c, c.d_queues.reserve((size_t)P * 4 * sizeof(uint32_t)));
+  SHB_TRY(c, c.d_counters.reserve(std::max<size_t>((size_t)nb * kCntStride * 4, 64)));
+  SHB_TRY(c, c.d_work.reserve(64));
+  SHB_TRY(c, launch_fill_u32(c.d_counters.as<uint32_t>(), 0u, (size_t)nb * kCntStride, st));
+  SHB_TRY(c, launch_fill_u32(c.d_work.as<uint32_t>(), 0u, 16, st));
+  int grid = 1;
+  int rc = trace_grid(c, false, &grid);
+  if (rc) return rc;
+  float4* base = c.d_paths.as<float4>();
+  BakeArgs b{};
+  b.scene = c.view();
+  b.surface = c.d_surface.as<float4>();
+  b.texel_list = c.d_list.as<uint32_t>();
+  b.spp = spp;
+  b.bounces = B;
+  b.P = P;
+  b.job = job;
+  b.seed = cfg->seed;
+  b.ray = base;
+  b.hit = base + 2 * (size_t)P;
+  b.thr = base + 3 * (size_t)P;
+  b.li = base + 4 * (size_t)P;
+  b.nee0 = base + 5 * (size_t)P;
+  b.nee0_dir = base + 6 * (size_t)P;
+  b.dir0 = base + 7 * (size_t)P;
+  b.sray = base + 8 * (size_t)P;
+  b.scontrib = base + 12 * (size_t)P;
+  uint32_t* q = c.d_queues.as<uint32_t>();
+  uint32_t* qext[2] = {q, q + P};
+  b.shadow_queue = q + 2 * (size_t)P;
+  b.sh_out = c.d_sh.as<float>();
+  b.vis_out = c.d_vis.as<float>();
+  unsigned long long* totals = reinterpret_cast<unsigned long long*>(c.d_work.as<uint32_t>() + 2);
+  std::vector<hipEvent_t> tev;
+  auto trace_ev = [&]() -> hipEvent_t {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    tev.push_back(e);
+    (void)hipEventRecord(e, st);
+    return e;
+  };
+  const int sgrid = std::max(1, std::min<int>((P + 255) / 256, c.num_cus * 16));
+  for (uint32_t bi = 0; bi < nb; ++bi) {
+    uint32_t* cnt = c.d_counters.as<uint32_t>() + (size_t)bi * kCntStride;
+    b.texel_begin = bi * T;
+    b.texels_in_batch = std::min(T, V - bi * T);
+    b.shadow_count = cnt + kCntSh;
+    SHB_TRY(c, launch_generate(b, sgrid, st));
+    for (uint32_t d = 0; d <= B; ++d) {
+      TraceArgs ta = trace_args(c);
+      ta.rays = b.ray;
+      ta.queue = d == 0 ? nullptr : qext[d & 1];
+      ta.count_ptr = d == 0 ? nullptr : cnt + kCntExt + d;
+      ta.count_const = b.texels_in_batch * spp;
+      ta.work = cnt + kCntWExt + d;
+      ta.totals = count ? totals : nullptr;
+      trace_ev();
+      SHB_TRY(c, launch_trace_hits(ta, b.hit, count, grid, c.tuning.block_size, st));
+      trace_ev();
+      b.shadow_count = cnt + kCntSh + d;
+      SHB_TRY(c, launch_shade(b, (int)d, d == 0 ? nullptr : qext[d & 1], cnt + kCntExt + d,
+                              qext[(d + 1) & 1], cnt + kCntExt + d + 1, sgrid, st));
+      TraceArgs sa = trace_args(c);
+      sa.rays = b.sray;
+      sa.queue = b.shadow_queue;
+      sa.count_ptr = cnt + kCntSh + d;
+      sa.work = cnt + kCntWSh + d;
+      sa.totals = count ? totals : nullptr;
+      trace_ev();
+      SHB_TRY(c, launch_trace_bake_shadow(sa, b, count, grid, c.tuning.block_size, st));
+      trace_ev();
+    }
+    SHB_TRY(c, launch_project(b, st));
+  }
+  SHB_TRY(c, hipEventRecord(e1, st));
+  SHB_TRY(c, hipEventSynchronize(e1));
+  float ms = 0.f;
+  SHB_TRY(c, hipEventElapsedTime(&ms, e0, e1));
+  run->dev_s = ms * 1e-3;
+  double tr = 0;
+  for (size_t i = 0; i + 1 < tev.size(); i += 2) {
+    float m = 0.f;
+    if (tev[i] && tev[i + 1] && hipEventElapsedTime(&m, tev[i], tev[i + 1]) == hipSuccess) tr += m;
+  }
+  for (hipEvent_t e : tev)
+    if (e) (void)hipEventDestroy(e);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  run->trace_s = tr * 1e-3;
+  // ray counts
+  std::vector<uint32_t> hc((size_t)nb * kCntStride);
+  if (nb) {
+    SHB_TRY(c, hipMemcpy(hc.data(), c.d_counters.ptr, hc.size() * 4, hipMemcpyDeviceToHost));
+  }
+  for (uint32_t bi = 0; bi < nb; ++bi) {
+    const uint32_t* k = hc.data() + (size_t)bi * kCntStride;
+    uint32_t tb = std::min(T, V - bi * T);
+    run->rays_ext += (uint64_t)tb * spp;
+    for (uint32_t d = 1; d <= B; ++d) run->rays_ext += k[kCntExt + d];
+    for (uint32_t d = 0; d <= B; ++d) run->rays_sh += k[kCntSh + d];
+  }
+  uint32_t hw[6] = {0};
+  SHB_TRY(c, hipMemcpy(hw, c.d_work.ptr, sizeof(hw), hipMemcpyDeviceToHost));
+  if (hw[0]) return fail(c, SHB_ERR_LIMIT, "traversal stack overflow");
+  if (count) {
+    unsigned long long t2[2];
+    std::memcpy(t2, hw + 2, sizeof(t2));
+    run->node_visits = t2[0];
+    run->tri_tests = t2[1];
+  }
+  return SHB_OK;
+}
+
+void fill_stats(shb_bake_stats* s, const BakeRun& r, uint32_t spp, double t0) {
+  if (!s) return;
+  s->valid_texels = r.valid;
+  s->paths = r.valid * spp;
+  s->rays_extension = r.rays_ext;
+  s->rays_shadow = r.rays_sh;
+  s->node_visits = r.node_visits;
+  s->tri_tests = r.tri_tests;
+  s->seconds_total = now_s() - t0;
+  s->seconds_device = r.dev_s;
+  s->seconds_trace = r.trace_s;
+}
+
+}  // namespace
+
+// =====================================================================
+extern "C" {
+
+const char* shb_version(void) { return "shb-hip 0.1 (gfx950)"; }
+
+int shb_create(shb_ctx** out, int device) {
+  if (!out) return SHB_ERR_INVALID_ARG;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return SHB_ERR_DEVICE;
+  if (device < 0 || device >= n) return SHB_ERR_INVALID_ARG;
+  if (hipSetDevice(device) != hipSuccess) return SHB_ERR_DEVICE;
+  shb_ctx* h = new (std::nothrow) shb_ctx();
+  if (!h) return SHB_ERR_OOM;
+  Ctx& c = h->c;
+  c.device = device;
+  if (hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking) != hipSuccess) {
+    delete h;
+    return SHB_ERR_DEVICE;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) {
+    (void)hipStreamDestroy(c.stream);
+    delete h;
+    return SHB_ERR_DEVICE;
+  }
+  c.num_cus = prop.multiProcessorCount;
+  // default parameters: all scales 1 (reproduces the reference bake)
+  *out = h;
+  return SHB_OK;
+}
+
+void shb_destroy(shb_ctx* h) {
+  if (!h) return;
+  Ctx& c = h->c;
+  (void)hipSetDevice(c.device);
+  if (c.stream) (void)hipStreamSynchronize(c.stream);
+  DevBuf* bufs[] = {&c.d_nodes,  &c.d_tris,     &c.d_shade,   &c.d_tri_info, &c.d_mats,
+                    &c.d_texels, &c.d_lut,      &c.d_lights,  &c.d_area,     &c.d_lscale,
+                    &c.d_ascale, &c.d_luv,      &c.d_wpos,    &c.d_wnrm,     &c.d_wtan,
+                    &c.d_tri_vidx, &c.d_tri_mat, &c.d_surface, &c.d_paths,   &c.d_queues,
+                    &c.d_counters, &c.d_spill,  &c.d_work,    &c.d_list,     &c.d_mask,
+                    &c.d_tmp,    &c.d_sh,       &c.d_vis,     &c.d_img_a,    &c.d_img_b,
+                    &c.d_mask_a, &c.d_mask_b,   &c.d_rays,    &c.d_hits,     &c.d_hit_out,
+                    &c.d_raycounts, &c.d_adj};
+  for (DevBuf* b : bufs) b->release();
+  if (c.stream) (void)hipStreamDestroy(c.stream);
+  delete h;
+}
+
+const char* shb_last_error(const shb_ctx* h) { return h ? h->c.err.c_str() : "ctx is NULL"; }
+
+int shb_upload_scene(shb_ctx* h, const shb_scene_desc* scene) {
+  if (!h) return SHB_ERR_INVALID_ARG;
+  Ctx& c = h->c;
+  c.err.clear();
+  SHB_TRY(c, hipSetDevice(c.device));
+  int rc = build_scene(c, scene);
+  if (rc) return rc;
+  c.has_surface = false;
+  rc = upload_bvh(c);
+  if (rc) return rc;
+  std::vector<float> lut(256);
+  for (int i = 0; i < 256; ++i) lut[i] = srgb_to_linear((uint8_t)i);
+  HostScene& hs = c.hs;
+  SHB_TRY(c, upload(c.d_shade, hs.shade, c.stream));
+  SHB_TRY(c, upload(c.d_tri_info, hs.tri_info, c.stream));
+  SHB_TRY(c, upload(c.d_mats, hs.mats, c.stream));
+  SHB_TRY(c, upload(c.d_texels, hs.texels, c.stream));
+  SHB_TRY(c, upload(c.d_lut, lut, c.stream));
+  SHB_TRY(c, upload(c.d_lights, hs.lights, c.stream));
+  SHB_TRY(c, upload(c.d_area, hs.area_tris, c.stream));
+  std::vector<float> ones_l(3 * std::max<size_t>(hs.lights.size(), 1), 1.f);
+  std::vector<float> ones_m(3 * std::max<size_t>(hs.mats.size(), 1), 1.f);
+  SHB_TRY(c, upload(c.d_lscale, ones_l, c.stream));
+  SHB_TRY(c, upload(c.d_ascale, ones_m, c.stream));
+  SHB_TRY(c, upload(c.d_luv, hs.luv, c.stream));
+  SHB_TRY(c, upload(c.d_wpos, hs.wpos, c.stream));
+  SHB_TRY(c, upload(c.d_wnrm, hs.wnrm, c.stream));
+  SHB_TRY(c, upload(c.d_wtan, hs.wtan, c.stream));
+  SHB_TRY(c, upload(c.d_tri_vidx, hs.tri_vidx, c.stream));
+  SHB_TRY(c, upload(c.d_tri_mat, hs.tri_mat, c.stream));
+  SHB_TRY(c, hipStreamSynchronize(c.stream));
+  c.has_scene = true;
+  return SHB_OK;
+}
+
+int shb_set_params(shb_ctx* h, const float* light_scale, const float* albedo_scale) {
+  if (!h) return SHB_ERR_INVALID_ARG;
+  Ctx& c = h->c;
+  if (!c.has_scene) return fail(c, SHB_ERR_NO_SCENE, "no scene uploaded");
+  SHB_TRY(c, hipSetDevice(c.device));
+  if (light_scale && !c.hs.lights.empty())
+    SHB_TRY(c, hipMemcpyAsync(c.d_lscale.ptr, light_scale, 12 * c.hs.lights.size(),
+                              hipMemcpyHostToDevice, c.stream));
+  if (albedo_scale && !c.hs.mats.empty())
+    SHB_TRY(c, hipMemcpyAsync(c.d_ascale.ptr, albedo_scale, 12 * c.hs.mats.size(),
+                              hipMemcpyHostToDevice, c.stream));
+  SHB_TRY(c, hipStreamSynchronize(c.stream));
+  return SHB_OK;
+}
+
+static int check_raster_cfg(Ctx& c, const shb_raster_cfg* cfg) {
+  if (!cfg) return fail(c, SHB_ERR_INVALID_ARG, "raster cfg is NULL");
+  if (cfg->width <= 0 || cfg->height <= 0 || cfg->supersample_scale <= 0)
+    return fail(c, SHB_ERR_INVALID_ARG, "width, height and supersample_scale must be > 0");
+  uint64_t n = (uint64_t)cfg->width * cfg->supersample_scale * cfg->height * cfg->supersample_scale;
+  if (n > (1ull << 31)) return fail(c, SHB_ERR_LIMIT, "lightmap larger than 2^31 texels");
+  return SHB_OK;
+}
+
+int shb_rasterize(shb_ctx* h, const shb_raster_cfg* cfg, shb_surface_point* out) {
+  if (!h) return SHB_ERR_INVALID_ARG;
+  Ctx& c = h->c;
+  c.err.clear();
+  if (!c.has_scene) return fail(c, SHB_ERR_NO_SCENE, "no scene uploaded");
+  int rc = check_raster_cfg(c, cfg);
+  if (rc) return rc;
+  if (!c.hs.raster_ready)
+    return fail(c, SHB_ERR_INVALID_ARG,
+                "every geometry needs lightmap uvs, normals and tangents to rasterize "
+                "(src/rasterizer.cpp:86-88)");
+  SHB_TRY(c, hipSetDevice(c.device));
+  const int W = cfg->width * cfg->supersample_scale, H = cfg->height * cfg->supersample_scale;
+  const size_t npx = (size_t)W * H;
+  const uint32_t n = c.hs.num_tris;
+  SHB_TRY(c, c.d_surface.reserve(npx * 3 * sizeof(float4)));
+  SHB_TRY(c, c.d_img_a.reserve(npx * 4 * 2 + 16));  // win + fb
+  SHB_TRY(c, c.d_list.reserve(std::max<size_t>((size_t)(n + 1) * 8, 64)));  // tri_hit + counts
+  size_t cub = raster_scan_tmp_bytes(n);
+  SHB_TRY(c, c.d_tmp.reserve(((cub + 255) & ~size_t(255)) + (size_t)(n + 1) * 16 + 256));
+  RasterArgs a{};
+  a.luv = c.d_luv.as<float>();
+  a.wpos = c.d_wpos.as<float>();
+  a.wnrm = c.d_wnrm.as<float>();
+  a.wtan = c.d_wtan.as<float>();
+  a.tri_vidx = c.d_tri_vidx.as<uint32_t>();
+  a.tri_mat = c.d_tri_mat.as<int32_t>();
+  a.num_tris = n;
+  a.W = W;
+  a.H = H;
+  a.win = c.d_img_a.as<uint32_t>();
+  a.fb = a.win + npx;
+  a.tri_hit = c.d_list.as<uint32_t>();
+  a.surface = c.d_surface.as<float4>();
+  uint32_t* counts = a.tri_hit + (n + 1);
+  SHB_TRY(c, launch_raster(a, counts, c.d_tmp.ptr, c.d_tmp.bytes, c.stream));
+  c.width = cfg->width;
+  c.height = cfg->height;
+  c.ss = cfg->supersample_scale;
+  c.has_surface = true;
+  if (out) {
+    SHB_TRY(c, hipMemcpyAsync(out, c.d_surface.ptr, npx * sizeof(shb_surface_point),
+                              hipMemcpyDeviceToHost, c.stream));
+  }
+  SHB_TRY(c, hipStreamSynchronize(c.stream));
+  return SHB_OK;
+}
+
+int shb_set_surface(shb_ctx* h, const shb_raster_cfg* cfg, const shb_surface_point* pts) {
+  if (!h) return SHB_ERR_INVALID_ARG;
+  Ctx& c = h->c;
+  c.err.clear();
+  int rc = check_raster_cfg(c, cfg);
+  if (rc) return rc;
+  if (!pts) return fail(c, SHB_ERR_INVALID_ARG, "points is NULL");
+  SHB_TRY(c, hipSetDevice(c.device));
+  size_t npx = (size_t)cfg->width * cfg->supersample_scale * cfg->height * cfg->supersample_scale;
+  for (size_t i = 0; i < npx; ++i) {
+    int m = pts[i].material_id;
+    if (m >= 0 && (!c.has_scene || (uint32_t)m >= c.hs.mats.size()))
+      return fail(c, SHB_ERR_INVALID_ARG,
+                  "surface point " + std::to_string(i) + ": material_id out of range");
+  }
+  SHB_TRY(c, c.d_surface.reserve(npx * sizeof(shb_surface_point)));
+  SHB_TRY(c, hipMemcpyAsync(c.d_surface.ptr, pts, npx * sizeof(shb_surface_point),
+                            hipMemcpyHostToDevice, c.stream));
+  SHB_TRY(c, hipStreamSynchronize(c.stream));
+  c.width = cfg->width;
+  c.height = cfg->height;
+  c.ss = cfg->supersample_scale;
+  c.has_surface = true;
+  return SHB_OK;
+}
+
+int shb_validity_mask(shb_ctx* h, uint8_t* mask_out) {
+  if (!h) return SHB_ERR_INVALID_ARG;
+  Ctx& c = h->c;
+  if (!c.has_surface) return fail(c, SHB_ERR_NO_SURFACE, "no surface");
+  if (!mask_out) return fail(c, SHB_ERR_INVALID_ARG, "mask_out is NULL");
+  SHB_TRY(c, hipSetDevice(c.device));
+  size_t npx = (size_t)c.width * c.ss * c.height * c.ss;
+  SHB_TRY(c, c.d_mask.reserve(std::max<size_t>(npx, 16)));
+  SHB_TRY(c, launch_validity_mask(c.d_surface.as<float4>(), npx, c.d_mask.as<uint8_t>(), c.stream));
+  SHB_TRY(c, hipMemcpyAsync(mask_out, c.d_mask.ptr, npx, hipMemcpyDeviceToHost, c.stream));
+  SHB_TRY(c, hipStreamSynchronize(c.stream));
+  return SHB_OK;
+}
+
+int shb_bake(shb_ctx* h, const shb_bake_cfg* cfg, float* sh_out, float* vis_out,
+             shb_bake_stats* stats) {
+  if (!h) return SHB_ERR_INVALID_ARG;
+  Ctx& c = h->c;
+  c.err.clear();
+  double t0 = now_s();
+  SHB_TRY(c, hipSetDevice(c.device));
+  BakeRun run;
+  int rc = bake_device(c, cfg, 0, &run);
+  if (rc) return rc;
+  size_t npx = (size_t)c.width * c.ss * c.height * c.ss;
+  if (sh_out)
+    SHB_TRY(c, hipMemcpyAsync(sh_out, c.d_sh.ptr, npx * 27 * 4, hipMemcpyDeviceToHost, c.stream));
+  if (vis_out)
+    SHB_TRY(c, hipMemcpyAsync(vis_out, c.d_vis.ptr, npx * 4, hipMemcpyDeviceToHost, c.stream));
+  SHB_TRY(c, hipStreamSynchronize(c.stream));
+  fill_stats(stats, run, (uint32_t)cfg->samples, t0);
+  return SHB_OK;
+}
+
+int shb_bake_lightmap(shb_ctx* h, const shb_bake_cfg* cfg, int32_t dilation, float* sh_out,
+                      float* vis_out, uint8_t* mask_out, shb_bake_stats* stats) {
+  if (!h) return SHB_ERR_INVALID_ARG;
+  Ctx& c = h->c;
+  c.err.clear();
+  if (dilation < 0) return fail(c, SHB_ERR_INVALID_ARG, "dilation must be >= 0");
+  double t0 = now_s();
+  SHB_TRY(c, hipSetDevice(c.device));
+  BakeRun run;
+  int rc = bake_device(c, cfg, 0, &run);
+  if (rc) return rc;
+  hipStream_t st = c.stream;
+  const int w = c.width, hgt = c.height, ss = c.ss;
+  const size_t n = (size_t)w * hgt, npx = n * ss * ss;
+  SHB_TRY(c, c.d_img_a.reserve(n * 28 * 4 + 16));
+  SHB_TRY(c, c.d_img_b.reserve(n * 28 * 4 + 16));
+  SHB_TRY(c, c.d_mask_a.reserve(n + 16));
+  SHB_TRY(c, c.d_mask_b.reserve(n + 16));
+  float* sa = c.d_img_a.as<float>();
+  float* sb = c.d_img_b.as<float>();
+  float* va = sa + n * 27;
+  float* vb = sb + n * 27;
+  uint8_t* ma = c.d_mask_a.as<uint8_t>();
+  uint8_t* mb = c.d_mask_b.as<uint8_t>();
+  // main.cpp:138-152: mask, then downsample SH / visibility / mask
+  SHB_TRY(c, launch_validity_mask(c.d_surface.as<float4>(), npx, c.d_mask.as<uint8_t>(), st));
+  if (ss > 1) {
+    SHB_TRY(c, launch_downsample(c.d_sh.as<float>(), w, hgt, ss, 27, sa, st));
+    SHB_TRY(c, launch_downsample(c.d_vis.as<float>(), w, hgt, ss, 1, va, st));
+    SHB_TRY(c, launch_downsample_mask(c.d_mask.as<uint8_t>(), w, hgt, ss, ma, st));
+  } else {
+    SHB_TRY(c, hipMemcpyAsync(sa, c.d_sh.ptr, n * 27 * 4, hipMemcpyDeviceToDevice, st));
+    SHB_TRY(c, hipMemcpyAsync(va, c.d_vis.ptr, n * 4, hipMemcpyDeviceToDevice, st));
+    SHB_TRY(c, hipMemcpyAsync(ma, c.d_mask.ptr, n, hipMemcpyDeviceToDevice, st));
+  }
+  // main.cpp:155-163: SH dilation grows the mask; the visibility dilation
+  // starts from that grown mask.
+  float* sh_res = sa;
+  float* vis_res = va;
+  uint8_t* m_res = ma;
+  if (dilation > 0) {
+    bool in_a = true;
+    SHB_TRY(c, launch_dilate(sa, sb, ma, mb, w, hgt, 27, dilation, nullptr, &in_a, st));
+    sh_res = in_a ? sa : sb;
+    uint8_t* m_sh = in_a ? ma : mb;
+    uint8_t* m_other = in_a ? mb : ma;
+    float* v_src = va;
+    float* v_other = vb;
+    bool vin = true;
+    SHB_TRY(c, launch_dilate(v_src, v_other, m_sh, m_other, w, hgt, 1, dilation, nullptr, &vin, st));
+    vis_res = vin ? v_src : v_other;
+    m_res = vin ? m_sh : m_other;
+  }
+  if (sh_out) SHB_TRY(c, hipMemcpyAsync(sh_out, sh_res, n * 27 * 4, hipMemcpyDeviceToHost, st));
+  if (vis_out) SHB_TRY(c, hipMemcpyAsync(vis_out, vis_res, n * 4, hipMemcpyDeviceToHost, st));
+  if (mask_out) SHB_TRY(c, hipMemcpyAsync(mask_out, m_res, n, hipMemcpyDeviceToHost, st));
+  SHB_TRY(c, hipStreamSynchronize(st));
+  fill_stats(stats, run, (uint32_t)cfg->samples, t0);
+  return SHB_OK;
+}
+
+int shb_downsample(shb_ctx* h, int32_t width, int32_t height, int32_t scale, int32_t channels,
+                   const float* in, float* out) {
+  if (!h) return SHB_ERR_INVALID_ARG;
+  Ctx& c = h->c;
+  if (width <= 0 || height <= 0 || scale <= 0 || channels <= 0 || !in || !out)
+    return fail(c, SHB_ERR_INVALID_ARG, "bad downsample arguments");
+  SHB_TRY(c, hipSetDevice(c.device));
+  size_t n_out = (size_t)width * height * channels, n_in = n_out * scale * scale;
+  SHB_TRY(c, c.d_img_a.reserve(n_in * 4));
+  SHB_TRY(c, c.d_img_b.reserve(n_out * 4));
+  SHB_TRY(c, hipMemcpyAsync(c.d_img_a.ptr, in, n_in * 4, hipMemcpyHostToDevice, c.stream));
+  SHB_TRY(c, launch_downsample(c.d_img_a.as<float>(), width, height, scale, channels,
+                               c.d_img_b.as<float>(), c.stream));
+  SHB_TRY(c, hipMemcpyAsync(out, c.d_img_b.ptr, n_out * 4, hipMemcpyDeviceToHost, c.stream));
+  SHB_TRY(c, hipStreamSynchronize(c.stream));
+  return SHB_OK;
+}
+
+int shb_downsample_mask(shb_ctx* h, int32_t width, int32_t height, int32_t scale,
+                        const uint8_t* in, uint8_t* out) {
+  if (!h) return SHB_ERR_INVALID_ARG;
+  Ctx& c = h->c;
+  if (width <= 0 || height <= 0 || scale <= 0 || !in || !out)
+    return fail(c, SHB_ERR_INVALID_ARG, "bad downsample_mask arguments");
+  SHB_TRY(c, hipSetDevice(c.device));
+  size_t n_out = (size_t)width * height, n_in = n_out * scale * scale;
+  SHB_TRY(c, c.d_mask_a.reserve(n_in));
+  SHB_TRY(c, c.d_mask_b.reserve(n_out));
+  SHB_TRY(c, hipMemcpyAsync(c.d_mask_a.ptr, in, n_in, hipMemcpyHostToDevice, c.stream));
+  SHB_TRY(c, launch_downsample_mask(c.d_mask_a.as<uint8_t>(), width, height, scale,
+                                    c.d_mask_b.as<uint8_t>(), c.stream));
+  SHB_TRY(c, hipMemcpyAsync(out, c.d_mask_b.ptr, n_out, hipMemcpyDeviceToHost, c.stream));
+  SHB_TRY(c, hipStreamSynchronize(c.stream));
+  return SHB_OK;
+}
+
+int shb_dilate(shb_ctx* h, int32_t width, int32_t height, int32_t channels, float* data,
+               uint8_t* mask, int32_t passes) {
+  if (!h) return SHB_ERR_INVALID_ARG;
+  Ctx& c = h->c;
+  if (width < 0 || height < 0 || channels <= 0 || passes < 0)
+    return fail(c, SHB_ERR_INVALID_ARG, "bad dilate arguments");
+  size_t n = (size_t)width * height;
+  if (n == 0 || passes == 0) return SHB_OK;  // dilation.cpp:11 / empty loop
+  if (!data || !mask) return fail(c, SHB_ERR_INVALID_ARG, "data or mask is NULL");
+  SHB_TRY(c, hipSetDevice(c.device));
+  SHB_TRY(c, c.d_img_a.reserve(n * channels * 4));
+  SHB_TRY(c, c.d_img_b.reserve(n * channels * 4));
+  SHB_TRY(c, c.d_mask_a.reserve(n));
+  SHB_TRY(c, c.d_mask_b.reserve(n));
+  SHB_TRY(c, hipMemcpyAsync(c.d_img_a.ptr, data, n * channels * 4, hipMemcpyHostToDevice, c.stream));
+  SHB_TRY(c, hipMemcpyAsync(c.d_mask_a.ptr, mask, n, hipMemcpyHostToDevice, c.stream));
+  bool in_a = true;
+  SHB_TRY(c, launch_dilate(c.d_img_a.as<float>(), c.d_img_b.as<float>(), c.d_mask_a.as<uint8_t>(),
+                           c.d_mask_b.as<uint8_t>(), width, height, channels, passes, nullptr,
+                           &in_a, c.stream));
+  SHB_TRY(c, hipMemcpyAsync(data, in_a ? c.d_img_a.ptr : c.d_img_b.ptr, n * channels * 4,
+                            hipMemcpyDeviceToHost, c.stream));
+  SHB_TRY(c, hipMemcpyAsync(mask, in_a ? c.d_mask_a.ptr : c.d_mask_b.ptr, n,
+                            hipMemcpyDeviceToHost, c.stream));
+  SHB_TRY(c, hipStreamSynchronize(c.stream));
+  return SHB_OK;
+}
+
+int shb_trace_rays(shb_ctx* h, const shb_ray* rays, uint32_t n, int32_t any_hit, shb_hit* hits,
+                   uint32_t* node_visits, uint32_t* tri_tests) {
+  if (!h) return SHB_ERR_INVALID_ARG;
+  Ctx& c = h->c;
+  c.err.clear();
+  if (!c.has_scene) return fail(c, SHB_ERR_NO_SCENE, "no scene uploaded");
+  if (n == 0) return SHB_OK;
+  if (!rays || !hits) return fail(c, SHB_ERR_INVALID_ARG, "rays or hits is NULL");
+  SHB_TRY(c, hipSetDevice(c.device));
+  hipStream_t st = c.stream;
+  SHB_TRY(c, c.d_rays.reserve((size_t)n * sizeof(shb_ray)));
+  SHB_TRY(c, c.d_hits.reserve((size_t)n * sizeof(float4)));
+  SHB_TRY(c, c.d_hit_out.reserve((size_t)n * sizeof(shb_hit)));
+  SHB_TRY(c, c.d_work.reserve(64));
+  const bool count = node_visits || tri_tests;
+  if (count) SHB_TRY(c, c.d_raycounts.reserve((size_t)n * 8));
+  SHB_TRY(c, hipMemcpyAsync(c.d_rays.ptr, rays, (size_t)n * sizeof(shb_ray), hipMemcpyHostToDevice, st));
+  SHB_TRY(c, launch_fill_u32(c.d_work.as<uint32_t>(), 0u, 16, st));
+  int grid = 1;
+  int rc = trace_grid(c, any_hit != 0, &grid);
+  if (rc) return rc;
+  TraceArgs ta = trace_args(c);
+  ta.rays = c.d_rays.as<float4>();
+  ta.queue = nullptr;
+  ta.count_ptr = nullptr;
+  ta.count_const = n;
+  ta.work = c.d_work.as<uint32_t>() + 1;
+  ta.ray_counts = count ? c.d_raycounts.as<uint32_t>() : nullptr;
+  if (any_hit) {
+    SHB_TRY(c, launch_trace_occluded(ta, c.d_hits.as<uint32_t>(), count, grid, c.tuning.block_size, st));
+    std::vector<uint32_t> occ(n);
+    SHB_TRY(c, hipMemcpyAsync(occ.data(), c.d_hits.ptr, (size_t)n * 4, hipMemcpyDeviceToHost, st));
+    SHB_TRY(c, hipStreamSynchronize(st));
+    for (uint32_t i = 0; i < n; ++i) {
+      std::memset(&hits[i], 0, sizeof(shb_hit));
+      hits[i].prim_id = occ[i] ? 0 : -1;
+      hits[i].geom_id = -1;
+      hits[i].material_id = -1;
+    }
+  } else {
+    SHB_TRY(c, launch_trace_hits(ta, c.d_hits.as<float4>(), count, grid, c.tuning.block_size, st));
+    SHB_TRY(c, launch_hit_details(c.view(), c.d_rays.as<float4>(), c.d_hits.as<float4>(), n,
+                                  c.d_hit_out.as<shb_hit>(), st));
+    SHB_TRY(c, hipMemcpyAsync(hits, c.d_hit_out.ptr, (size_t)n * sizeof(shb_hit), hipMemcpyDeviceToHost, st));
+    SHB_TRY(c, hipStreamSynchronize(st));
+  }
+  if (count) {
+    std::vector<uint32_t> rc2((size_t)n * 2);
+    SHB_TRY(c, hipMemcpy(rc2.data(), c.d_raycounts.ptr, rc2.size() * 4, hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < n; ++i) {
+      if (node_visits) node_visits[i] = rc2[2 * i];
+      if (tri_tests) tri_tests[i] = rc2[2 * i + 1];
+    }
+  }
+  return check_overflow(c);
+}
+
+int shb_set_tuning(shb_ctx* h, const shb_tuning* t) {
+  if (!h || !t) return SHB_ERR_INVALID_ARG;
+  Ctx& c = h->c;
+  if (t->block_size != 64 && t->block_size != 128 && t->block_size != 256)
+    return fail(c, SHB_ERR_INVALID_ARG, "block_size must be 64, 128 or 256");
+  if (t->batch_paths < 1024) return fail(c, SHB_ERR_INVALID_ARG, "batch_paths must be >= 1024");
+  if (t->refill_threshold < 1 || t->refill_threshold > 64)
+    return fail(c, SHB_ERR_INVALID_ARG, "refill_threshold must be in [1, 64]");
+  if (t->bvh_width < 2 || t->bvh_width > 8)
+    return fail(c, SHB_ERR_INVALID_ARG, "bvh_width must be in [2, 8]");
+  bool rebuild = c.has_scene && t->bvh_width != c.tuning.bvh_width;
+  c.tuning = *t;
+  if (rebuild) {
+    SHB_TRY(c, hipSetDevice(c.device));
+    int rc = upload_bvh(c);
+    if (rc) return rc;
+    SHB_TRY(c, hipStreamSynchronize(c.stream));
+  }
+  return SHB_OK;
+}
+
+int shb_get_tuning(const shb_ctx* h, shb_tuning* t) {
+  if (!h || !t) return SHB_ERR_INVALID_ARG;
+  *t = h->c.tuning;
+  return SHB_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,217 @@
+// Shared host/device definitions of the MI355X SH-lightmap baker: device
+// scene layout, counter-based random stream, and the small vector math of
+// the bake. All HIP code is compiled with -ffp-contract=off; fused ops are
+// written as fmaf where the design wants them (traversal), so results are
+// reproducible bit for bit by a host restatement.
+#ifndef SHB_COMMON_H_
+#define SHB_COMMON_H_
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define SHB_HD __host__ __device__ __forceinline__
+
+namespace shb {
+
+// ------------------------------------------------------------ limits
+constexpr int kMaxBounces = 15;       // depth fits 4 bits of the RNG counter
+constexpr int kMaxSamples = 65535;    // sample fits 16 bits of the counter
+constexpr int kStackLds = 16;         // traversal stack entries kept in LDS
+constexpr int kStackTotal = 64;       // entries incl. the global spill area
+constexpr float kRayEps = 0.001f;     // baker.cpp:118, 122, light.h:40,83
+
+// ------------------------------------------------------------ BVH node
+// Compressed 8-wide node (after Ylitie, Karras, Laine, "Efficient Incoherent
+// Ray Traversal on GPUs Through Compressed Wide BVHs", HPG 2017): child boxes
+// quantised to 8 bits per plane on a per-node power-of-two grid anchored at
+// (px,py,pz). 80 bytes = five 16-byte loads.
+//   meta[s] == 0            : empty slot
+//   meta[s] & 0x80          : internal child, rank (meta & 7) among the
+//                             node's internal children -> child_base + rank
+//   otherwise               : leaf, count = meta >> 5 (1..3) triangles at
+//                             tri_base + (meta & 31)
+// Slots are arranged so that visiting slot s in order of s ^ sstar, where
+// sstar is the slot opposite to the ray's direction octant, is roughly front
+// to back.
+struct alignas(16) CWNode {
+  float px, py, pz;
+  int8_t ex, ey, ez;
+  uint8_t imask;
+  uint32_t child_base;
+  uint32_t tri_base;
+  uint8_t meta[8];
+  uint8_t qlox[8], qloy[8], qloz[8];
+  uint8_t qhix[8], qhiy[8], qhiz[8];
+};
+static_assert(sizeof(CWNode) == 80, "CWNode must be 80 bytes");
+
+// ------------------------------------------------------------ lights
+enum : int32_t { kLightPoint = 0, kLightDirectional = 1, kLightSpot = 2, kLightArea = 3 };
+
+struct alignas(16) LightDev {
+  float ic[3];         // intensity * color (light.h:48, 93, 143)
+  int32_t type;
+  float pos[3];
+  float cos_inner;
+  float dir[3];
+  float cos_outer;
+  float emit[3];       // area lights with a constant emitter: emission
+  int32_t emit_const;  // 1: emit[] is valid; 0: look up the texture per sample
+  uint32_t first_tri;  // area lights: first record in area_tris
+  uint32_t num_tris;   // area lights: triangle count of the geometry (0: off)
+  int32_t material;    // area lights: emitting material
+  int32_t pad;
+};
+
+// Area-light triangle record (object-space vertices, light.cpp:45-49).
+struct alignas(16) AreaTri {
+  float v0[3];
+  float pdf;           // max(1e-6, 1/ntri * 1/area)  (light.cpp:75-76)
+  float v1[3];
+  float has_uv;
+  float v2[3];
+  float pad;
+  float uv[6];
+  float pad2[2];
+};
+
+// ------------------------------------------------------------ materials
+struct alignas(16) MaterialDev {
+  uint32_t tex_w, tex_h, tex_c;
+  float emission;      // emission_intensity
+  uint64_t tex_off;    // byte offset into the texel pool (tex_c == 0: none)
+  uint32_t pad[2];
+};
+
+// Per-triangle shading data, indexed by global triangle id.
+//   s0 = (n0.xyz, uv0.x), s1 = (n1.xyz, uv0.y), s2 = (n2.xyz, uv1.x),
+//   s3 = (uv1.y, uv2.x, uv2.y, flags)  flags: bit0 vertex normals,
+//   bit1 texture uvs. Without vertex normals n0 holds the normalised world
+//   geometric normal.
+// TriInfo = (material, emitting light or -1, geometry, local triangle).
+
+// ------------------------------------------------------------ scene view
+struct SceneView {
+  const CWNode* nodes;
+  const float4* tris;      // leaf order: 3 float4 per triangle, v0.w = id
+  const float4* shade;     // 4 float4 per global triangle
+  const int4* tri_info;
+  const MaterialDev* mats;
+  const uint8_t* texels;
+  const float* srgb_lut;   // 256 entries, SRGBToLinear
+  const LightDev* lights;
+  const AreaTri* area_tris;
+  const float* light_scale;   // 3 per light
+  const float* albedo_scale;  // 3 per material
+  int32_t num_lights;
+  int32_t num_tris;
+  int32_t has_nodes;
+  int32_t pad;
+};
+
+// ------------------------------------------------------------ RNG
+// Philox4x32-10 (Salmon et al., SC'11) seeds one xoshiro128** state per path
+// vertex ("site"); counter = (texel, sample | depth<<16 | branch<<20, kind,
+// job), key = 64-bit seed.
+enum : uint32_t { kSiteTexel = 0, kSiteVertex = 1 };
+
+SHB_HD void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+    uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+    uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0;
+    uint32_t n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
+    c[1] = (uint32_t)p1;
+    c[3] = (uint32_t)p0;
+    c[0] = n0;
+    c[2] = n2;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+}
+
+struct Rng {
+  uint32_t s0, s1, s2, s3;
+  SHB_HD static uint32_t rotl(uint32_t x, int k) { return (x << k) | (x >> (32 - k)); }
+  SHB_HD uint32_t next() {
+    const uint32_t result = rotl(s1 * 5u, 7) * 9u;
+    const uint32_t t = s1 << 9;
+    s2 ^= s0;
+    s3 ^= s1;
+    s1 ^= s2;
+    s0 ^= s3;
+    s2 ^= t;
+    s3 = rotl(s3, 11);
+    return result;
+  }
+  SHB_HD float uniform() { return (float)(next() >> 8) * 0x1p-24f; }
+  SHB_HD uint32_t index(uint32_t n) {
+    return (uint32_t)(((uint64_t)next() * (uint64_t)n) >> 32);
+  }
+};
+
+SHB_HD Rng site_rng(uint64_t seed, uint32_t texel, uint32_t sample,
+                    uint32_t depth, uint32_t branch, uint32_t kind,
+                    uint32_t job) {
+  uint32_t c[4] = {texel, sample | (depth << 16) | (branch << 20), kind, job};
+  philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+  Rng r{c[0], c[1], c[2], c[3]};
+  if ((c[0] | c[1] | c[2] | c[3]) == 0u) r.s0 = 1u;
+  return r;
+}
+
+// ------------------------------------------------------------ math
+// Reductions keep Eigen's fixed-size order e0 + (e1 + e2).
+struct f3 {
+  float x, y, z;
+};
+SHB_HD f3 mk(float x, float y, float z) { return f3{x, y, z}; }
+SHB_HD f3 operator+(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+SHB_HD f3 operator-(f3 a, f3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+SHB_HD f3 operator-(f3 a) { return {-a.x, -a.y, -a.z}; }
+SHB_HD f3 operator*(f3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+SHB_HD f3 operator*(float s, f3 a) { return {s * a.x, s * a.y, s * a.z}; }
+SHB_HD f3 operator/(f3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+SHB_HD f3 cmul(f3 a, f3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+SHB_HD float dot(f3 a, f3 b) { return a.x * b.x + (a.y * b.y + a.z * b.z); }
+SHB_HD float sqnorm(f3 a) { return a.x * a.x + (a.y * a.y + a.z * a.z); }
+SHB_HD f3 normalized(f3 a) {
+  float z = sqnorm(a);
+  if (z > 0.f) {
+    float s = sqrtf(z);
+    return a / s;
+  }
+  return a;
+}
+SHB_HD f3 cross(f3 a, f3 b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+SHB_HD float maxc(f3 a) {
+  float m = a.y < a.z ? a.z : a.y;
+  return a.x < m ? m : a.x;
+}
+SHB_HD bool is_zero(f3 a) { return a.x == 0.f && a.y == 0.f && a.z == 0.f; }
+SHB_HD f3 f3_of(float4 v) { return {v.x, v.y, v.z}; }
+
+// Bit casts usable on host and device.
+SHB_HD float as_float(uint32_t u) {
+  union {
+    uint32_t u;
+    float f;
+  } x;
+  x.u = u;
+  return x.f;
+}
+SHB_HD uint32_t as_uint(float f) {
+  union {
+    uint32_t u;
+    float f;
+  } x;
+  x.f = f;
+  return x.u;
+}
+
+}  // namespace shb
+
+#endif  // SHB_COMMON_H_

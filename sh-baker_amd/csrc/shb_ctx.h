@@ -1,0 +1,117 @@
+// Context state of the baker (host side of the C ABI).
+#ifndef SHB_CTX_H_
+#define SHB_CTX_H_
+
+#include <string>
+#include <vector>
+
+#include "../../include/shb.h"
+#include "bvh_build.h"
+#include "kernels.h"
+
+namespace shb {
+
+// Grow-only device buffer.
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  hipError_t reserve(size_t n) {
+    if (n <= bytes) return hipSuccess;
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+    hipError_t e = hipMalloc(&ptr, n);
+    if (e == hipSuccess) bytes = n;
+    return e;
+  }
+  void release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+  }
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(ptr);
+  }
+};
+
+// Host copy of the parts of the scene the kernels need, kept so the BVH can
+// be rebuilt with another width and the raster inputs re-made.
+struct HostScene {
+  uint32_t num_vertices = 0, num_tris = 0;
+  std::vector<float> world_xyz;      // 9 per triangle (BVH input)
+  std::vector<float> wpos, wnrm, wtan, luv;  // per vertex (raster)
+  std::vector<uint32_t> tri_vidx;    // 3 global vertex ids per triangle
+  std::vector<int32_t> tri_mat;
+  std::vector<float4> shade;         // 4 per triangle
+  std::vector<int4> tri_info;
+  std::vector<MaterialDev> mats;
+  std::vector<uint8_t> texels;
+  std::vector<LightDev> lights;
+  std::vector<AreaTri> area_tris;
+  bool raster_ready = true;          // every geometry has luv, normals, tangents
+  bool has_transparency = false;     // some RGBA texel with alpha < 255
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int num_cus = 0;
+  std::string err;
+  shb_tuning tuning{256, 1 << 23, 16, 8};
+  int iters = 8;
+
+  // scene
+  bool has_scene = false;
+  HostScene hs;
+  BvhBuildOut bvh;
+  DevBuf d_nodes, d_tris, d_shade, d_tri_info, d_mats, d_texels, d_lut, d_lights, d_area,
+      d_lscale, d_ascale;
+  // raster inputs
+  DevBuf d_luv, d_wpos, d_wnrm, d_wtan, d_tri_vidx, d_tri_mat;
+  // surface
+  bool has_surface = false;
+  int width = 0, height = 0, ss = 1;
+  DevBuf d_surface;  // 3 float4 per supersampled texel
+  // bake scratch
+  DevBuf d_paths;    // path state arrays
+  DevBuf d_queues;
+  DevBuf d_counters;
+  DevBuf d_spill;
+  DevBuf d_work;     // misc small device scratch (error flags, totals)
+  DevBuf d_list, d_mask, d_tmp;
+  DevBuf d_sh, d_vis;         // supersampled outputs
+  DevBuf d_img_a, d_img_b, d_mask_a, d_mask_b;
+  DevBuf d_rays, d_hits, d_hit_out, d_raycounts;
+  DevBuf d_adj;               // adjoint accumulators
+  int trace_bpc[2] = {0, 0};  // blocks per CU (closest, any) at tuning.block_size
+  int trace_bpc_block = 0;
+
+  SceneView view() const {
+    SceneView v;
+    v.nodes = d_nodes.as<CWNode>();
+    v.tris = d_tris.as<float4>();
+    v.shade = d_shade.as<float4>();
+    v.tri_info = d_tri_info.as<int4>();
+    v.mats = d_mats.as<MaterialDev>();
+    v.texels = d_texels.as<uint8_t>();
+    v.srgb_lut = d_lut.as<float>();
+    v.lights = d_lights.as<LightDev>();
+    v.area_tris = d_area.as<AreaTri>();
+    v.light_scale = d_lscale.as<float>();
+    v.albedo_scale = d_ascale.as<float>();
+    v.num_lights = (int32_t)hs.lights.size();
+    v.num_tris = (int32_t)hs.num_tris;
+    v.has_nodes = bvh.nodes.empty() ? 0 : 1;
+    v.pad = 0;
+    return v;
+  }
+};
+
+}  // namespace shb
+
+struct shb_ctx {
+  shb::Ctx c;
+};
+
+#endif  // SHB_CTX_H_

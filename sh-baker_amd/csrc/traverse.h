@@ -1,0 +1,234 @@
+// Ray traversal of the compressed 8-wide BVH on CDNA4 (wave64).
+//
+// Per lane: one ray, a current "node group" (child base index + 8-bit hit
+// mask in octant order + the parent's internal-child mask) and a stack of
+// groups whose bottom kStackLds entries live in LDS (laid out entry-major,
+// lane-minor, so a wave's accesses hit 64 distinct banks) and whose deeper
+// entries spill to a per-lane global area. Triangles hit in a node are tested
+// as soon as the node is decoded. Ray/triangle: the watertight test of Woop,
+// Benthin, Wald (JCGT 2013), double sided, ties in t resolved to the lowest
+// triangle id so the closest hit does not depend on traversal order.
+#ifndef SHB_TRAVERSE_H_
+#define SHB_TRAVERSE_H_
+
+#include "shb_common.h"
+
+namespace shb {
+
+// Far-plane inflation 1 + 2^-21 against rounding in the slab test (after
+// Ize, "Robust BVH Ray Traversal", JCGT 2013).
+constexpr float kFarInflate = 1.00000048f;
+
+struct TravRay {
+  float ox, oy, oz;
+  float dx, dy, dz;
+  float ix, iy, iz;     // 1/d with zero components replaced by +-1e20
+  float tnear;
+  int kx, ky, kz;
+  float Sx, Sy, Sz;
+  uint32_t sstar;       // nearest octant slot
+  uint32_t negx, negy, negz;
+};
+
+__device__ __forceinline__ float safe_inv(float d) {
+  float ad = fabsf(d);
+  float r = 1.0f / d;
+  return ad < 1e-20f ? (d < 0.f || (d == 0.f && signbit(d)) ? -1e20f : 1e20f) : r;
+}
+
+__device__ __forceinline__ float sel3(int k, float x, float y, float z) {
+  return k == 0 ? x : (k == 1 ? y : z);
+}
+
+__device__ __forceinline__ void trav_setup(TravRay& r, float4 o, float4 d) {
+  r.ox = o.x;
+  r.oy = o.y;
+  r.oz = o.z;
+  r.tnear = o.w;
+  r.dx = d.x;
+  r.dy = d.y;
+  r.dz = d.z;
+  r.ix = safe_inv(d.x);
+  r.iy = safe_inv(d.y);
+  r.iz = safe_inv(d.z);
+  r.negx = d.x < 0.f;
+  r.negy = d.y < 0.f;
+  r.negz = d.z < 0.f;
+  uint32_t oct = r.negx | (r.negy << 1) | (r.negz << 2);
+  r.sstar = oct ^ 7u;
+  float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+  int kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
+  int kx = kz + 1 == 3 ? 0 : kz + 1;
+  int ky = kx + 1 == 3 ? 0 : kx + 1;
+  float dkz = sel3(kz, d.x, d.y, d.z);
+  if (dkz < 0.f) {
+    int t = kx;
+    kx = ky;
+    ky = t;
+  }
+  r.kx = kx;
+  r.ky = ky;
+  r.kz = kz;
+  r.Sx = sel3(kx, d.x, d.y, d.z) / dkz;
+  r.Sy = sel3(ky, d.x, d.y, d.z) / dkz;
+  r.Sz = 1.0f / dkz;
+}
+
+// Watertight test (see the host restatement for the identical arithmetic).
+__device__ __forceinline__ bool tri_test(const TravRay& r, float4 a, float4 b,
+                                         float4 c, float& t, float& u,
+                                         float& v) {
+  float Axv = a.x - r.ox, Ayv = a.y - r.oy, Azv = a.z - r.oz;
+  float Bxv = b.x - r.ox, Byv = b.y - r.oy, Bzv = b.z - r.oz;
+  float Cxv = c.x - r.ox, Cyv = c.y - r.oy, Czv = c.z - r.oz;
+  float Akz = sel3(r.kz, Axv, Ayv, Azv), Bkz = sel3(r.kz, Bxv, Byv, Bzv),
+        Ckz = sel3(r.kz, Cxv, Cyv, Czv);
+  float Ax = fmaf(-r.Sx, Akz, sel3(r.kx, Axv, Ayv, Azv));
+  float Ay = fmaf(-r.Sy, Akz, sel3(r.ky, Axv, Ayv, Azv));
+  float Bx = fmaf(-r.Sx, Bkz, sel3(r.kx, Bxv, Byv, Bzv));
+  float By = fmaf(-r.Sy, Bkz, sel3(r.ky, Bxv, Byv, Bzv));
+  float Cx = fmaf(-r.Sx, Ckz, sel3(r.kx, Cxv, Cyv, Czv));
+  float Cy = fmaf(-r.Sy, Ckz, sel3(r.ky, Cxv, Cyv, Czv));
+  float U = Cx * By - Cy * Bx;
+  float V = Ax * Cy - Ay * Cx;
+  float W = Bx * Ay - By * Ax;
+  if (U == 0.f || V == 0.f || W == 0.f) {
+    U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
+    V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
+    W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
+  }
+  if ((U < 0.f || V < 0.f || W < 0.f) && (U > 0.f || V > 0.f || W > 0.f)) return false;
+  float det = U + V + W;
+  if (det == 0.f) return false;
+  float Az = r.Sz * Akz, Bz = r.Sz * Bkz, Cz = r.Sz * Ckz;
+  float T = fmaf(U, Az, fmaf(V, Bz, W * Cz));
+  float inv = 1.0f / det;
+  t = T * inv;
+  u = V * inv;
+  v = W * inv;
+  return true;
+}
+
+struct TravState {
+  uint32_t gbase;   // group child base
+  uint32_t gbits;   // hits (bits 0-7, octant order) | imask << 8
+  int sp;
+  float tbest;
+  int best;         // INT_MAX: none
+  float bu, bv;
+  uint32_t nodes, tris;  // visit counters
+};
+
+__device__ __forceinline__ void trav_init(TravState& s, const TravRay& r, float tfar) {
+  s.gbase = 0;
+  s.gbits = (1u << r.sstar) | (1u << 8);
+  s.sp = 0;
+  s.tbest = tfar;
+  s.best = 0x7fffffff;
+  s.bu = s.bv = 0.f;
+  s.nodes = s.tris = 0;
+}
+
+// One traversal step: visit one node and test its triangles. Returns true
+// when the ray is finished (stack empty, or any-hit found).
+template <bool ANY, bool COUNT>
+__device__ __forceinline__ bool trav_step(const SceneView& sc, const TravRay& r,
+                                          TravState& s, uint2* lds, int lds_stride,
+                                          uint2* spill, bool& overflow, int stack_cap) {
+  if ((s.gbits & 0xffu) == 0u) {
+    if (s.sp == 0) return true;
+    --s.sp;
+    uint2 e = s.sp < kStackLds ? lds[s.sp * lds_stride] : spill[s.sp - kStackLds];
+    s.gbase = e.x;
+    s.gbits = e.y;
+  }
+  uint32_t hk = s.gbits & 0xffu;
+  uint32_t k = __builtin_ctz(hk);
+  s.gbits &= ~(1u << k);
+  uint32_t slot = k ^ r.sstar;
+  uint32_t im = (s.gbits >> 8) & 0xffu;
+  uint32_t ni = s.gbase + __builtin_popcount(im & ((1u << slot) - 1u));
+  if ((s.gbits & 0xffu) != 0u) {
+    if (s.sp < kStackLds) {
+      lds[s.sp * lds_stride] = make_uint2(s.gbase, s.gbits);
+    } else if (s.sp < stack_cap) {
+      spill[s.sp - kStackLds] = make_uint2(s.gbase, s.gbits);
+    } else {
+      overflow = true;
+      return true;
+    }
+    ++s.sp;
+  }
+  // ---- decode node ni
+  const uint4* np = reinterpret_cast<const uint4*>(sc.nodes + ni);
+  uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3], w4 = np[4];
+  if (COUNT) ++s.nodes;
+  float px = as_float(w0.x), py = as_float(w0.y), pz = as_float(w0.z);
+  uint32_t ew = w0.w;
+  float sx = as_float((uint32_t)((int)(int8_t)(ew & 0xffu) + 127) << 23);
+  float sy = as_float((uint32_t)((int)(int8_t)((ew >> 8) & 0xffu) + 127) << 23);
+  float sz = as_float((uint32_t)((int)(int8_t)((ew >> 16) & 0xffu) + 127) << 23);
+  uint32_t imask = ew >> 24;
+  float ax = sx * r.ix, bx = (px - r.ox) * r.ix;
+  float ay = sy * r.iy, by = (py - r.oy) * r.iy;
+  float az = sz * r.iz, bz = (pz - r.oz) * r.iz;
+  // near / far plane words per axis by direction sign
+  uint32_t nx0 = r.negx ? w3.z : w2.x, nx1 = r.negx ? w3.w : w2.y;
+  uint32_t fx0 = r.negx ? w2.x : w3.z, fx1 = r.negx ? w2.y : w3.w;
+  uint32_t ny0 = r.negy ? w4.x : w2.z, ny1 = r.negy ? w4.y : w2.w;
+  uint32_t fy0 = r.negy ? w2.z : w4.x, fy1 = r.negy ? w2.w : w4.y;
+  uint32_t nz0 = r.negz ? w4.z : w3.x, nz1 = r.negz ? w4.w : w3.y;
+  uint32_t fz0 = r.negz ? w3.x : w4.z, fz1 = r.negz ? w3.y : w4.w;
+  uint32_t hits = 0, tmask = 0;
+  const float tmax_ray = s.tbest;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const int sh = 8 * (c & 3);
+    uint32_t nxw = c < 4 ? nx0 : nx1, fxw = c < 4 ? fx0 : fx1;
+    uint32_t nyw = c < 4 ? ny0 : ny1, fyw = c < 4 ? fy0 : fy1;
+    uint32_t nzw = c < 4 ? nz0 : nz1, fzw = c < 4 ? fz0 : fz1;
+    uint32_t mw = c < 4 ? w1.z : w1.w;
+    float tnx = fmaf((float)((nxw >> sh) & 0xffu), ax, bx);
+    float tfx = fmaf((float)((fxw >> sh) & 0xffu), ax, bx);
+    float tny = fmaf((float)((nyw >> sh) & 0xffu), ay, by);
+    float tfy = fmaf((float)((fyw >> sh) & 0xffu), ay, by);
+    float tnz = fmaf((float)((nzw >> sh) & 0xffu), az, bz);
+    float tfz = fmaf((float)((fzw >> sh) & 0xffu), az, bz);
+    float tmin = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, r.tnear));
+    float tmax = fminf(fminf(fminf(tfx, tfy), tfz) * kFarInflate, tmax_ray);
+    uint32_t m = (mw >> sh) & 0xffu;
+    if (m != 0u && tmin <= tmax) {
+      if (m & 0x80u)
+        hits |= 1u << (c ^ r.sstar);
+      else
+        tmask |= ((1u << (m >> 5)) - 1u) << (m & 31u);
+    }
+  }
+  const uint32_t tri_base = w1.y;
+  const uint32_t child_base = w1.x;
+  while (tmask) {
+    uint32_t j = __builtin_ctz(tmask);
+    tmask &= tmask - 1u;
+    const float4* tp = sc.tris + 3 * (size_t)(tri_base + j);
+    float4 a = tp[0], b = tp[1], c = tp[2];
+    if (COUNT) ++s.tris;
+    float t, u, v;
+    if (tri_test(r, a, b, c, t, u, v)) {
+      int id = (int)as_uint(a.w);
+      if (t >= r.tnear && (t < s.tbest || (t == s.tbest && id < s.best))) {
+        s.tbest = t;
+        s.best = id;
+        s.bu = u;
+        s.bv = v;
+        if (ANY) return true;
+      }
+    }
+  }
+  s.gbase = child_base;
+  s.gbits = hits | (imask << 8);
+  return false;
+}
+
+}  // namespace shb
+
+#endif  // SHB_TRAVERSE_H_

@@ -1,0 +1,348 @@
+"""Python bindings of libshb_hip.so (the C ABI in include/shb.h).
+
+The GPU path is the only path: importing this package loads the in-tree
+HIP library and raises if it is missing. Host buffers are numpy arrays.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libshb_hip.so")
+
+# ------------------------------------------------------------------ structs
+SHB_OK = 0
+ERRORS = {
+    -1: "SHB_ERR_INVALID_ARG",
+    -2: "SHB_ERR_NO_SCENE",
+    -3: "SHB_ERR_DEVICE",
+    -4: "SHB_ERR_OOM",
+    -5: "SHB_ERR_LIMIT",
+    -6: "SHB_ERR_NO_SURFACE",
+    -7: "SHB_ERR_UNSUPPORTED",
+}
+LIGHT_POINT, LIGHT_DIRECTIONAL, LIGHT_SPOT, LIGHT_AREA = 0, 1, 2, 3
+GEOM_HAS_NORMALS, GEOM_HAS_TEXTURE_UVS, GEOM_HAS_LIGHTMAP_UVS, GEOM_HAS_TANGENTS = 1, 2, 4, 8
+BAKE_COUNT_VISITS = 1
+
+
+class ShbGeometry(C.Structure):
+    _fields_ = [("vertex_offset", C.c_uint32), ("vertex_count", C.c_uint32),
+                ("index_offset", C.c_uint32), ("index_count", C.c_uint32),
+                ("material_id", C.c_int32), ("flags", C.c_uint32),
+                ("transform", C.c_float * 12)]
+
+
+class ShbMaterial(C.Structure):
+    _fields_ = [("albedo_width", C.c_uint32), ("albedo_height", C.c_uint32),
+                ("albedo_channels", C.c_uint32), ("reserved", C.c_uint32),
+                ("albedo_offset", C.c_uint64), ("emission_intensity", C.c_float),
+                ("reserved2", C.c_float)]
+
+
+class ShbLight(C.Structure):
+    _fields_ = [("type", C.c_int32), ("position", C.c_float * 3),
+                ("direction", C.c_float * 3), ("color", C.c_float * 3),
+                ("intensity", C.c_float), ("cos_inner_cone", C.c_float),
+                ("cos_outer_cone", C.c_float), ("geometry_index", C.c_int32),
+                ("material_index", C.c_int32)]
+
+
+_fp = C.POINTER(C.c_float)
+
+
+class ShbSceneDesc(C.Structure):
+    _fields_ = [("num_vertices", C.c_uint32), ("positions", _fp), ("normals", _fp),
+                ("texture_uvs", _fp), ("lightmap_uvs", _fp), ("tangents", _fp),
+                ("num_indices", C.c_uint32), ("indices", C.POINTER(C.c_uint32)),
+                ("num_geometries", C.c_uint32), ("geometries", C.POINTER(ShbGeometry)),
+                ("num_materials", C.c_uint32), ("materials", C.POINTER(ShbMaterial)),
+                ("texel_pool_size", C.c_uint64), ("texel_pool", C.POINTER(C.c_uint8)),
+                ("num_lights", C.c_uint32), ("lights", C.POINTER(ShbLight))]
+
+
+class ShbRasterCfg(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("supersample_scale", C.c_int32)]
+
+
+SURFACE_DTYPE = np.dtype([("position", "<f4", 3), ("normal", "<f4", 3), ("tangent", "<f4", 4),
+                          ("material_id", "<i4"), ("reserved", "<i4")])
+assert SURFACE_DTYPE.itemsize == 48
+
+
+class ShbBakeCfg(C.Structure):
+    _fields_ = [("samples", C.c_int32), ("bounces", C.c_int32),
+                ("num_light_samples", C.c_int32), ("flags", C.c_uint32), ("seed", C.c_uint64)]
+
+
+class ShbBakeStats(C.Structure):
+    _fields_ = [("valid_texels", C.c_uint64), ("paths", C.c_uint64),
+                ("rays_extension", C.c_uint64), ("rays_shadow", C.c_uint64),
+                ("node_visits", C.c_uint64), ("tri_tests", C.c_uint64),
+                ("seconds_total", C.c_double), ("seconds_device", C.c_double),
+                ("seconds_trace", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ShbTuning(C.Structure):
+    _fields_ = [("block_size", C.c_int32), ("batch_paths", C.c_int32),
+                ("refill_threshold", C.c_int32), ("bvh_width", C.c_int32)]
+
+
+RAY_DTYPE = np.dtype([("origin", "<f4", 3), ("tnear", "<f4"), ("direction", "<f4", 3),
+                      ("tfar", "<f4")])
+HIT_DTYPE = np.dtype([("t", "<f4"), ("u", "<f4"), ("v", "<f4"), ("prim_id", "<i4"),
+                      ("geom_id", "<i4"), ("material_id", "<i4"), ("position", "<f4", 3),
+                      ("normal", "<f4", 3), ("uv", "<f4", 2)])
+assert HIT_DTYPE.itemsize == 56
+
+EXPORTED = [
+    "shb_create", "shb_destroy", "shb_last_error", "shb_version", "shb_upload_scene",
+    "shb_rasterize", "shb_set_surface", "shb_validity_mask", "shb_bake", "shb_bake_lightmap",
+    "shb_downsample", "shb_downsample_mask", "shb_dilate", "shb_trace_rays",
+    "shb_bake_batched", "shb_autotune", "shb_set_tuning", "shb_get_tuning", "shb_set_params",
+    "shb_bake_adjoint",
+]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `make -C sh-baker_amd` "
+            "(or __graft_entry__.build()); there is no CPU fallback")
+    lib = C.CDLL(LIB_PATH)
+    vp, ip, u8p = C.c_void_p, C.c_int, C.POINTER(C.c_uint8)
+    lib.shb_create.argtypes = [C.POINTER(vp), ip]
+    lib.shb_destroy.argtypes = [vp]
+    lib.shb_destroy.restype = None
+    lib.shb_last_error.argtypes = [vp]
+    lib.shb_last_error.restype = C.c_char_p
+    lib.shb_version.restype = C.c_char_p
+    lib.shb_upload_scene.argtypes = [vp, C.POINTER(ShbSceneDesc)]
+    lib.shb_rasterize.argtypes = [vp, C.POINTER(ShbRasterCfg), vp]
+    lib.shb_set_surface.argtypes = [vp, C.POINTER(ShbRasterCfg), vp]
+    lib.shb_validity_mask.argtypes = [vp, u8p]
+    lib.shb_bake.argtypes = [vp, C.POINTER(ShbBakeCfg), _fp, _fp, C.POINTER(ShbBakeStats)]
+    lib.shb_bake_lightmap.argtypes = [vp, C.POINTER(ShbBakeCfg), C.c_int32, _fp, _fp, u8p,
+                                      C.POINTER(ShbBakeStats)]
+    lib.shb_downsample.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _fp, _fp]
+    lib.shb_downsample_mask.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, u8p, u8p]
+    lib.shb_dilate.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, _fp, u8p, C.c_int32]
+    lib.shb_trace_rays.argtypes = [vp, vp, C.c_uint32, C.c_int32, vp,
+                                   C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+    lib.shb_bake_batched.argtypes = [C.POINTER(vp), C.c_int32, C.POINTER(ShbBakeCfg),
+                                     C.POINTER(_fp), C.POINTER(_fp), C.POINTER(ShbBakeStats)]
+    lib.shb_autotune.argtypes = [vp, C.POINTER(ShbBakeCfg), C.POINTER(ShbTuning)]
+    lib.shb_set_tuning.argtypes = [vp, C.POINTER(ShbTuning)]
+    lib.shb_get_tuning.argtypes = [vp, C.POINTER(ShbTuning)]
+    lib.shb_set_params.argtypes = [vp, _fp, _fp]
+    lib.shb_bake_adjoint.argtypes = [vp, C.POINTER(ShbBakeCfg), _fp, _fp, _fp,
+                                     C.POINTER(ShbBakeStats)]
+    return lib
+
+
+lib = None
+
+
+def get_lib():
+    """Loads libshb_hip.so on first use (raises ImportError if it is missing)."""
+    global lib
+    if lib is None:
+        lib = _load()
+    return lib
+
+
+def version() -> str:
+    return get_lib().shb_version().decode()
+
+
+class ShbError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+def _f(a):
+    return a.ctypes.data_as(_fp) if a is not None else None
+
+
+def _u8(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint8)) if a is not None else None
+
+
+class Baker:
+    """One shb_ctx: a scene, a surface, and the bake entry points."""
+
+    def __init__(self, device: int = 0):
+        get_lib()
+        h = C.c_void_p()
+        rc = lib.shb_create(C.byref(h), device)
+        if rc != SHB_OK:
+            raise ShbError(rc, "shb_create failed (no HIP device?)")
+        self.h = h
+        self.width = self.height = 0
+        self.ss = 1
+        self.scene = None
+
+    def close(self):
+        if self.h:
+            lib.shb_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc != SHB_OK:
+            raise ShbError(rc, lib.shb_last_error(self.h).decode())
+
+    # -- scene / surface
+    def upload_scene(self, scene):
+        desc, keep = scene.to_desc()
+        self._chk(lib.shb_upload_scene(self.h, C.byref(desc)))
+        self.scene = scene
+        del keep
+
+    def rasterize(self, width, height, ss=1, want=True):
+        cfg = ShbRasterCfg(width, height, ss)
+        out = np.zeros(width * ss * height * ss, SURFACE_DTYPE) if want else None
+        self._chk(lib.shb_rasterize(self.h, C.byref(cfg),
+                                    out.ctypes.data_as(C.c_void_p) if want else None))
+        self.width, self.height, self.ss = width, height, ss
+        return out
+
+    def set_surface(self, points, width, height, ss=1):
+        points = np.ascontiguousarray(points, SURFACE_DTYPE)
+        assert points.size == width * ss * height * ss
+        cfg = ShbRasterCfg(width, height, ss)
+        self._chk(lib.shb_set_surface(self.h, C.byref(cfg), points.ctypes.data_as(C.c_void_p)))
+        self.width, self.height, self.ss = width, height, ss
+
+    def validity_mask(self):
+        n = self.width * self.ss * self.height * self.ss
+        m = np.zeros(n, np.uint8)
+        self._chk(lib.shb_validity_mask(self.h, _u8(m)))
+        return m
+
+    # -- bake
+    @staticmethod
+    def cfg(samples, bounces, seed=12345, flags=0, num_light_samples=1):
+        return ShbBakeCfg(samples, bounces, num_light_samples, flags, seed)
+
+    def bake(self, samples, bounces, seed=12345, flags=0, num_light_samples=1):
+        n = self.width * self.ss * self.height * self.ss
+        sh = np.zeros((n, 27), np.float32)
+        vis = np.zeros(n, np.float32)
+        st = ShbBakeStats()
+        cfg = self.cfg(samples, bounces, seed, flags, num_light_samples)
+        self._chk(lib.shb_bake(self.h, C.byref(cfg), _f(sh), _f(vis), C.byref(st)))
+        return sh, vis, st.as_dict()
+
+    def bake_lightmap(self, samples, bounces, dilation, seed=12345, flags=0):
+        n = self.width * self.height
+        sh = np.zeros((n, 27), np.float32)
+        vis = np.zeros(n, np.float32)
+        mask = np.zeros(n, np.uint8)
+        st = ShbBakeStats()
+        cfg = self.cfg(samples, bounces, seed, flags)
+        self._chk(lib.shb_bake_lightmap(self.h, C.byref(cfg), dilation, _f(sh), _f(vis),
+                                        _u8(mask), C.byref(st)))
+        return sh, vis, mask, st.as_dict()
+
+    # -- image ops
+    def downsample(self, img, width, height, scale):
+        img = np.ascontiguousarray(img, np.float32)
+        ch = img.size // (width * scale * height * scale)
+        out = np.zeros((width * height, ch), np.float32)
+        self._chk(lib.shb_downsample(self.h, width, height, scale, ch, _f(img), _f(out)))
+        return out
+
+    def downsample_mask(self, mask, width, height, scale):
+        mask = np.ascontiguousarray(mask, np.uint8)
+        out = np.zeros(width * height, np.uint8)
+        self._chk(lib.shb_downsample_mask(self.h, width, height, scale, _u8(mask), _u8(out)))
+        return out
+
+    def dilate(self, data, mask, width, height, passes):
+        data = np.array(data, np.float32, order="C", copy=True)
+        mask = np.array(mask, np.uint8, order="C", copy=True)
+        ch = data.size // max(1, width * height)
+        self._chk(lib.shb_dilate(self.h, width, height, ch, _f(data), _u8(mask), passes))
+        return data, mask
+
+    # -- rays
+    def trace_rays(self, rays, any_hit=False, counts=False):
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        n = rays.size
+        hits = np.zeros(n, HIT_DTYPE)
+        nv = np.zeros(n, np.uint32) if counts else None
+        tt = np.zeros(n, np.uint32) if counts else None
+        p32 = C.POINTER(C.c_uint32)
+        self._chk(lib.shb_trace_rays(
+            self.h, rays.ctypes.data_as(C.c_void_p), n, 1 if any_hit else 0,
+            hits.ctypes.data_as(C.c_void_p),
+            nv.ctypes.data_as(p32) if counts else None,
+            tt.ctypes.data_as(p32) if counts else None))
+        return (hits, nv, tt) if counts else hits
+
+    # -- tuning / params
+    def tuning(self):
+        t = ShbTuning()
+        self._chk(lib.shb_get_tuning(self.h, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in t._fields_}
+
+    def set_tuning(self, **kw):
+        cur = self.tuning()
+        cur.update(kw)
+        t = ShbTuning(**cur)
+        self._chk(lib.shb_set_tuning(self.h, C.byref(t)))
+
+    def autotune(self, samples, bounces, seed=12345):
+        t = ShbTuning()
+        cfg = self.cfg(samples, bounces, seed)
+        self._chk(lib.shb_autotune(self.h, C.byref(cfg), C.byref(t)))
+        return {k: getattr(t, k) for k, _ in t._fields_}
+
+    def set_params(self, light_scale=None, albedo_scale=None):
+        ls = None if light_scale is None else np.ascontiguousarray(light_scale, np.float32)
+        al = None if albedo_scale is None else np.ascontiguousarray(albedo_scale, np.float32)
+        self._chk(lib.shb_set_params(self.h, _f(ls), _f(al)))
+
+    def bake_adjoint(self, dloss_dsh, samples, bounces, seed=12345):
+        g = np.ascontiguousarray(dloss_dsh, np.float32)
+        nl = len(self.scene.lights)
+        nm = len(self.scene.materials)
+        dl = np.zeros((max(nl, 1), 3), np.float32)
+        dm = np.zeros((max(nm, 1), 3), np.float32)
+        st = ShbBakeStats()
+        cfg = self.cfg(samples, bounces, seed)
+        self._chk(lib.shb_bake_adjoint(self.h, C.byref(cfg), _f(g), _f(dl), _f(dm), C.byref(st)))
+        return dl[:nl], dm[:nm], st.as_dict()
+
+
+def bake_batched(bakers, cfgs):
+    """Bakes several Baker contexts in one batched launch sequence."""
+    get_lib()
+    n = len(bakers)
+    hs = (C.c_void_p * n)(*[b.h for b in bakers])
+    carr = (ShbBakeCfg * n)(*[Baker.cfg(*c) if isinstance(c, tuple) else c for c in cfgs])
+    outs_sh, outs_vis = [], []
+    for b in bakers:
+        npx = b.width * b.ss * b.height * b.ss
+        outs_sh.append(np.zeros((npx, 27), np.float32))
+        outs_vis.append(np.zeros(npx, np.float32))
+    psh = (_fp * n)(*[_f(a) for a in outs_sh])
+    pvis = (_fp * n)(*[_f(a) for a in outs_vis])
+    st = ShbBakeStats()
+    rc = lib.shb_bake_batched(hs, n, carr, psh, pvis, C.byref(st))
+    if rc != SHB_OK:
+        raise ShbError(rc, lib.shb_last_error(bakers[0].h).decode())
+    return outs_sh, outs_vis, st.as_dict()

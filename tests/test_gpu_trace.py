@@ -1,0 +1,151 @@
+"""GPU traversal parity: shb_trace_rays (compressed 8-wide BVH, watertight
+FP32 test) against the oracle's FP32 restatement (bit exact: same triangle,
+same t/u/v bits) and its FP64 brute force (same triangle except edge ties,
+t within 1e-5 relative). Reference: FindOcclusion, src/occlusion.cpp:9-107,
+and the KATs of src/occlusion_test.cpp."""
+import numpy as np
+import pytest
+
+import shb
+from shb import RAY_DTYPE, scenes
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def seeded_rays(scene, n, seed, inside_box=None, unnormalized=True):
+    rng = np.random.default_rng(seed)
+    lo = scenes_bounds(scene)[0] if inside_box is None else np.array(inside_box[0])
+    hi = scenes_bounds(scene)[1] if inside_box is None else np.array(inside_box[1])
+    r = np.zeros(n, RAY_DTYPE)
+    r["origin"] = rng.uniform(lo, hi, (n, 3))
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    if unnormalized:
+        d *= rng.uniform(0.1, 10.0, (n, 1))
+    d[: n // 50, 1] = 0.0  # axis-parallel components
+    r["direction"] = d
+    r["tnear"] = np.where(rng.random(n) < 0.5, 0.0, 0.001)
+    r["tfar"] = np.where(rng.random(n) < 0.7, np.inf, rng.uniform(0.1, 20.0, n))
+    return r
+
+
+def scenes_bounds(scene):
+    p = scene.positions
+    return p.min(0), p.max(0)
+
+
+def compare_exact(hg, ho):
+    np.testing.assert_array_equal(hg["prim_id"], ho["prim_id"])
+    hit = ho["prim_id"] >= 0
+    for k in ("t", "u", "v", "position", "normal", "uv"):
+        a, b = hg[k][hit], ho[k][hit]
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), k
+    np.testing.assert_array_equal(hg["material_id"], ho["material_id"])
+    np.testing.assert_array_equal(hg["geom_id"], ho["geom_id"])
+
+
+@pytest.mark.parametrize("which", ["c1", "synth20k"])
+def test_closest_hit_bit_exact_vs_oracle(baker, which):
+    s = scenes.box_room_scene() if which == "c1" else scenes.scaled_synthetic_scene(20000, 256, 7)
+    baker.upload_scene(s)
+    rays = seeded_rays(s, 20000, 1)
+    hg = baker.trace_rays(rays)
+    o = orc.Oracle(s)
+    ho = o.trace(rays, mode=0)
+    compare_exact(hg, ho)
+    assert (hg["prim_id"] >= 0).mean() > 0.3
+
+
+def test_closest_hit_vs_fp64_bruteforce(baker):
+    s = scenes.scaled_synthetic_scene(6000, 128, 3)
+    baker.upload_scene(s)
+    rays = seeded_rays(s, 4000, 2)
+    hg = baker.trace_rays(rays)
+    h64 = orc.Oracle(s, bvh=False).trace(rays, mode=1)
+    same = hg["prim_id"] == h64["prim_id"]
+    # edge / vertex ties and grazing hits may pick a neighbour triangle
+    assert same.mean() > 0.995, same.mean()
+    both = same & (hg["prim_id"] >= 0)
+    rel = np.abs(hg["t"][both] - h64["t"][both]) / np.maximum(np.abs(h64["t"][both]), 1e-30)
+    assert rel.max() < 1e-5, rel.max()
+    # a mismatch must be a near tie: the FP64 t of both candidates agrees
+    mism = ~same & (hg["prim_id"] >= 0) & (h64["prim_id"] >= 0)
+    if mism.any():
+        r2 = np.abs(hg["t"][mism] - h64["t"][mism]) / np.abs(h64["t"][mism])
+        assert r2.max() < 1e-4
+
+
+def test_any_hit_matches_closest(baker):
+    s = scenes.scaled_synthetic_scene(20000, 256, 9)
+    baker.upload_scene(s)
+    rays = seeded_rays(s, 20000, 3)
+    rays["tfar"] = np.random.default_rng(4).uniform(0.01, 30.0, rays.size)
+    ha = baker.trace_rays(rays, any_hit=True)
+    ho = orc.Oracle(s).trace(rays, mode=2)
+    np.testing.assert_array_equal(ha["prim_id"] >= 0, ho["prim_id"] >= 0)
+
+
+def test_occlusion_kats(baker):
+    # occlusion_test.cpp:18-62 TriangleIntersection
+    s = scenes.triangles_scene([[[-1, -1, 0], [1, -1, 0], [0, 1, 0]]])
+    baker.upload_scene(s)
+    r = np.zeros(3, RAY_DTYPE)
+    r["origin"] = [[0, 0, 1], [2, 0, 1], [0, 0, 1]]
+    r["direction"] = [[0, 0, -1], [0, 0, -1], [0, 0, 1]]
+    r["tfar"] = np.inf
+    h = baker.trace_rays(r)
+    assert h["prim_id"][0] == 0 and abs(h["position"][0][2]) < 1e-4
+    assert abs(h["normal"][0][2] - 1.0) < 1e-4
+    assert h["prim_id"][1] < 0 and h["prim_id"][2] < 0
+    # occlusion_test.cpp:64-115 RayClipping
+    r = np.zeros(3, RAY_DTYPE)
+    r["origin"] = [0, 0, 1]
+    r["direction"] = [0, 0, -1]
+    r["tnear"] = [0.0, 1.2, 0.1]
+    r["tfar"] = [0.5, np.inf, 10.0]
+    h = baker.trace_rays(r)
+    assert h["prim_id"][0] < 0 and h["prim_id"][1] < 0
+    assert h["prim_id"][2] == 0 and abs(h["position"][2][2]) < 1e-4
+    # occlusion_test.cpp:117-184 MultiGeometryMaterialTest
+    quad_a = ([[-3, -1, 0], [-1, -1, 0], [-3, 1, 0], [-1, 1, 0]], [0, 1, 2, 1, 3, 2])
+    quad_b = ([[1, -1, 0], [3, -1, 0], [1, 1, 0], [3, 1, 0]], [0, 1, 2, 1, 3, 2])
+    s = scenes.triangles_scene([quad_a, quad_b], materials=[10, 20])
+    baker.upload_scene(s)
+    r = np.zeros(1, RAY_DTYPE)
+    r["origin"] = [2.8, 0.8, 5.0]
+    r["direction"] = [0, 0, -1]
+    r["tfar"] = np.inf
+    h = baker.trace_rays(r)
+    assert h["material_id"][0] == 20
+    np.testing.assert_allclose(h["position"][0], [2.8, 0.8, 0.0], atol=1e-4)
+
+
+def test_empty_scene_misses(baker):
+    s = scenes.triangles_scene([[[0, 0, 0], [1, 0, 0], [0, 1, 0]]])
+    s.geometries[0]["index_count"] = 0
+    baker.upload_scene(s)
+    r = np.zeros(4, RAY_DTYPE)
+    r["direction"] = [0, 0, 1]
+    r["tfar"] = np.inf
+    h = baker.trace_rays(r)
+    assert (h["prim_id"] < 0).all()
+
+
+def test_bvh_widths_agree(baker):
+    s = scenes.scaled_synthetic_scene(8000, 128, 5)
+    baker.upload_scene(s)
+    rays = seeded_rays(s, 8000, 6)
+    ref = orc.Oracle(s).trace(rays, mode=0)
+    for w in (2, 4, 8):
+        baker.set_tuning(bvh_width=w)
+        compare_exact(baker.trace_rays(rays), ref)
+    baker.set_tuning(bvh_width=8)
+
+
+def test_visit_counts(baker):
+    s = scenes.scaled_synthetic_scene(20000, 256, 7)
+    baker.upload_scene(s)
+    rays = seeded_rays(s, 5000, 8)
+    hits, nv, tt = baker.trace_rays(rays, counts=True)
+    assert (nv >= 1).all() and nv.mean() < 200 and tt.mean() < 400
