@@ -162,7 +162,7 @@ typedef struct shb_bake_stats {
 typedef struct shb_tuning {
   int32_t block_size;       /* threads per block of the traversal kernels   */
   int32_t batch_paths;      /* path slots resident per wavefront batch      */
-  int32_t refill_threshold; /* refill a wave when fewer lanes are active    */
+  int32_t reserved;         /* must be 0                                    */
   int32_t bvh_width;        /* children per BVH node, 2..8 (rebuilds)       */
 } shb_tuning;
 

@@ -1,9 +1,8 @@
 // Lightmap-space kernels: UV rasterizer, validity mask, box downsample,
-// dilation, valid-texel compaction. Compiled with -ffp-contract=off; every
+// dilation, valid-texel compaction, and the exclusive scans they use (a
+// single-block scan of per-block partial sums: no cross-block waits). Compiled with -ffp-contract=off; every
 // float expression keeps the reference's operation order so results are bit
 // exact against a serial restatement.
-#include <hipcub/hipcub.hpp>
-
 #include "kernels.h"
 
 namespace shb {
@@ -148,22 +147,39 @@ __global__ void k_raster_resolve(RasterArgs a) {
   out[2] = make_float4(t3.z, tw, as_float((uint32_t)a.tri_mat[t]), 0.f);
 }
 
-__global__ void k_u64_from_u32(const uint32_t* in, unsigned long long* out, uint32_t n) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = in[i];
+// Exclusive scan of n uint32 counts into n + 1 uint64 offsets (out[n] =
+// total) by one block of 1024 threads: each thread sums a contiguous chunk,
+// the 1024 chunk sums are scanned in LDS, then every chunk is rewritten.
+constexpr int kScanThreads = 1024;
+__global__ void __launch_bounds__(kScanThreads) k_scan_u32_to_u64(const uint32_t* in, uint32_t n,
+                                                                 unsigned long long* out) {
+  __shared__ unsigned long long part[kScanThreads];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t chunk = (n + kScanThreads - 1) / kScanThreads;
+  const uint32_t b = tid * chunk, e = min(n, b + chunk);
+  unsigned long long acc = 0;
+  for (uint32_t k = b; k < e; ++k) acc += in[k];
+  part[tid] = acc;
+  __syncthreads();
+  if (tid == 0) {
+    unsigned long long run = 0;
+    for (int k = 0; k < kScanThreads; ++k) {
+      unsigned long long v = part[k];
+      part[k] = run;
+      run += v;
+    }
+    out[n] = run;
+  }
+  __syncthreads();
+  unsigned long long run = part[tid];
+  for (uint32_t k = b; k < e; ++k) {
+    out[k] = run;
+    run += in[k];
+  }
 }
 
-size_t raster_scan_tmp_bytes(uint32_t n) {
-  size_t bytes = 0;
-  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (unsigned long long*)nullptr,
-                                   (unsigned long long*)nullptr, (int)n + 1);
-  return bytes;
-}
-
-// counts: scratch of num_tris + 1 uint32; scan_tmp: hipcub workspace followed
-// by (num_tris + 1) * 2 unsigned long long.
-hipError_t launch_raster(const RasterArgs& a, uint32_t* counts, void* scan_tmp,
-                         size_t scan_tmp_bytes, hipStream_t st) {
+hipError_t launch_raster(const RasterArgs& a, uint32_t* counts, unsigned long long* scan,
+                         hipStream_t st) {
   const uint32_t n = a.num_tris;
   size_t npx = (size_t)a.W * a.H;
   hipError_t e;
@@ -171,24 +187,16 @@ hipError_t launch_raster(const RasterArgs& a, uint32_t* counts, void* scan_tmp,
   if ((e = launch_fill_u32(a.fb, 0xffffffffu, npx, st)) != hipSuccess) return e;
   if (n > 0) {
     if ((e = launch_fill_u32(a.tri_hit, 0u, n, st)) != hipSuccess) return e;
-    if ((e = launch_fill_u32(counts, 0u, (size_t)n + 1, st)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_raster_count, dim3((n + 255) / 256), dim3(256), 0, st, a, counts);
-    size_t cub_bytes = raster_scan_tmp_bytes(n);
-    unsigned long long* c64 =
-        reinterpret_cast<unsigned long long*>(static_cast<char*>(scan_tmp) + ((cub_bytes + 255) & ~size_t(255)));
-    unsigned long long* s64 = c64 + (n + 1);
-    hipLaunchKernelGGL(k_u64_from_u32, dim3((n + 1 + 255) / 256), dim3(256), 0, st, counts, c64,
-                       n + 1);
-    e = hipcub::DeviceScan::ExclusiveSum(scan_tmp, cub_bytes, c64, s64, (int)n + 1, st);
-    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_scan_u32_to_u64, dim3(1), dim3(kScanThreads), 0, st, counts, n, scan);
     unsigned long long total = 0;
-    if ((e = hipMemcpyAsync(&total, s64 + n, sizeof(total), hipMemcpyDeviceToHost, st)) != hipSuccess)
+    if ((e = hipMemcpyAsync(&total, scan + n, sizeof(total), hipMemcpyDeviceToHost, st)) != hipSuccess)
       return e;
     if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
     if (total > 0) {
       unsigned long long blocks = (total + 255) / 256;
       int grid = (int)(blocks < 65536ull ? blocks : 65536ull);
-      hipLaunchKernelGGL(k_raster_cover, dim3(grid), dim3(256), 0, st, a, s64, total);
+      hipLaunchKernelGGL(k_raster_cover, dim3(grid), dim3(256), 0, st, a, scan, total);
     }
     hipLaunchKernelGGL(k_raster_fallback, dim3((n + 255) / 256), dim3(256), 0, st, a);
   }
@@ -338,21 +346,86 @@ hipError_t launch_dilate(float* a, float* b, uint8_t* ma, uint8_t* mb, int w, in
 }
 
 // ------------------------------------------------------------- compaction
-// Ascending list of texels whose mask is set (hipCUB select over a counting
-// iterator).
-size_t compact_tmp_bytes(uint32_t n) {
-  size_t bytes = 0;
-  hipcub::CountingInputIterator<uint32_t> it(0u);
-  (void)hipcub::DeviceSelect::Flagged(nullptr, bytes, it, (const uint8_t*)nullptr, (uint32_t*)nullptr,
-                                (uint32_t*)nullptr, (int)n);
-  return bytes;
+// Ascending list of the set mask bytes: per-block counts, a single-block scan
+// of the counts, then an ordered in-block scatter (wave ballots + LDS).
+constexpr int kCompactBlock = 256;
+
+uint32_t compact_blocks(uint32_t n) { return (n + kCompactBlock - 1) / kCompactBlock; }
+
+__device__ __forceinline__ uint32_t block_exclusive_flags(bool flag, uint32_t* total,
+                                                          uint32_t* wave_sums) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint64_t m = __ballot(flag);
+  uint32_t in_wave = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+  if (lane == 0) wave_sums[wave] = (uint32_t)__popcll(m);
+  __syncthreads();
+  uint32_t before = 0, all = 0;
+  for (int w = 0; w < kCompactBlock / 64; ++w) {
+    if (w < wave) before += wave_sums[w];
+    all += wave_sums[w];
+  }
+  *total = all;
+  return before + in_wave;
 }
 
-hipError_t launch_compact_valid(const uint8_t* mask, uint32_t n, uint32_t* list, uint32_t* count,
-                                void* tmp, size_t tmp_bytes, hipStream_t st) {
-  if (n == 0) return launch_fill_u32(count, 0u, 1, st);
-  hipcub::CountingInputIterator<uint32_t> it(0u);
-  return hipcub::DeviceSelect::Flagged(tmp, tmp_bytes, it, mask, list, count, (int)n, st);
+__global__ void __launch_bounds__(kCompactBlock) k_compact_count(const uint8_t* mask, uint32_t n,
+                                                                 uint32_t* block_counts) {
+  __shared__ uint32_t wave_sums[kCompactBlock / 64];
+  uint32_t i = blockIdx.x * kCompactBlock + threadIdx.x;
+  uint32_t total;
+  block_exclusive_flags(i < n && mask[i] != 0, &total, wave_sums);
+  if (threadIdx.x == 0) block_counts[blockIdx.x] = total;
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_scan_u32_inplace(uint32_t* v, uint32_t n,
+                                                                   uint32_t* total) {
+  __shared__ uint32_t part[kScanThreads];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t chunk = (n + kScanThreads - 1) / kScanThreads;
+  const uint32_t b = tid * chunk, e = min(n, b + chunk);
+  uint32_t acc = 0;
+  for (uint32_t k = b; k < e; ++k) acc += v[k];
+  part[tid] = acc;
+  __syncthreads();
+  if (tid == 0) {
+    uint32_t run = 0;
+    for (int k = 0; k < kScanThreads; ++k) {
+      uint32_t x = part[k];
+      part[k] = run;
+      run += x;
+    }
+    *total = run;
+  }
+  __syncthreads();
+  uint32_t run = part[tid];
+  for (uint32_t k = b; k < e; ++k) {
+    uint32_t x = v[k];
+    v[k] = run;
+    run += x;
+  }
+}
+
+__global__ void __launch_bounds__(kCompactBlock) k_compact_scatter(const uint8_t* mask, uint32_t n,
+                                                                   const uint32_t* block_off,
+                                                                   uint32_t* list) {
+  __shared__ uint32_t wave_sums[kCompactBlock / 64];
+  uint32_t i = blockIdx.x * kCompactBlock + threadIdx.x;
+  bool f = i < n && mask[i] != 0;
+  uint32_t total;
+  uint32_t at = block_exclusive_flags(f, &total, wave_sums);
+  if (f) list[block_off[blockIdx.x] + at] = i;
+}
+
+hipError_t launch_compact_valid(const uint8_t* mask, uint32_t n, uint32_t* list,
+                                uint32_t* block_tmp, hipStream_t st) {
+  uint32_t nb = compact_blocks(n);
+  if (nb == 0) return launch_fill_u32(list + n, 0u, 1, st);
+  hipLaunchKernelGGL(k_compact_count, dim3(nb), dim3(kCompactBlock), 0, st, mask, n, block_tmp);
+  hipLaunchKernelGGL(k_scan_u32_inplace, dim3(1), dim3(kScanThreads), 0, st, block_tmp, nb,
+                     list + n);
+  hipLaunchKernelGGL(k_compact_scatter, dim3(nb), dim3(kCompactBlock), 0, st, mask, n, block_tmp,
+                     list);
+  return hipGetLastError();
 }
 
 }  // namespace shb

@@ -14,15 +14,11 @@ struct TraceArgs {
   const uint32_t* queue;      // ray slots to trace, or null for 0..count-1
   const uint32_t* count_ptr;  // device count of queue entries, or null
   uint32_t count_const;       // count when count_ptr is null
-  uint32_t* work;             // zeroed work counter for the dynamic fetch
-  uint2* spill;               // per-lane global stack spill (may be null)
-  int spill_cap;              // spill entries per lane
-  int refill;                 // refill a wave when >= this many lanes idle
-  int iters;                  // traversal steps between refill checks
   int has_nodes;              // 0: empty scene
-  uint32_t* error;            // set to 1 on stack overflow
+  uint32_t max_steps;         // per-ray traversal step cap (node count + 1)
+  uint32_t* error;            // set to 1 on stack overflow / step cap
   uint32_t* ray_counts;       // optional: node visits, triangle tests per slot
-  unsigned long long* totals; // optional: summed node visits, triangle tests
+  unsigned long long* totals; // optional: 64 pairs of summed (nodes, tris)
 };
 
 struct BakeArgs {
@@ -51,12 +47,15 @@ struct BakeArgs {
   float* vis_out;             // 1 per supersampled texel
 };
 
-hipError_t launch_trace_hits(const TraceArgs& a, float4* hits, bool count, int grid,
+#ifndef SHB_HOST_ONLY
+// Traversal launches: `capacity` is the largest possible queue length (the
+// grid covers it; threads past the device-side count exit).
+hipError_t launch_trace_hits(const TraceArgs& a, float4* hits, bool count, uint32_t capacity,
                              int block, hipStream_t st);
 hipError_t launch_trace_occluded(const TraceArgs& a, uint32_t* occ, bool count,
-                                 int grid, int block, hipStream_t st);
+                                 uint32_t capacity, int block, hipStream_t st);
 hipError_t launch_trace_bake_shadow(const TraceArgs& a, const BakeArgs& b, bool count,
-                                    int grid, int block, hipStream_t st);
+                                    uint32_t capacity, int block, hipStream_t st);
 hipError_t launch_hit_details(const SceneView& sc, const float4* rays, const float4* hits,
                               uint32_t n, shb_hit* out, hipStream_t st);
 hipError_t launch_generate(const BakeArgs& b, int grid, hipStream_t st);
@@ -66,7 +65,8 @@ hipError_t launch_shade(const BakeArgs& b, int depth, const uint32_t* in_q,
 hipError_t launch_project(const BakeArgs& b, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t st);
 hipError_t launch_fill_f32(float* p, float v, size_t n, hipStream_t st);
-int trace_blocks_per_cu(bool any, int block);
+
+#endif  // SHB_HOST_ONLY
 
 // image / raster kernels (image_kernels.hip)
 struct RasterArgs {
@@ -85,9 +85,10 @@ struct RasterArgs {
   float4* surface;            // 3 float4 per texel (out)
 };
 
-hipError_t launch_raster(const RasterArgs& a, uint32_t* tri_count, void* scan_tmp,
-                         size_t scan_tmp_bytes, hipStream_t st);
-size_t raster_scan_tmp_bytes(uint32_t n);
+#ifndef SHB_HOST_ONLY
+// counts: num_tris + 1 uint32 scratch; scan: num_tris + 1 uint64 scratch.
+hipError_t launch_raster(const RasterArgs& a, uint32_t* counts, unsigned long long* scan,
+                         hipStream_t st);
 hipError_t launch_validity_mask(const float4* surface, size_t n, uint8_t* mask,
                                 hipStream_t st);
 hipError_t launch_downsample(const float* in, int w, int h, int scale, int ch, float* out,
@@ -96,9 +97,13 @@ hipError_t launch_downsample_mask(const uint8_t* in, int w, int h, int scale, ui
                                   hipStream_t st);
 hipError_t launch_dilate(float* a, float* b, uint8_t* ma, uint8_t* mb, int w, int h, int ch,
                          int passes, uint32_t* changed, bool* result_in_a, hipStream_t st);
-size_t compact_tmp_bytes(uint32_t n);
-hipError_t launch_compact_valid(const uint8_t* mask, uint32_t n, uint32_t* list, uint32_t* count,
-                                void* tmp, size_t tmp_bytes, hipStream_t st);
+// Ascending list of the indices whose mask byte is set; block_tmp holds
+// compact_blocks(n) + 1 uint32 (count written to list[n]).
+uint32_t compact_blocks(uint32_t n);
+hipError_t launch_compact_valid(const uint8_t* mask, uint32_t n, uint32_t* list,
+                                uint32_t* block_tmp, hipStream_t st);
+
+#endif  // SHB_HOST_ONLY
 
 }  // namespace shb
 

@@ -4,10 +4,12 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
 
+#include "scene_host.h"
 #include "shb_ctx.h"
 
 using namespace shb;
@@ -28,67 +30,11 @@ int fail(Ctx& c, int code, const std::string& msg) {
   return code;
 }
 
-double now_s() {
-  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
-      .count();
-}
-
-// ------------------------------------------------------ host float math
-// Same evaluation order as the kernels (e0 + (e1 + e2) reductions); this
-// file is compiled with -ffp-contract=off.
-struct H3 {
-  float x, y, z;
-};
-inline H3 h3sub(H3 a, H3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
-inline float h3sq(H3 a) { return a.x * a.x + (a.y * a.y + a.z * a.z); }
-inline H3 h3norm(H3 a) {
-  float z = h3sq(a);
-  if (z > 0.f) {
-    float s = std::sqrt(z);
-    return {a.x / s, a.y / s, a.z / s};
-  }
-  return a;
-}
-inline H3 h3cross(H3 a, H3 b) {
-  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
-}
-inline float rowdot(const float* r, H3 v) { return r[0] * v.x + (r[1] * v.y + r[2] * v.z); }
-
-// Unitary factor of the polar decomposition of the 3x3 part (what Eigen's
-// Transform::rotation() returns), by Newton iteration R <- (R + R^-T)/2.
-void rotation_of(const float* xf, float R[9]) {
-  double m[9] = {xf[0], xf[1], xf[2], xf[4], xf[5], xf[6], xf[8], xf[9], xf[10]};
-  for (int it = 0; it < 64; ++it) {
-    double a = m[0], b = m[1], c = m[2], d = m[3], e = m[4], f = m[5], g = m[6], h = m[7],
-           i = m[8];
-    double A = e * i - f * h, B = -(d * i - f * g), C = d * h - e * g;
-    double det = a * A + b * B + c * C;
-    if (det == 0.0) break;
-    double it9[9] = {A / det,
-                     B / det,
-                     C / det,
-                     -(b * i - c * h) / det,
-                     (a * i - c * g) / det,
-                     -(a * h - b * g) / det,
-                     (b * f - c * e) / det,
-                     -(a * f - c * d) / det,
-                     (a * e - b * d) / det};
-    double diff = 0;
-    for (int k = 0; k < 9; ++k) {
-      double nv = 0.5 * (m[k] + it9[k]);
-      diff = std::max(diff, std::fabs(nv - m[k]));
-      m[k] = nv;
-    }
-    if (diff < 1e-15) break;
-  }
-  for (int k = 0; k < 9; ++k) R[k] = (float)m[k];
-}
-
-// src/colorspace.h:10-14
-float srgb_to_linear(uint8_t x) {
-  float v = x / 255.0f;
-  if (v <= 0.04045f) return v / 12.92f;
-  return std::pow((v + 0.055f) / 1.055f, 2.4f);
+int build_scene(Ctx& c, const shb_scene_desc* d) {
+  std::string err;
+  int rc = build_host_scene(d, &c.hs, &err);
+  if (rc) c.err = err;
+  return rc;
 }
 
 template <class T>
@@ -96,249 +42,33 @@ hipError_t upload(DevBuf& b, const std::vector<T>& v, hipStream_t st) {
   size_t n = std::max<size_t>(v.size() * sizeof(T), 16);
   hipError_t e = b.reserve(n);
   if (e != hipSuccess) return e;
-  if (!v.empty()) return hipMemcpyAsync(b.ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st);
+  if (!v.empty())
+    return hipMemcpyAsync(b.ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st);
   return hipSuccess;
 }
 
-int build_scene(Ctx& c, const shb_scene_desc* d) {
-  HostScene hs;
-  if (!d) return fail(c, SHB_ERR_INVALID_ARG, "scene is NULL");
-  if (d->num_vertices && !d->positions) return fail(c, SHB_ERR_INVALID_ARG, "positions is NULL");
-  if (d->num_indices && !d->indices) return fail(c, SHB_ERR_INVALID_ARG, "indices is NULL");
-  if (d->num_geometries && !d->geometries)
-    return fail(c, SHB_ERR_INVALID_ARG, "geometries is NULL");
-  if (d->num_materials && !d->materials) return fail(c, SHB_ERR_INVALID_ARG, "materials is NULL");
-  if (d->num_lights && !d->lights) return fail(c, SHB_ERR_INVALID_ARG, "lights is NULL");
-  hs.num_vertices = d->num_vertices;
-  // materials (src/scene.h:43-53)
-  for (uint32_t m = 0; m < d->num_materials; ++m) {
-    const shb_material& sm = d->materials[m];
-    MaterialDev md{};
-    md.emission = sm.emission_intensity;
-    if (sm.albedo_channels != 0) {
-      if (sm.albedo_channels < 3 || sm.albedo_channels > 4 || sm.albedo_width == 0 ||
-          sm.albedo_height == 0)
-        return fail(c, SHB_ERR_INVALID_ARG, "material " + std::to_string(m) + ": bad texture");
-      uint64_t nb = (uint64_t)sm.albedo_width * sm.albedo_height * sm.albedo_channels;
-      if (!d->texel_pool || sm.albedo_offset + nb > d->texel_pool_size)
-        return fail(c, SHB_ERR_INVALID_ARG,
-                    "material " + std::to_string(m) + ": texture outside the texel pool");
-      md.tex_w = sm.albedo_width;
-      md.tex_h = sm.albedo_height;
-      md.tex_c = sm.albedo_channels;
-      md.tex_off = sm.albedo_offset;
-      if (sm.albedo_channels == 4) {
-        const uint8_t* px = d->texel_pool + sm.albedo_offset;
-        for (uint64_t k = 3; k < nb; k += 4)
-          if (px[k] < 255) hs.has_transparency = true;
-      }
-    }
-    hs.mats.push_back(md);
-  }
-  if (d->texel_pool_size) hs.texels.assign(d->texel_pool, d->texel_pool + d->texel_pool_size);
-  // geometries -> global triangles
-  std::vector<int> geom_first_tri(d->num_geometries, 0);
-  hs.wpos.assign(3 * (size_t)d->num_vertices, 0.f);
-  hs.wnrm.assign(3 * (size_t)d->num_vertices, 0.f);
-  hs.wtan.assign(4 * (size_t)d->num_vertices, 0.f);
-  hs.luv.assign(2 * (size_t)d->num_vertices, 0.f);
-  for (uint32_t gi = 0; gi < d->num_geometries; ++gi) {
-    const shb_geometry& g = d->geometries[gi];
-    std::string gs = "geometry " + std::to_string(gi);
-    if ((uint64_t)g.vertex_offset + g.vertex_count > d->num_vertices)
-      return fail(c, SHB_ERR_INVALID_ARG, gs + ": vertices out of range");
-    if ((uint64_t)g.index_offset + g.index_count > d->num_indices)
-      return fail(c, SHB_ERR_INVALID_ARG, gs + ": indices out of range");
-    if (g.material_id < 0 || (uint32_t)g.material_id >= d->num_materials)
-      return fail(c, SHB_ERR_INVALID_ARG, gs + ": material_id out of range");
-    bool hasN = (g.flags & SHB_GEOM_HAS_NORMALS) && d->normals;
-    bool hasUV = (g.flags & SHB_GEOM_HAS_TEXTURE_UVS) && d->texture_uvs;
-    bool hasL = (g.flags & SHB_GEOM_HAS_LIGHTMAP_UVS) && d->lightmap_uvs;
-    bool hasT = (g.flags & SHB_GEOM_HAS_TANGENTS) && d->tangents;
-    if (!hasL || !hasN || !hasT) hs.raster_ready = false;
-    float R[9];
-    rotation_of(g.transform, R);
-    // world attributes (src/scene.cpp:162-195)
-    for (uint32_t k = 0; k < g.vertex_count; ++k) {
-      uint32_t vi = g.vertex_offset + k;
-      H3 p = {d->positions[3 * vi], d->positions[3 * vi + 1], d->positions[3 * vi + 2]};
-      hs.wpos[3 * vi] = rowdot(g.transform, p) + g.transform[3];
-      hs.wpos[3 * vi + 1] = rowdot(g.transform + 4, p) + g.transform[7];
-      hs.wpos[3 * vi + 2] = rowdot(g.transform + 8, p) + g.transform[11];
-      if (hasN) {
-        H3 n = {d->normals[3 * vi], d->normals[3 * vi + 1], d->normals[3 * vi + 2]};
-        H3 wn = h3norm(H3{rowdot(R, n), rowdot(R + 3, n), rowdot(R + 6, n)});
-        hs.wnrm[3 * vi] = wn.x;
-        hs.wnrm[3 * vi + 1] = wn.y;
-        hs.wnrm[3 * vi + 2] = wn.z;
-      }
-      if (hasT) {
-        H3 t = {d->tangents[4 * vi], d->tangents[4 * vi + 1], d->tangents[4 * vi + 2]};
-        H3 wt = h3norm(H3{rowdot(R, t), rowdot(R + 3, t), rowdot(R + 6, t)});
-        hs.wtan[4 * vi] = wt.x;
-        hs.wtan[4 * vi + 1] = wt.y;
-        hs.wtan[4 * vi + 2] = wt.z;
-        hs.wtan[4 * vi + 3] = d->tangents[4 * vi + 3];
-      }
-      if (hasL) {
-        hs.luv[2 * vi] = d->lightmap_uvs[2 * vi];
-        hs.luv[2 * vi + 1] = d->lightmap_uvs[2 * vi + 1];
-      }
-    }
-    uint32_t nt = g.index_count / 3;
-    geom_first_tri[gi] = (int)hs.num_tris;
-    for (uint32_t t = 0; t < nt; ++t) {
-      uint32_t li[3];
-      for (int k = 0; k < 3; ++k) {
-        li[k] = d->indices[g.index_offset + 3 * t + k];
-        if (li[k] >= g.vertex_count)
-          return fail(c, SHB_ERR_INVALID_ARG, gs + ": index out of range");
-      }
-      uint32_t gv[3] = {g.vertex_offset + li[0], g.vertex_offset + li[1], g.vertex_offset + li[2]};
-      H3 w[3];
-      for (int k = 0; k < 3; ++k) {
-        w[k] = {hs.wpos[3 * gv[k]], hs.wpos[3 * gv[k] + 1], hs.wpos[3 * gv[k] + 2]};
-        hs.world_xyz.push_back(w[k].x);
-        hs.world_xyz.push_back(w[k].y);
-        hs.world_xyz.push_back(w[k].z);
-        hs.tri_vidx.push_back(gv[k]);
-      }
-      hs.tri_mat.push_back(g.material_id);
-      // shading record (src/occlusion.cpp:63-93: object-space normals)
-      float n[9] = {0}, uv[6] = {0};
-      uint32_t flags = 0;
-      if (hasN) {
-        flags |= 1u;
-        for (int k = 0; k < 3; ++k)
-          for (int a = 0; a < 3; ++a) n[3 * k + a] = d->normals[3 * gv[k] + a];
-      } else {
-        H3 ng = h3norm(h3cross(h3sub(w[1], w[0]), h3sub(w[2], w[0])));
-        n[0] = ng.x;
-        n[1] = ng.y;
-        n[2] = ng.z;
-      }
-      if (hasUV) {
-        flags |= 2u;
-        for (int k = 0; k < 3; ++k) {
-          uv[2 * k] = d->texture_uvs[2 * gv[k]];
-          uv[2 * k + 1] = d->texture_uvs[2 * gv[k] + 1];
-        }
-      }
-      hs.shade.push_back(make_float4(n[0], n[1], n[2], uv[0]));
-      hs.shade.push_back(make_float4(n[3], n[4], n[5], uv[1]));
-      hs.shade.push_back(make_float4(n[6], n[7], n[8], uv[2]));
-      hs.shade.push_back(make_float4(uv[3], uv[4], uv[5], as_float(flags)));
-      hs.tri_info.push_back(make_int4(g.material_id, -1, (int)gi, (int)t));
-      hs.num_tris++;
-    }
-  }
-  // lights (src/scene.h:70-87, src/light.h, src/light.cpp:16-78)
-  for (uint32_t li = 0; li < d->num_lights; ++li) {
-    const shb_light& sl = d->lights[li];
-    LightDev L{};
-    L.type = sl.type;
-    if (sl.type < SHB_LIGHT_POINT || sl.type > SHB_LIGHT_AREA)
-      return fail(c, SHB_ERR_INVALID_ARG, "light " + std::to_string(li) + ": bad type");
-    for (int a = 0; a < 3; ++a) {
-      L.ic[a] = sl.intensity * sl.color[a];
-      L.pos[a] = sl.position[a];
-      L.dir[a] = sl.direction[a];
-    }
-    L.cos_inner = sl.cos_inner_cone;
-    L.cos_outer = sl.cos_outer_cone;
-    L.material = -1;
-    if (sl.type == SHB_LIGHT_AREA) {
-      int gi = sl.geometry_index, mi = sl.material_index;
-      if (gi >= 0 && (uint32_t)gi < d->num_geometries && mi >= 0 &&
-          (uint32_t)mi < d->num_materials && d->geometries[gi].index_count >= 3) {
-        const shb_geometry& g = d->geometries[gi];
-        uint32_t nt = g.index_count / 3;
-        L.first_tri = (uint32_t)hs.area_tris.size();
-        L.num_tris = nt;
-        L.material = mi;
-        bool hasUV = (g.flags & SHB_GEOM_HAS_TEXTURE_UVS) && d->texture_uvs;
-        for (uint32_t t = 0; t < nt; ++t) {
-          AreaTri at{};
-          H3 v[3];
-          for (int k = 0; k < 3; ++k) {
-            uint32_t vi = g.vertex_offset + d->indices[g.index_offset + 3 * t + k];
-            v[k] = {d->positions[3 * vi], d->positions[3 * vi + 1], d->positions[3 * vi + 2]};
-            if (hasUV) {
-              at.uv[2 * k] = d->texture_uvs[2 * vi];
-              at.uv[2 * k + 1] = d->texture_uvs[2 * vi + 1];
-            }
-          }
-          at.v0[0] = v[0].x, at.v0[1] = v[0].y, at.v0[2] = v[0].z;
-          at.v1[0] = v[1].x, at.v1[1] = v[1].y, at.v1[2] = v[1].z;
-          at.v2[0] = v[2].x, at.v2[1] = v[2].y, at.v2[2] = v[2].z;
-          float area = std::sqrt(h3sq(h3cross(h3sub(v[0], v[1]), h3sub(v[0], v[2])))) / 2.f;
-          at.pdf = std::max(1e-6f, 1.f / (float)nt * 1.f / area);
-          at.has_uv = hasUV ? 1.f : 0.f;
-          hs.area_tris.push_back(at);
-        }
-        const MaterialDev& md = hs.mats[mi];
-        if (md.tex_c == 0 || (md.tex_w == 1 && md.tex_h == 1)) {
-          float alb[3] = {0.8f, 0.8f, 0.8f};
-          if (md.tex_c != 0)
-            for (int a = 0; a < 3; ++a) alb[a] = srgb_to_linear(hs.texels[md.tex_off + a]);
-          L.emit_const = 1;
-          for (int a = 0; a < 3; ++a)
-            L.emit[a] = md.emission > 0.0f ? alb[a] * md.emission : 0.f;
-        }
-        // triangles of the emitting geometry carry the light for the
-        // emission they show when hit directly
-        for (uint32_t t = 0; t < nt; ++t) hs.tri_info[geom_first_tri[gi] + t].y = (int)li;
-      }
-    }
-    hs.lights.push_back(L);
-  }
-  c.hs = std::move(hs);
-  return SHB_OK;
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
+      .count();
 }
 
 int upload_bvh(Ctx& c) {
   build_cwbvh(c.hs.world_xyz.data(), c.hs.num_tris, c.tuning.bvh_width, &c.bvh);
-  if (c.bvh.max_depth - 1 > kStackTotal)
+  // the group stack holds at most one entry per level below the root
+  if (c.bvh.max_depth - 1 > kStackLds)
     return fail(c, SHB_ERR_LIMIT,
-                "BVH depth " + std::to_string(c.bvh.max_depth) + " exceeds the traversal stack");
+                "BVH depth " + std::to_string(c.bvh.max_depth) + " exceeds the traversal stack (" +
+                    std::to_string(kStackLds + 1) + " levels)");
   SHB_TRY(c, upload(c.d_nodes, c.bvh.nodes, c.stream));
   SHB_TRY(c, upload(c.d_tris, c.bvh.tris, c.stream));
-  return SHB_OK;
-}
-
-int spill_cap(const Ctx& c) { return std::max(0, c.bvh.max_depth - 1 - kStackLds); }
-
-int ensure_trace_occupancy(Ctx& c) {
-  if (c.trace_bpc_block != c.tuning.block_size) {
-    c.trace_bpc[0] = trace_blocks_per_cu(false, c.tuning.block_size);
-    c.trace_bpc[1] = trace_blocks_per_cu(true, c.tuning.block_size);
-    c.trace_bpc_block = c.tuning.block_size;
-  }
-  return SHB_OK;
-}
-
-// Grid of the persistent traversal kernels and its stack spill area.
-int trace_grid(Ctx& c, bool any, int* grid) {
-  ensure_trace_occupancy(c);
-  int g = c.num_cus * std::max(c.trace_bpc[0], c.trace_bpc[1]);
-  *grid = std::max(1, g);
-  (void)any;
-  int cap = spill_cap(c);
-  if (cap > 0) {
-    size_t need = (size_t)(*grid) * c.tuning.block_size * cap * sizeof(uint2);
-    SHB_TRY(c, c.d_spill.reserve(need));
-  }
   return SHB_OK;
 }
 
 TraceArgs trace_args(Ctx& c) {
   TraceArgs a{};
   a.scene = c.view();
-  a.spill = spill_cap(c) > 0 ? c.d_spill.as<uint2>() : nullptr;
-  a.spill_cap = spill_cap(c);
-  a.refill = c.tuning.refill_threshold;
-  a.iters = c.iters;
   a.has_nodes = c.bvh.nodes.empty() ? 0 : 1;
+  a.max_steps = (uint32_t)c.bvh.nodes.size() + 1;
   a.error = c.d_work.as<uint32_t>();
   return a;
 }
@@ -353,7 +83,33 @@ int check_overflow(Ctx& c) {
 
 // Counter layout per batch (uint32): extension-queue counts by depth, shadow
 // counts by depth, work counters of the extension and shadow traversals.
-constexpr int kCntExt = 0, kCntSh = 20, kCntWExt = 40, kCntWSh = 60, kCntStride = 80;
+constexpr int kCntExt = 0, kCntSh = 20, kCntStride = 40;
+
+// SHB_DEBUG_STAGES=1: synchronise after every bake stage and log its time to
+// stderr (diagnostics; off by default).
+bool debug_stages() {
+  static int v = -1;
+  if (v < 0) {
+    const char* e = getenv("SHB_DEBUG_STAGES");
+    v = (e && e[0] == '1') ? 1 : 0;
+  }
+  return v == 1;
+}
+
+hipError_t stage_done(hipStream_t st, const char* name, uint32_t batch, int depth) {
+  if (!debug_stages()) return hipSuccess;
+  static double last = 0;
+  hipError_t e = hipStreamSynchronize(st);
+  double t = std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+  fprintf(stderr, "[shb] batch %u depth %d %-8s %.3f ms (%s)\n", batch, depth, name,
+          last > 0 ? (t - last) * 1e3 : 0.0, hipGetErrorString(e));
+  fflush(stderr);
+  last = t;
+  return e;
+}
+
+// d_work layout: [0] error flag, [4..4+256) 64 pairs of u64 visit totals.
+constexpr size_t kWorkBytes = 16 + 64 * 16;
 
 struct BakeRun {
   uint64_t rays_ext = 0, rays_sh = 0, valid = 0, node_visits = 0, tri_tests = 0;
@@ -391,13 +147,13 @@ int bake_device(Ctx& c, const shb_bake_cfg* cfg, uint32_t job, BakeRun* run) {
   // valid texel list
   SHB_TRY(c, c.d_mask.reserve(std::max<size_t>(npx, 16)));
   SHB_TRY(c, c.d_list.reserve(std::max<size_t>(npx * 4 + 16, 64)));
-  size_t tmpb = compact_tmp_bytes((uint32_t)npx);
-  SHB_TRY(c, c.d_tmp.reserve(std::max<size_t>(tmpb, 16)));
+  SHB_TRY(c, c.d_tmp.reserve(((size_t)compact_blocks((uint32_t)npx) + 1) * 4 + 64));
   SHB_TRY(c, launch_validity_mask(c.d_surface.as<float4>(), npx, c.d_mask.as<uint8_t>(), st));
   uint32_t* d_nvalid = c.d_list.as<uint32_t>() + npx;
   SHB_TRY(c, launch_compact_valid(c.d_mask.as<uint8_t>(), (uint32_t)npx, c.d_list.as<uint32_t>(),
-                                  d_nvalid, c.d_tmp.ptr, tmpb, st));
+                                  c.d_tmp.as<uint32_t>(), st));
   uint32_t V = 0;
+  SHB_TRY(c, stage_done(st, "compact", 0, -1));
   SHB_TRY(c, hipMemcpyAsync(&V, d_nvalid, 4, hipMemcpyDeviceToHost, st));
   SHB_TRY(c, hipStreamSynchronize(st));
   run->valid = V;
@@ -411,12 +167,9 @@ int bake_device(Ctx& c, const shb_bake_cfg* cfg, uint32_t job, BakeRun* run) {
   SHB_TRY(c, c.d_paths.reserve((size_t)P * 13 * sizeof(float4)));
   SHB_TRY(c, c.d_queues.reserve((size_t)P * 4 * sizeof(uint32_t)));
   SHB_TRY(c, c.d_counters.reserve(std::max<size_t>((size_t)nb * kCntStride * 4, 64)));
-  SHB_TRY(c, c.d_work.reserve(64));
+  SHB_TRY(c, c.d_work.reserve(kWorkBytes));
   SHB_TRY(c, launch_fill_u32(c.d_counters.as<uint32_t>(), 0u, (size_t)nb * kCntStride, st));
-  SHB_TRY(c, launch_fill_u32(c.d_work.as<uint32_t>(), 0u, 16, st));
-  int grid = 1;
-  int rc = trace_grid(c, false, &grid);
-  if (rc) return rc;
+  SHB_TRY(c, launch_fill_u32(c.d_work.as<uint32_t>(), 0u, kWorkBytes / 4, st));
   float4* base = c.d_paths.as<float4>();
   BakeArgs b{};
   b.scene = c.view();
@@ -441,7 +194,7 @@ int bake_device(Ctx& c, const shb_bake_cfg* cfg, uint32_t job, BakeRun* run) {
   b.shadow_queue = q + 2 * (size_t)P;
   b.sh_out = c.d_sh.as<float>();
   b.vis_out = c.d_vis.as<float>();
-  unsigned long long* totals = reinterpret_cast<unsigned long long*>(c.d_work.as<uint32_t>() + 2);
+  unsigned long long* totals = reinterpret_cast<unsigned long long*>(c.d_work.as<uint32_t>() + 4);
   std::vector<hipEvent_t> tev;
   auto trace_ev = [&]() -> hipEvent_t {
     hipEvent_t e;
@@ -451,37 +204,42 @@ int bake_device(Ctx& c, const shb_bake_cfg* cfg, uint32_t job, BakeRun* run) {
     return e;
   };
   const int sgrid = std::max(1, std::min<int>((P + 255) / 256, c.num_cus * 16));
+  (void)sgrid;
   for (uint32_t bi = 0; bi < nb; ++bi) {
     uint32_t* cnt = c.d_counters.as<uint32_t>() + (size_t)bi * kCntStride;
     b.texel_begin = bi * T;
     b.texels_in_batch = std::min(T, V - bi * T);
     b.shadow_count = cnt + kCntSh;
     SHB_TRY(c, launch_generate(b, sgrid, st));
+    SHB_TRY(c, stage_done(st, "generate", bi, -1));
     for (uint32_t d = 0; d <= B; ++d) {
       TraceArgs ta = trace_args(c);
       ta.rays = b.ray;
       ta.queue = d == 0 ? nullptr : qext[d & 1];
       ta.count_ptr = d == 0 ? nullptr : cnt + kCntExt + d;
       ta.count_const = b.texels_in_batch * spp;
-      ta.work = cnt + kCntWExt + d;
       ta.totals = count ? totals : nullptr;
       trace_ev();
-      SHB_TRY(c, launch_trace_hits(ta, b.hit, count, grid, c.tuning.block_size, st));
+      SHB_TRY(c, launch_trace_hits(ta, b.hit, count, b.texels_in_batch * spp, c.tuning.block_size, st));
       trace_ev();
+      SHB_TRY(c, stage_done(st, "extend", bi, (int)d));
       b.shadow_count = cnt + kCntSh + d;
       SHB_TRY(c, launch_shade(b, (int)d, d == 0 ? nullptr : qext[d & 1], cnt + kCntExt + d,
                               qext[(d + 1) & 1], cnt + kCntExt + d + 1, sgrid, st));
+      SHB_TRY(c, stage_done(st, "shade", bi, (int)d));
       TraceArgs sa = trace_args(c);
       sa.rays = b.sray;
       sa.queue = b.shadow_queue;
       sa.count_ptr = cnt + kCntSh + d;
-      sa.work = cnt + kCntWSh + d;
       sa.totals = count ? totals : nullptr;
       trace_ev();
-      SHB_TRY(c, launch_trace_bake_shadow(sa, b, count, grid, c.tuning.block_size, st));
+      SHB_TRY(c, launch_trace_bake_shadow(sa, b, count, 2 * b.texels_in_batch * spp,
+                                          c.tuning.block_size, st));
       trace_ev();
+      SHB_TRY(c, stage_done(st, "shadow", bi, (int)d));
     }
     SHB_TRY(c, launch_project(b, st));
+    SHB_TRY(c, stage_done(st, "project", bi, -1));
   }
   SHB_TRY(c, hipEventRecord(e1, st));
   SHB_TRY(c, hipEventSynchronize(e1));
@@ -510,14 +268,15 @@ int bake_device(Ctx& c, const shb_bake_cfg* cfg, uint32_t job, BakeRun* run) {
     for (uint32_t d = 1; d <= B; ++d) run->rays_ext += k[kCntExt + d];
     for (uint32_t d = 0; d <= B; ++d) run->rays_sh += k[kCntSh + d];
   }
-  uint32_t hw[6] = {0};
-  SHB_TRY(c, hipMemcpy(hw, c.d_work.ptr, sizeof(hw), hipMemcpyDeviceToHost));
-  if (hw[0]) return fail(c, SHB_ERR_LIMIT, "traversal stack overflow");
+  std::vector<uint32_t> hw(kWorkBytes / 4);
+  SHB_TRY(c, hipMemcpy(hw.data(), c.d_work.ptr, kWorkBytes, hipMemcpyDeviceToHost));
+  if (hw[0]) return fail(c, SHB_ERR_LIMIT, "traversal stack overflow or corrupt BVH");
   if (count) {
-    unsigned long long t2[2];
-    std::memcpy(t2, hw + 2, sizeof(t2));
-    run->node_visits = t2[0];
-    run->tri_tests = t2[1];
+    const unsigned long long* t2 = reinterpret_cast<const unsigned long long*>(hw.data() + 4);
+    for (int k = 0; k < 64; ++k) {
+      run->node_visits += t2[2 * k];
+      run->tri_tests += t2[2 * k + 1];
+    }
   }
   return SHB_OK;
 }
@@ -578,7 +337,7 @@ void shb_destroy(shb_ctx* h) {
                     &c.d_texels, &c.d_lut,      &c.d_lights,  &c.d_area,     &c.d_lscale,
                     &c.d_ascale, &c.d_luv,      &c.d_wpos,    &c.d_wnrm,     &c.d_wtan,
                     &c.d_tri_vidx, &c.d_tri_mat, &c.d_surface, &c.d_paths,   &c.d_queues,
-                    &c.d_counters, &c.d_spill,  &c.d_work,    &c.d_list,     &c.d_mask,
+                    &c.d_counters, &c.d_work,    &c.d_list,     &c.d_mask,
                     &c.d_tmp,    &c.d_sh,       &c.d_vis,     &c.d_img_a,    &c.d_img_b,
                     &c.d_mask_a, &c.d_mask_b,   &c.d_rays,    &c.d_hits,     &c.d_hit_out,
                     &c.d_raycounts, &c.d_adj};
@@ -666,8 +425,7 @@ int shb_rasterize(shb_ctx* h, const shb_raster_cfg* cfg, shb_surface_point* out)
   SHB_TRY(c, c.d_surface.reserve(npx * 3 * sizeof(float4)));
   SHB_TRY(c, c.d_img_a.reserve(npx * 4 * 2 + 16));  // win + fb
   SHB_TRY(c, c.d_list.reserve(std::max<size_t>((size_t)(n + 1) * 8, 64)));  // tri_hit + counts
-  size_t cub = raster_scan_tmp_bytes(n);
-  SHB_TRY(c, c.d_tmp.reserve(((cub + 255) & ~size_t(255)) + (size_t)(n + 1) * 16 + 256));
+  SHB_TRY(c, c.d_tmp.reserve((size_t)(n + 1) * 8 + 64));  // scan
   RasterArgs a{};
   a.luv = c.d_luv.as<float>();
   a.wpos = c.d_wpos.as<float>();
@@ -683,7 +441,7 @@ int shb_rasterize(shb_ctx* h, const shb_raster_cfg* cfg, shb_surface_point* out)
   a.tri_hit = c.d_list.as<uint32_t>();
   a.surface = c.d_surface.as<float4>();
   uint32_t* counts = a.tri_hit + (n + 1);
-  SHB_TRY(c, launch_raster(a, counts, c.d_tmp.ptr, c.d_tmp.bytes, c.stream));
+  SHB_TRY(c, launch_raster(a, counts, c.d_tmp.as<unsigned long long>(), c.stream));
   c.width = cfg->width;
   c.height = cfg->height;
   c.ss = cfg->supersample_scale;
@@ -894,23 +652,19 @@ int shb_trace_rays(shb_ctx* h, const shb_ray* rays, uint32_t n, int32_t any_hit,
   SHB_TRY(c, c.d_rays.reserve((size_t)n * sizeof(shb_ray)));
   SHB_TRY(c, c.d_hits.reserve((size_t)n * sizeof(float4)));
   SHB_TRY(c, c.d_hit_out.reserve((size_t)n * sizeof(shb_hit)));
-  SHB_TRY(c, c.d_work.reserve(64));
+  SHB_TRY(c, c.d_work.reserve(kWorkBytes));
   const bool count = node_visits || tri_tests;
   if (count) SHB_TRY(c, c.d_raycounts.reserve((size_t)n * 8));
   SHB_TRY(c, hipMemcpyAsync(c.d_rays.ptr, rays, (size_t)n * sizeof(shb_ray), hipMemcpyHostToDevice, st));
-  SHB_TRY(c, launch_fill_u32(c.d_work.as<uint32_t>(), 0u, 16, st));
-  int grid = 1;
-  int rc = trace_grid(c, any_hit != 0, &grid);
-  if (rc) return rc;
+  SHB_TRY(c, launch_fill_u32(c.d_work.as<uint32_t>(), 0u, kWorkBytes / 4, st));
   TraceArgs ta = trace_args(c);
   ta.rays = c.d_rays.as<float4>();
   ta.queue = nullptr;
   ta.count_ptr = nullptr;
   ta.count_const = n;
-  ta.work = c.d_work.as<uint32_t>() + 1;
   ta.ray_counts = count ? c.d_raycounts.as<uint32_t>() : nullptr;
   if (any_hit) {
-    SHB_TRY(c, launch_trace_occluded(ta, c.d_hits.as<uint32_t>(), count, grid, c.tuning.block_size, st));
+    SHB_TRY(c, launch_trace_occluded(ta, c.d_hits.as<uint32_t>(), count, n, c.tuning.block_size, st));
     std::vector<uint32_t> occ(n);
     SHB_TRY(c, hipMemcpyAsync(occ.data(), c.d_hits.ptr, (size_t)n * 4, hipMemcpyDeviceToHost, st));
     SHB_TRY(c, hipStreamSynchronize(st));
@@ -921,7 +675,7 @@ int shb_trace_rays(shb_ctx* h, const shb_ray* rays, uint32_t n, int32_t any_hit,
       hits[i].material_id = -1;
     }
   } else {
-    SHB_TRY(c, launch_trace_hits(ta, c.d_hits.as<float4>(), count, grid, c.tuning.block_size, st));
+    SHB_TRY(c, launch_trace_hits(ta, c.d_hits.as<float4>(), count, n, c.tuning.block_size, st));
     SHB_TRY(c, launch_hit_details(c.view(), c.d_rays.as<float4>(), c.d_hits.as<float4>(), n,
                                   c.d_hit_out.as<shb_hit>(), st));
     SHB_TRY(c, hipMemcpyAsync(hits, c.d_hit_out.ptr, (size_t)n * sizeof(shb_hit), hipMemcpyDeviceToHost, st));
@@ -944,8 +698,7 @@ int shb_set_tuning(shb_ctx* h, const shb_tuning* t) {
   if (t->block_size != 64 && t->block_size != 128 && t->block_size != 256)
     return fail(c, SHB_ERR_INVALID_ARG, "block_size must be 64, 128 or 256");
   if (t->batch_paths < 1024) return fail(c, SHB_ERR_INVALID_ARG, "batch_paths must be >= 1024");
-  if (t->refill_threshold < 1 || t->refill_threshold > 64)
-    return fail(c, SHB_ERR_INVALID_ARG, "refill_threshold must be in [1, 64]");
+  if (t->reserved != 0) return fail(c, SHB_ERR_INVALID_ARG, "reserved must be 0");
   if (t->bvh_width < 2 || t->bvh_width > 8)
     return fail(c, SHB_ERR_INVALID_ARG, "bvh_width must be in [2, 8]");
   bool rebuild = c.has_scene && t->bvh_width != c.tuning.bvh_width;

@@ -6,8 +6,33 @@
 #ifndef SHB_COMMON_H_
 #define SHB_COMMON_H_
 
-#include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#ifdef SHB_HOST_ONLY
+// Host-only build of the traversal / shading code (CPU checks of the exact
+// device arithmetic): minimal stand-ins for the HIP vector types.
+#include <cmath>
+#include <cstring>
+#define __host__
+#define __device__
+#define __forceinline__ inline
+struct float2 { float x, y; };
+struct float4 { float x, y, z, w; };
+struct uint2 { uint32_t x, y; };
+struct uint4 { uint32_t x, y, z, w; };
+struct int4 { int32_t x, y, z, w; };
+inline float4 make_float4(float x, float y, float z, float w) { return {x, y, z, w}; }
+inline uint2 make_uint2(uint32_t x, uint32_t y) { return {x, y}; }
+inline int4 make_int4(int x, int y, int z, int w) { return {x, y, z, w}; }
+#include <math.h>
+#include <algorithm>
+using std::max;
+using std::min;
+using std::signbit;
+inline uint32_t __builtin_popcount_u(uint32_t x) { return (uint32_t)__builtin_popcount(x); }
+#else
+#include <hip/hip_runtime.h>
+#endif
 
 #define SHB_HD __host__ __device__ __forceinline__
 
@@ -16,8 +41,7 @@ namespace shb {
 // ------------------------------------------------------------ limits
 constexpr int kMaxBounces = 15;       // depth fits 4 bits of the RNG counter
 constexpr int kMaxSamples = 65535;    // sample fits 16 bits of the counter
-constexpr int kStackLds = 16;         // traversal stack entries kept in LDS
-constexpr int kStackTotal = 64;       // entries incl. the global spill area
+constexpr int kStackLds = 16;         // traversal stack entries (LDS)
 constexpr float kRayEps = 0.001f;     // baker.cpp:118, 122, light.h:40,83
 
 // ------------------------------------------------------------ BVH node

@@ -7,6 +7,7 @@
 
 #include "../../include/shb.h"
 #include "bvh_build.h"
+#include "scene_host.h"
 #include "kernels.h"
 
 namespace shb {
@@ -35,31 +36,12 @@ struct DevBuf {
   }
 };
 
-// Host copy of the parts of the scene the kernels need, kept so the BVH can
-// be rebuilt with another width and the raster inputs re-made.
-struct HostScene {
-  uint32_t num_vertices = 0, num_tris = 0;
-  std::vector<float> world_xyz;      // 9 per triangle (BVH input)
-  std::vector<float> wpos, wnrm, wtan, luv;  // per vertex (raster)
-  std::vector<uint32_t> tri_vidx;    // 3 global vertex ids per triangle
-  std::vector<int32_t> tri_mat;
-  std::vector<float4> shade;         // 4 per triangle
-  std::vector<int4> tri_info;
-  std::vector<MaterialDev> mats;
-  std::vector<uint8_t> texels;
-  std::vector<LightDev> lights;
-  std::vector<AreaTri> area_tris;
-  bool raster_ready = true;          // every geometry has luv, normals, tangents
-  bool has_transparency = false;     // some RGBA texel with alpha < 255
-};
-
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   int num_cus = 0;
   std::string err;
-  shb_tuning tuning{256, 1 << 23, 16, 8};
-  int iters = 8;
+  shb_tuning tuning{256, 1 << 23, 0, 8};
 
   // scene
   bool has_scene = false;
@@ -77,15 +59,12 @@ struct Ctx {
   DevBuf d_paths;    // path state arrays
   DevBuf d_queues;
   DevBuf d_counters;
-  DevBuf d_spill;
   DevBuf d_work;     // misc small device scratch (error flags, totals)
   DevBuf d_list, d_mask, d_tmp;
   DevBuf d_sh, d_vis;         // supersampled outputs
   DevBuf d_img_a, d_img_b, d_mask_a, d_mask_b;
   DevBuf d_rays, d_hits, d_hit_out, d_raycounts;
   DevBuf d_adj;               // adjoint accumulators
-  int trace_bpc[2] = {0, 0};  // blocks per CU (closest, any) at tuning.block_size
-  int trace_bpc_block = 0;
 
   SceneView view() const {
     SceneView v;

@@ -2,10 +2,10 @@
 //
 // Per lane: one ray, a current "node group" (child base index + 8-bit hit
 // mask in octant order + the parent's internal-child mask) and a stack of
-// groups whose bottom kStackLds entries live in LDS (laid out entry-major,
-// lane-minor, so a wave's accesses hit 64 distinct banks) and whose deeper
-// entries spill to a per-lane global area. Triangles hit in a node are tested
-// as soon as the node is decoded. Ray/triangle: the watertight test of Woop,
+// up to kStackLds groups in LDS (laid out entry-major, lane-minor, so a
+// wave's accesses hit 64 distinct banks); the group stack never holds more
+// than one entry per tree level, and the upload rejects deeper trees.
+// Triangles hit in a node are tested as soon as the node is decoded. Ray/triangle: the watertight test of Woop,
 // Benthin, Wald (JCGT 2013), double sided, ties in t resolved to the lowest
 // triangle id so the closest hit does not depend on traversal order.
 #ifndef SHB_TRAVERSE_H_
@@ -14,10 +14,6 @@
 #include "shb_common.h"
 
 namespace shb {
-
-// Far-plane inflation 1 + 2^-21 against rounding in the slab test (after
-// Ize, "Robust BVH Ray Traversal", JCGT 2013).
-constexpr float kFarInflate = 1.00000048f;
 
 struct TravRay {
   float ox, oy, oz;
@@ -28,12 +24,20 @@ struct TravRay {
   float Sx, Sy, Sz;
   uint32_t sstar;       // nearest octant slot
   uint32_t negx, negy, negz;
+  uint32_t zx, zy, zz;  // direction component below 1e-20 in magnitude
 };
 
 __device__ __forceinline__ float safe_inv(float d) {
   float ad = fabsf(d);
   float r = 1.0f / d;
   return ad < 1e-20f ? (d < 0.f || (d == 0.f && signbit(d)) ? -1e20f : 1e20f) : r;
+}
+
+// Finite origin and a finite, non-zero direction (anything else cannot hit).
+__device__ __forceinline__ bool ray_is_finite(float4 o, float4 d) {
+  float sum = o.x + o.y + o.z + d.x + d.y + d.z;
+  float dd = d.x * d.x + d.y * d.y + d.z * d.z;
+  return (sum - sum) == 0.0f && dd > 0.0f && (dd - dd) == 0.0f;
 }
 
 __device__ __forceinline__ float sel3(int k, float x, float y, float z) {
@@ -51,6 +55,9 @@ __device__ __forceinline__ void trav_setup(TravRay& r, float4 o, float4 d) {
   r.ix = safe_inv(d.x);
   r.iy = safe_inv(d.y);
   r.iz = safe_inv(d.z);
+  r.zx = fabsf(d.x) < 1e-20f;
+  r.zy = fabsf(d.y) < 1e-20f;
+  r.zz = fabsf(d.z) < 1e-20f;
   r.negx = d.x < 0.f;
   r.negy = d.y < 0.f;
   r.negz = d.z < 0.f;
@@ -131,14 +138,15 @@ __device__ __forceinline__ void trav_init(TravState& s, const TravRay& r, float 
 
 // One traversal step: visit one node and test its triangles. Returns true
 // when the ray is finished (stack empty, or any-hit found).
-template <bool ANY, bool COUNT>
+// Stack: any type with get(int) / set(int, uint2) over kStackLds entries
+// (on the GPU a slice of an LDS array, on the host a local array).
+template <bool ANY, bool COUNT, class Stack>
 __device__ __forceinline__ bool trav_step(const SceneView& sc, const TravRay& r,
-                                          TravState& s, uint2* lds, int lds_stride,
-                                          uint2* spill, bool& overflow, int stack_cap) {
+                                          TravState& s, Stack& stack, bool& overflow) {
   if ((s.gbits & 0xffu) == 0u) {
     if (s.sp == 0) return true;
     --s.sp;
-    uint2 e = s.sp < kStackLds ? lds[s.sp * lds_stride] : spill[s.sp - kStackLds];
+    uint2 e = stack.get(s.sp);
     s.gbase = e.x;
     s.gbits = e.y;
   }
@@ -149,14 +157,11 @@ __device__ __forceinline__ bool trav_step(const SceneView& sc, const TravRay& r,
   uint32_t im = (s.gbits >> 8) & 0xffu;
   uint32_t ni = s.gbase + __builtin_popcount(im & ((1u << slot) - 1u));
   if ((s.gbits & 0xffu) != 0u) {
-    if (s.sp < kStackLds) {
-      lds[s.sp * lds_stride] = make_uint2(s.gbase, s.gbits);
-    } else if (s.sp < stack_cap) {
-      spill[s.sp - kStackLds] = make_uint2(s.gbase, s.gbits);
-    } else {
+    if (s.sp >= kStackLds) {  // deeper than the BVH the upload accepted
       overflow = true;
       return true;
     }
+    stack.set(s.sp, make_uint2(s.gbase, s.gbits));
     ++s.sp;
   }
   // ---- decode node ni
@@ -172,6 +177,18 @@ __device__ __forceinline__ bool trav_step(const SceneView& sc, const TravRay& r,
   float ax = sx * r.ix, bx = (px - r.ox) * r.ix;
   float ay = sy * r.iy, by = (py - r.oy) * r.iy;
   float az = sz * r.iz, bz = (pz - r.oz) * r.iz;
+  // Per-axis absolute bound on the rounding of fmaf(q, a, b) against the
+  // exact plane distance (p + q s - o) / d: 16 ulps of (|b| + 256 |a|).
+  // Near planes move down and far planes up by it, so no box holding a hit
+  // in [tnear, tbest] is culled and closest-hit ties resolve as in a
+  // brute-force search. Axes the ray is parallel to (d = 0, 1/d replaced by
+  // +-1e20) keep an exact sign test.
+  float ex = r.zx ? 0.f : (fabsf(bx) + 256.0f * fabsf(ax)) * 0x1p-20f;
+  float ey = r.zy ? 0.f : (fabsf(by) + 256.0f * fabsf(ay)) * 0x1p-20f;
+  float ez = r.zz ? 0.f : (fabsf(bz) + 256.0f * fabsf(az)) * 0x1p-20f;
+  const float bxn = bx - ex, bxf = bx + ex;
+  const float byn = by - ey, byf = by + ey;
+  const float bzn = bz - ez, bzf = bz + ez;
   // near / far plane words per axis by direction sign
   uint32_t nx0 = r.negx ? w3.z : w2.x, nx1 = r.negx ? w3.w : w2.y;
   uint32_t fx0 = r.negx ? w2.x : w3.z, fx1 = r.negx ? w2.y : w3.w;
@@ -188,14 +205,14 @@ __device__ __forceinline__ bool trav_step(const SceneView& sc, const TravRay& r,
     uint32_t nyw = c < 4 ? ny0 : ny1, fyw = c < 4 ? fy0 : fy1;
     uint32_t nzw = c < 4 ? nz0 : nz1, fzw = c < 4 ? fz0 : fz1;
     uint32_t mw = c < 4 ? w1.z : w1.w;
-    float tnx = fmaf((float)((nxw >> sh) & 0xffu), ax, bx);
-    float tfx = fmaf((float)((fxw >> sh) & 0xffu), ax, bx);
-    float tny = fmaf((float)((nyw >> sh) & 0xffu), ay, by);
-    float tfy = fmaf((float)((fyw >> sh) & 0xffu), ay, by);
-    float tnz = fmaf((float)((nzw >> sh) & 0xffu), az, bz);
-    float tfz = fmaf((float)((fzw >> sh) & 0xffu), az, bz);
+    float tnx = fmaf((float)((nxw >> sh) & 0xffu), ax, bxn);
+    float tfx = fmaf((float)((fxw >> sh) & 0xffu), ax, bxf);
+    float tny = fmaf((float)((nyw >> sh) & 0xffu), ay, byn);
+    float tfy = fmaf((float)((fyw >> sh) & 0xffu), ay, byf);
+    float tnz = fmaf((float)((nzw >> sh) & 0xffu), az, bzn);
+    float tfz = fmaf((float)((fzw >> sh) & 0xffu), az, bzf);
     float tmin = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, r.tnear));
-    float tmax = fminf(fminf(fminf(tfx, tfy), tfz) * kFarInflate, tmax_ray);
+    float tmax = fminf(fminf(fminf(tfx, tfy), tfz), tmax_ray);
     uint32_t m = (mw >> sh) & 0xffu;
     if (m != 0u && tmin <= tmax) {
       if (m & 0x80u)

@@ -284,20 +284,22 @@ class SceneBuilder:
         return len(self.lights) - 1
 
     # -------------------------------------------------------------- atlas
-    def pack(self, width, height, padding=2, fill=0.85):
-        """Shelf-packs every chart (sorted by height) at one texel density,
-        the largest that fits, `padding` texels around each chart."""
+    def pack(self, width, height, padding=2):
+        """Shelf-packs every chart (tallest first) at one texel density, the
+        largest found by bisection that fits, with `padding` texels around
+        each chart. Raises ValueError when the charts cannot fit at all."""
         ext = np.array(self.charts, np.float64).reshape(-1, 2)
         order = np.argsort(-ext[:, 1], kind="stable")
 
         def place(d):
+            ws = np.maximum(1, np.ceil(ext[:, 0] * d)).astype(np.int64) + 2 * padding
+            hs = np.maximum(1, np.ceil(ext[:, 1] * d)).astype(np.int64) + 2 * padding
+            if ws.max() > width:
+                return None
             x = y = row_h = 0
             pos = np.zeros((len(ext), 2), np.int64)
             for ci in order:
-                w = max(1, int(math.ceil(ext[ci, 0] * d))) + 2 * padding
-                h = max(1, int(math.ceil(ext[ci, 1] * d))) + 2 * padding
-                if w > width:
-                    return None
+                w, h = ws[ci], hs[ci]
                 if x + w > width:
                     x, y, row_h = 0, y + row_h, 0
                 if y + h > height:
@@ -307,15 +309,23 @@ class SceneBuilder:
                 row_h = max(row_h, h)
             return pos
 
+        lo = 0.0
+        if place(lo) is None:
+            raise ValueError(f"{len(ext)} charts do not fit a {width}x{height} atlas "
+                             f"with {padding} texels of padding")
         area = float((ext[:, 0] * ext[:, 1]).sum())
-        d = math.sqrt(fill * width * height / max(area, 1e-12))
-        pos = place(d)
-        while pos is None:
-            d *= 0.97
-            pos = place(d)
-        self.density = d
-        self.chart_pos = pos
-        return d
+        hi = 2.0 * math.sqrt(width * height / max(area, 1e-12))
+        while place(hi) is not None:
+            lo, hi = hi, hi * 2
+        for _ in range(24):
+            mid = 0.5 * (lo + hi)
+            if place(mid) is None:
+                hi = mid
+            else:
+                lo = mid
+        self.density = lo
+        self.chart_pos = place(lo)
+        return lo
 
     def build(self, width, height, padding=2) -> Scene:
         self.pack(width, height, padding)

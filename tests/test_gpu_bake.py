@@ -70,12 +70,15 @@ def test_bake_within_3_sigma_of_reference_stream(baker, room):
     spp, B = 64, 3
     g_sh, g_vis, _ = baker.bake(spp, B, seed=777)
     r_sh, r_vis, r_var, _ = o.bake(pts, 48, 48, spp, B, mode=orc.MODE_REF)
+    # the GPU's per-texel sample variance: the oracle's GPU-stream mode draws
+    # the same samples, so its variance estimate is the GPU's
+    _, _, g_var, _ = o.bake(pts, 48, 48, spp, B, mode=orc.MODE_GPU, seed=777)
     v = pts["material_id"] >= 0
-    sig = np.sqrt(2.0 * r_var[v]) + 1e-7
-    z = (g_sh[v] - r_sh[v]) / sig
-    live = r_var[v] > 0
-    assert 0.5 < np.abs(z[live]).mean() < 1.1, np.abs(z[live]).mean()
-    assert (np.abs(z[live]) > 4).mean() < 5e-3
+    live = (r_var[v] + g_var[v]) > 0
+    z = (g_sh[v] - r_sh[v])[live] / np.sqrt(r_var[v] + g_var[v])[live]
+    assert 0.6 < np.abs(z).mean() < 1.0, np.abs(z).mean()
+    assert (np.abs(z) > 4.5).mean() < 1e-3
+    np.testing.assert_allclose(g_vis[v].mean(), r_vis[v].mean(), atol=0.02)
 
 
 def test_bake_deterministic_and_seeded(baker, room):
