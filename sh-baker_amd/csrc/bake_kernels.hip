@@ -103,16 +103,23 @@ __global__ void __launch_bounds__(256) k_trace(TraceArgs a, Epi epi) {
   if (!a.has_nodes || !ray_is_finite(o, d)) s.gbits = 0;
   LdsStack stack{stack_mem, (int)threadIdx.x, (int)blockDim.x};
   bool overflow = false;
+  bool capped = false;
   uint32_t steps = 0;
   while (!trav_step<ANY, COUNT>(a.scene, r, s, stack, overflow)) {
     // a ray visits each node at most once: more steps means a corrupt tree
     if (++steps > a.max_steps) {
-      overflow = true;
+      capped = true;
       break;
     }
   }
-  if (overflow) {
-    atomicOr(a.error, 1u);
+  if (overflow || capped) {
+    // error[0]: bit 0 stack overflow, bit 1 step cap; the first failing ray
+    // records slot + 1, its stack depth and its step count in error[1..3]
+    atomicOr(a.error, (overflow ? 1u : 0u) | (capped ? 2u : 0u));
+    if (atomicCAS(a.error + 1, 0u, slot + 1u) == 0u) {
+      a.error[2] = (uint32_t)s.sp;
+      a.error[3] = steps;
+    }
     s.best = 0x7fffffff;
   }
   if (COUNT) {

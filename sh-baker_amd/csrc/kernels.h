@@ -16,7 +16,7 @@ struct TraceArgs {
   uint32_t count_const;       // count when count_ptr is null
   int has_nodes;              // 0: empty scene
   uint32_t max_steps;         // per-ray traversal step cap (node count + 1)
-  uint32_t* error;            // set to 1 on stack overflow / step cap
+  uint32_t* error;            // [0] bits: 1 stack overflow, 2 step cap; [1..3] first ray
   uint32_t* ray_counts;       // optional: node visits, triangle tests per slot
   unsigned long long* totals; // optional: 64 pairs of summed (nodes, tris)
 };

@@ -74,11 +74,16 @@ TraceArgs trace_args(Ctx& c) {
 }
 
 int check_overflow(Ctx& c) {
-  uint32_t e = 0;
-  SHB_TRY(c, hipMemcpyAsync(&e, c.d_work.ptr, 4, hipMemcpyDeviceToHost, c.stream));
+  uint32_t e[4] = {0, 0, 0, 0};
+  SHB_TRY(c, hipMemcpyAsync(e, c.d_work.ptr, sizeof(e), hipMemcpyDeviceToHost, c.stream));
   SHB_TRY(c, hipStreamSynchronize(c.stream));
-  if (e) return fail(c, SHB_ERR_LIMIT, "traversal stack overflow");
-  return SHB_OK;
+  if (e[0] == 0) return SHB_OK;
+  std::string what = (e[0] & 1u) ? "traversal stack overflow" : "";
+  if (e[0] & 2u) what += std::string(what.empty() ? "" : " and ") + "traversal step cap reached";
+  return fail(c, SHB_ERR_LIMIT,
+              what + " (first ray slot " + std::to_string((long long)e[1] - 1) + ", stack depth " +
+                  std::to_string(e[2]) + ", " + std::to_string(e[3]) + " steps, cap " +
+                  std::to_string(c.bvh.nodes.size() + 1) + ")");
 }
 
 // Counter layout per batch (uint32): extension-queue counts by depth, shadow
@@ -108,7 +113,8 @@ hipError_t stage_done(hipStream_t st, const char* name, uint32_t batch, int dept
   return e;
 }
 
-// d_work layout: [0] error flag, [4..4+256) 64 pairs of u64 visit totals.
+// d_work layout: [0] error bits, [1..3] first failing ray (slot + 1, stack
+// depth, steps), bytes [16, 16 + 1024): 64 pairs of u64 visit totals.
 constexpr size_t kWorkBytes = 16 + 64 * 16;
 
 struct BakeRun {

@@ -221,8 +221,15 @@ __device__ __forceinline__ bool trav_step(const SceneView& sc, const TravRay& r,
         tmask |= ((1u << (m >> 5)) - 1u) << (m & 31u);
     }
   }
+  // The next group is set before the triangle tests, and the test loop has a
+  // single exit (an any-hit ray finishes the node's few triangles and then
+  // stops): with an early return from inside this loop, hipcc's gfx950 code
+  // for the any-hit variant carried the old group base instead of the
+  // child base out of nodes whose triangles all missed.
+  s.gbase = w1.x;
+  s.gbits = hits | (imask << 8);
   const uint32_t tri_base = w1.y;
-  const uint32_t child_base = w1.x;
+  bool found = false;
   while (tmask) {
     uint32_t j = __builtin_ctz(tmask);
     tmask &= tmask - 1u;
@@ -237,13 +244,11 @@ __device__ __forceinline__ bool trav_step(const SceneView& sc, const TravRay& r,
         s.best = id;
         s.bu = u;
         s.bv = v;
-        if (ANY) return true;
+        found = true;
       }
     }
   }
-  s.gbase = child_base;
-  s.gbits = hits | (imask << 8);
-  return false;
+  return ANY && found;
 }
 
 }  // namespace shb

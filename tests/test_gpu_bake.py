@@ -127,7 +127,7 @@ def test_lightmap_pipeline_bit_exact(baker, room, ss, dil):
 
 
 def test_bake_errors(baker):
-    s = scenes.box_room_scene(16, 16)
+    s = scenes.box_room_scene(64, 64)
     fresh = shb.Baker(0)
     with pytest.raises(shb.ShbError) as e:
         fresh.bake(4, 1)
@@ -136,7 +136,7 @@ def test_bake_errors(baker):
     with pytest.raises(shb.ShbError) as e:
         fresh.bake(4, 1)
     assert e.value.code == -6
-    fresh.rasterize(16, 16, 1, want=False)
+    fresh.rasterize(64, 64, 1, want=False)
     with pytest.raises(shb.ShbError) as e:
         fresh.bake(4, 1, num_light_samples=2)
     assert e.value.code == -7
