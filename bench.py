@@ -33,6 +33,35 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_k_trace_synthetic_room_1024_ss3_spp128.md")
+
+
+def read_traffic():
+    """HBM bytes per launch of each traversal kernel kind, measured with
+    rocprofv3 FETCH_SIZE + WRITE_SIZE (scaled by a copy kernel of known size)
+    and recorded as `traffic_bytes_per_launch <kind>: <bytes>` lines."""
+    out = {}
+    if os.path.exists(TRAFFIC_FILE):
+        for line in open(TRAFFIC_FILE):
+            if line.startswith("traffic_bytes_per_launch "):
+                k, v = line[len("traffic_bytes_per_launch "):].split(":")
+                out[k.strip()] = int(float(v))
+    return out
+
+
+def copy_probe(mib):
+    """A device-to-device copy of known size (the calibration kernel for the
+    FETCH_SIZE / WRITE_SIZE counters): copies `mib` MiB three times."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    c = torch.empty_like(a)
+    for _ in range(3):
+        c.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_each_way": n * 4, "copies": 3}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -48,7 +77,12 @@ def main():
                     help="target CPU-baseline sample duration (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-count", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run a device copy of this size (counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe_mib:
+        copy_probe(args.copy_probe_mib)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); "
               "this build has no multi-GPU path", file=sys.stderr)
@@ -65,9 +99,10 @@ def main():
     log(f"scene: {scene.num_triangles} tris, {len(scene.lights)} lights, "
         f"{len(scene.materials)} materials ({time.time() - t0:.1f}s)")
     b = shb.Baker(0)
-    t0 = time.time()
+    t0 = time.perf_counter()
     b.upload_scene(scene)
-    log(f"upload + BVH build {time.time() - t0:.2f}s")
+    upload_s = time.perf_counter() - t0
+    log(f"upload + BVH build {upload_s:.3f}s")
     pts = b.rasterize(args.res, args.res, args.ss)
     valid = int((pts["material_id"] >= 0).sum())
     log(f"raster {args.res}x{args.res} ss{args.ss}: {valid} valid texels "
@@ -79,8 +114,9 @@ def main():
     for i in range(args.warmup):
         _, _, _, st = step()
         log(f"warmup {i}: {st['seconds_total']:.3f}s rays {st['rays_extension'] + st['rays_shadow']}")
-    import torch  # noqa: F401  (the harness's timing convention: synchronize around the region)
-    times, rays, trace_s, dev_s = [], 0, 0.0, 0.0
+    # every entry point returns after its stream has drained, so the timed
+    # region is bracketed by device synchronisation on both sides
+    times, rays, trace_s, trace_sh_s, launches = [], 0, 0.0, 0.0, [0, 0]
     t_start = time.perf_counter()
     for i in range(args.steps):
         ts = time.perf_counter()
@@ -88,29 +124,50 @@ def main():
         times.append(time.perf_counter() - ts)
         rays += st["rays_extension"] + st["rays_shadow"]
         trace_s += st["seconds_trace"]
-        dev_s += st["seconds_device"]
+        trace_sh_s += st["seconds_trace_shadow"]
+        launches[0] += st["launches_extension"]
+        launches[1] += st["launches_shadow"]
         log(f"step {i}: {times[-1]:.3f}s  device {st['seconds_device']:.3f}s  "
-            f"trace {st['seconds_trace']:.3f}s  ext {st['rays_extension']}  "
-            f"shadow {st['rays_shadow']}")
+            f"trace {st['seconds_trace']:.3f}s (shadow {st['seconds_trace_shadow']:.3f}s)  "
+            f"ext {st['rays_extension']}  shadow {st['rays_shadow']}")
     elapsed = time.perf_counter() - t_start
     mrays = rays / elapsed / 1e6
     ms_per_step = elapsed / args.steps * 1e3
+    bake_s = float(np.median(times))
 
-    roofline = None
+    roofline, kernels = None, None
     if not args.no_count:
         _, _, stc = b.bake(args.spp, args.bounces, seed=12345, flags=shb.BAKE_COUNT_VISITS)
-        nrays = stc["rays_extension"] + stc["rays_shadow"]
-        alg_bytes = (nrays * (RAY_BYTES + HIT_BYTES) + stc["node_visits"] * NODE_BYTES
-                     + stc["tri_tests"] * TRI_BYTES)
-        per_step_bytes = alg_bytes  # same work as one timed step
-        achieved = per_step_bytes / (trace_s / args.steps) / 1e9
-        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                    "kernel": "k_trace (closest + any hit)",
-                    "nodes_per_ray": round(stc["node_visits"] / nrays, 2),
-                    "tris_per_ray": round(stc["tri_tests"] / nrays, 2)}
-        log(f"counted bake: nodes/ray {stc['node_visits'] / nrays:.2f} "
-            f"tris/ray {stc['tri_tests'] / nrays:.2f} alg bytes {alg_bytes / 1e9:.1f} GB")
+        n_sh = stc["rays_shadow"]
+        n_ext = stc["rays_extension"]
+        nodes_sh, tris_sh = stc["node_visits_shadow"], stc["tri_tests_shadow"]
+        nodes_ext, tris_ext = stc["node_visits"] - nodes_sh, stc["tri_tests"] - tris_sh
+        traffic = read_traffic()
+        kernels = {}
+        for name, n, nodes, tris, secs, nl in (
+                ("k_trace<closest, HitEpilogue> (extension rays)", n_ext, nodes_ext, tris_ext,
+                 trace_s - trace_sh_s, launches[0]),
+                ("k_trace<any, BakeShadowEpilogue> (shadow rays)", n_sh, nodes_sh, tris_sh,
+                 trace_sh_s, launches[1])):
+            alg = n * (RAY_BYTES + HIT_BYTES) + nodes * NODE_BYTES + tris * TRI_BYTES  # per bake
+            per_launch = alg / max(nl / args.steps, 1)
+            avg_launch_s = secs / max(nl, 1)
+            achieved = per_launch / avg_launch_s / 1e9 if avg_launch_s > 0 else 0.0
+            kernels[name] = {
+                "kind": "shadow" if "shadow" in name else "extension",
+                "rays_per_bake": n, "nodes_per_ray": round(nodes / max(n, 1), 2),
+                "tris_per_ray": round(tris / max(n, 1), 2),
+                "alg_bytes_per_launch": int(per_launch),
+                "avg_launch_ms": round(avg_launch_s * 1e3, 4),
+                "launches_per_bake": nl // args.steps,
+                "achieved_GBs": round(achieved, 1), "seconds_per_bake": round(secs / args.steps, 4)}
+        dom = max(kernels, key=lambda k: kernels[k]["seconds_per_bake"])
+        kd = kernels[dom]
+        roofline = {"bound": "hbm", "achieved": kd["achieved_GBs"], "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(kd["achieved_GBs"] / HBM_PEAK_GBS, 4),
+                    "traffic": traffic.get(kd["kind"]),
+                    "kernel": dom}
+        log(f"counted bake: {json.dumps(kernels)}")
 
     cpu = None
     if args.cpu_seconds > 0:
@@ -151,7 +208,8 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(ms_per_step, 1),
-        "bake_seconds": round(ms_per_step / 1e3, 3),
+        "bake_seconds": round(bake_s, 4),
+        "bake_seconds_incl_bvh_build": round(bake_s + upload_s, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -163,10 +221,12 @@ def main():
                    "dilation": args.dilation, "valid_texels": valid,
                    "rays_per_step": rays // args.steps},
         "roofline": roofline,
+        "roofline_kernels": kernels,
         "cpu_baseline": cpu,
     }
     if cpu:
         out["speedup_vs_cpu"] = round(mrays / cpu["value"], 1)
+        out["speedup_bake_seconds_vs_cpu"] = round(cpu["bake_seconds_extrapolated"] / bake_s, 1)
     print(json.dumps(out), flush=True)
 
 

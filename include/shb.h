@@ -156,6 +156,12 @@ typedef struct shb_bake_stats {
   double seconds_total;     /* wall time of the call                        */
   double seconds_device;    /* GPU time between the first and last kernel   */
   double seconds_trace;     /* GPU time inside the traversal kernels        */
+  /* the shadow (any-hit) part of node_visits, tri_tests and seconds_trace  */
+  uint64_t node_visits_shadow;
+  uint64_t tri_tests_shadow;
+  double seconds_trace_shadow;
+  uint32_t launches_extension;  /* traversal kernel launches of each kind   */
+  uint32_t launches_shadow;
 } shb_bake_stats;
 
 /* Launch parameters of the wavefront kernels (chosen by shb_autotune). */

@@ -114,12 +114,15 @@ hipError_t stage_done(hipStream_t st, const char* name, uint32_t batch, int dept
 }
 
 // d_work layout: [0] error bits, [1..3] first failing ray (slot + 1, stack
-// depth, steps), bytes [16, 16 + 1024): 64 pairs of u64 visit totals.
-constexpr size_t kWorkBytes = 16 + 64 * 16;
+// depth, steps); then 64 pairs of u64 visit totals of the extension
+// traversals, then 64 pairs of the shadow traversals.
+constexpr size_t kWorkBytes = 16 + 2 * 64 * 16;
 
 struct BakeRun {
   uint64_t rays_ext = 0, rays_sh = 0, valid = 0, node_visits = 0, tri_tests = 0;
-  double dev_s = 0, trace_s = 0;
+  uint64_t node_visits_sh = 0, tri_tests_sh = 0;
+  double dev_s = 0, trace_s = 0, trace_sh_s = 0;
+  uint32_t launches_ext = 0, launches_sh = 0;
 };
 
 // The bake into c.d_sh / c.d_vis (supersampled), with per-batch kernels:
@@ -201,12 +204,16 @@ int bake_device(Ctx& c, const shb_bake_cfg* cfg, uint32_t job, BakeRun* run) {
   b.sh_out = c.d_sh.as<float>();
   b.vis_out = c.d_vis.as<float>();
   unsigned long long* totals = reinterpret_cast<unsigned long long*>(c.d_work.as<uint32_t>() + 4);
+  unsigned long long* totals_sh = totals + 128;
+  // event pairs around each traversal launch; kind 0 extension, 1 shadow
   std::vector<hipEvent_t> tev;
-  auto trace_ev = [&]() -> hipEvent_t {
+  std::vector<int> tkind;
+  auto trace_ev = [&](int kind) -> hipEvent_t {
     hipEvent_t e;
-    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    if (hipEventCreate(&e) != hipSuccess) e = nullptr;
     tev.push_back(e);
-    (void)hipEventRecord(e, st);
+    tkind.push_back(kind);
+    if (e) (void)hipEventRecord(e, st);
     return e;
   };
   const int sgrid = std::max(1, std::min<int>((P + 255) / 256, c.num_cus * 16));
@@ -225,9 +232,10 @@ int bake_device(Ctx& c, const shb_bake_cfg* cfg, uint32_t job, BakeRun* run) {
       ta.count_ptr = d == 0 ? nullptr : cnt + kCntExt + d;
       ta.count_const = b.texels_in_batch * spp;
       ta.totals = count ? totals : nullptr;
-      trace_ev();
+      trace_ev(0);
       SHB_TRY(c, launch_trace_hits(ta, b.hit, count, b.texels_in_batch * spp, c.tuning.block_size, st));
-      trace_ev();
+      trace_ev(0);
+      ++run->launches_ext;
       SHB_TRY(c, stage_done(st, "extend", bi, (int)d));
       b.shadow_count = cnt + kCntSh + d;
       SHB_TRY(c, launch_shade(b, (int)d, d == 0 ? nullptr : qext[d & 1], cnt + kCntExt + d,
@@ -237,11 +245,12 @@ int bake_device(Ctx& c, const shb_bake_cfg* cfg, uint32_t job, BakeRun* run) {
       sa.rays = b.sray;
       sa.queue = b.shadow_queue;
       sa.count_ptr = cnt + kCntSh + d;
-      sa.totals = count ? totals : nullptr;
-      trace_ev();
+      sa.totals = count ? totals_sh : nullptr;
+      trace_ev(1);
       SHB_TRY(c, launch_trace_bake_shadow(sa, b, count, 2 * b.texels_in_batch * spp,
                                           c.tuning.block_size, st));
-      trace_ev();
+      trace_ev(1);
+      ++run->launches_sh;
       SHB_TRY(c, stage_done(st, "shadow", bi, (int)d));
     }
     SHB_TRY(c, launch_project(b, st));
@@ -252,16 +261,20 @@ int bake_device(Ctx& c, const shb_bake_cfg* cfg, uint32_t job, BakeRun* run) {
   float ms = 0.f;
   SHB_TRY(c, hipEventElapsedTime(&ms, e0, e1));
   run->dev_s = ms * 1e-3;
-  double tr = 0;
+  double tr = 0, tr_sh = 0;
   for (size_t i = 0; i + 1 < tev.size(); i += 2) {
     float m = 0.f;
-    if (tev[i] && tev[i + 1] && hipEventElapsedTime(&m, tev[i], tev[i + 1]) == hipSuccess) tr += m;
+    if (tev[i] && tev[i + 1] && hipEventElapsedTime(&m, tev[i], tev[i + 1]) == hipSuccess) {
+      tr += m;
+      if (tkind[i] == 1) tr_sh += m;
+    }
   }
   for (hipEvent_t e : tev)
     if (e) (void)hipEventDestroy(e);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   run->trace_s = tr * 1e-3;
+  run->trace_sh_s = tr_sh * 1e-3;
   // ray counts
   std::vector<uint32_t> hc((size_t)nb * kCntStride);
   if (nb) {
@@ -276,12 +289,14 @@ int bake_device(Ctx& c, const shb_bake_cfg* cfg, uint32_t job, BakeRun* run) {
   }
   std::vector<uint32_t> hw(kWorkBytes / 4);
   SHB_TRY(c, hipMemcpy(hw.data(), c.d_work.ptr, kWorkBytes, hipMemcpyDeviceToHost));
-  if (hw[0]) return fail(c, SHB_ERR_LIMIT, "traversal stack overflow or corrupt BVH");
+  if (hw[0]) return check_overflow(c);
   if (count) {
     const unsigned long long* t2 = reinterpret_cast<const unsigned long long*>(hw.data() + 4);
     for (int k = 0; k < 64; ++k) {
-      run->node_visits += t2[2 * k];
-      run->tri_tests += t2[2 * k + 1];
+      run->node_visits += t2[2 * k] + t2[128 + 2 * k];
+      run->tri_tests += t2[2 * k + 1] + t2[128 + 2 * k + 1];
+      run->node_visits_sh += t2[128 + 2 * k];
+      run->tri_tests_sh += t2[128 + 2 * k + 1];
     }
   }
   return SHB_OK;
@@ -298,6 +313,11 @@ void fill_stats(shb_bake_stats* s, const BakeRun& r, uint32_t spp, double t0) {
   s->seconds_total = now_s() - t0;
   s->seconds_device = r.dev_s;
   s->seconds_trace = r.trace_s;
+  s->node_visits_shadow = r.node_visits_sh;
+  s->tri_tests_shadow = r.tri_tests_sh;
+  s->seconds_trace_shadow = r.trace_sh_s;
+  s->launches_extension = r.launches_ext;
+  s->launches_shadow = r.launches_sh;
 }
 
 }  // namespace

@@ -83,7 +83,10 @@ class ShbBakeStats(C.Structure):
                 ("rays_extension", C.c_uint64), ("rays_shadow", C.c_uint64),
                 ("node_visits", C.c_uint64), ("tri_tests", C.c_uint64),
                 ("seconds_total", C.c_double), ("seconds_device", C.c_double),
-                ("seconds_trace", C.c_double)]
+                ("seconds_trace", C.c_double),
+                ("node_visits_shadow", C.c_uint64), ("tri_tests_shadow", C.c_uint64),
+                ("seconds_trace_shadow", C.c_double),
+                ("launches_extension", C.c_uint32), ("launches_shadow", C.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
