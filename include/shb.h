@@ -61,7 +61,8 @@ typedef struct shb_geometry {
   uint32_t vertex_count;
   uint32_t index_offset;
   uint32_t index_count;
-  int32_t material_id;  /* index into materials; must be valid              */
+  int32_t material_id;  /* index into materials (-1: none); the bake needs a
+                           valid one, raster and ray queries pass it through */
   uint32_t flags;       /* SHB_GEOM_HAS_*                                    */
   float transform[12];  /* row-major 3x4 affine (object -> world)            */
 } shb_geometry;
@@ -244,6 +245,15 @@ int shb_bake(shb_ctx* ctx, const shb_bake_cfg* cfg, float* sh_out,
 int shb_bake_lightmap(shb_ctx* ctx, const shb_bake_cfg* cfg,
                       int32_t dilation, float* sh_out, float* vis_out,
                       uint8_t* mask_out, shb_bake_stats* stats);
+
+/* Copies the result of the last shb_bake_lightmap of this context to host
+ * memory (any pointer may be NULL). With NULL outputs, shb_bake_lightmap
+ * leaves its result in device memory only (no PCIe transfer); this call
+ * fetches it later. Valid until the next bake, rasterize or set_surface.
+ * No reference counterpart: the reference returns host vectors
+ * (main.cpp:131-163). */
+int shb_read_lightmap(shb_ctx* ctx, float* sh_out, float* vis_out,
+                      uint8_t* mask_out);
 
 /* ss x ss box average of a (width*scale) x (height*scale) image of
  * `channels` floats per texel into width x height. Replaces DownsampleTexture

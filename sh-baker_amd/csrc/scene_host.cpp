@@ -124,8 +124,11 @@ int build_host_scene(const shb_scene_desc* d, HostScene* out, std::string* err) 
       return fail(SHB_ERR_INVALID_ARG, gs + ": vertices out of range");
     if ((uint64_t)g.index_offset + g.index_count > d->num_indices)
       return fail(SHB_ERR_INVALID_ARG, gs + ": indices out of range");
-    if (g.material_id < 0 || (uint32_t)g.material_id >= d->num_materials)
-      return fail(SHB_ERR_INVALID_ARG, gs + ": material_id out of range");
+    // material ids only matter to the bake (baker.cpp:94 indexes materials
+    // with them); rasterization and ray queries pass them through as the
+    // reference does (rasterizer.cpp:133-151, occlusion.cpp:50).
+    if (g.index_count > 0 && (g.material_id < 0 || (uint32_t)g.material_id >= d->num_materials))
+      hs.bad_material = true;
     bool hasN = (g.flags & SHB_GEOM_HAS_NORMALS) && d->normals;
     bool hasUV = (g.flags & SHB_GEOM_HAS_TEXTURE_UVS) && d->texture_uvs;
     bool hasL = (g.flags & SHB_GEOM_HAS_LIGHTMAP_UVS) && d->lightmap_uvs;

@@ -28,6 +28,7 @@ struct HostScene {
   std::vector<AreaTri> area_tris;
   bool raster_ready = true;          // every geometry has luv, normals, tangents
   bool has_transparency = false;     // some RGBA texel with alpha < 255
+  bool bad_material = false;         // a geometry's material_id is not a material
 };
 
 // Validates and converts the scene; returns SHB_OK or an SHB_ERR_* code with

@@ -128,6 +128,7 @@ struct BakeRun {
 // The bake into c.d_sh / c.d_vis (supersampled), with per-batch kernels:
 // generate, then per depth: extend, shade, shadow; then project.
 int bake_device(Ctx& c, const shb_bake_cfg* cfg, uint32_t job, BakeRun* run) {
+  c.lm_sh = nullptr;  // the device buffers below are about to be reused
   if (!c.has_scene) return fail(c, SHB_ERR_NO_SCENE, "no scene uploaded");
   if (!c.has_surface) return fail(c, SHB_ERR_NO_SURFACE, "no surface: call shb_rasterize or shb_set_surface");
   if (!cfg) return fail(c, SHB_ERR_INVALID_ARG, "cfg is NULL");
@@ -137,6 +138,9 @@ int bake_device(Ctx& c, const shb_bake_cfg* cfg, uint32_t job, BakeRun* run) {
     return fail(c, SHB_ERR_INVALID_ARG, "bounces must be in [0, 15]");
   if (cfg->num_light_samples != 1)
     return fail(c, SHB_ERR_UNSUPPORTED, "num_light_samples must be 1");
+  if (c.hs.bad_material)
+    return fail(c, SHB_ERR_INVALID_ARG,
+                "a geometry's material_id does not name a material (the bake shades every hit)");
   if (c.hs.has_transparency)
     return fail(c, SHB_ERR_UNSUPPORTED,
                 "materials with alpha < 1 (path splitting) are not supported by the GPU bake");
@@ -382,6 +386,7 @@ int shb_upload_scene(shb_ctx* h, const shb_scene_desc* scene) {
   int rc = build_scene(c, scene);
   if (rc) return rc;
   c.has_surface = false;
+  c.lm_sh = nullptr;
   rc = upload_bvh(c);
   if (rc) return rc;
   std::vector<float> lut(256);
@@ -472,6 +477,7 @@ int shb_rasterize(shb_ctx* h, const shb_raster_cfg* cfg, shb_surface_point* out)
   c.height = cfg->height;
   c.ss = cfg->supersample_scale;
   c.has_surface = true;
+  c.lm_sh = nullptr;
   if (out) {
     SHB_TRY(c, hipMemcpyAsync(out, c.d_surface.ptr, npx * sizeof(shb_surface_point),
                               hipMemcpyDeviceToHost, c.stream));
@@ -503,6 +509,7 @@ int shb_set_surface(shb_ctx* h, const shb_raster_cfg* cfg, const shb_surface_poi
   c.height = cfg->height;
   c.ss = cfg->supersample_scale;
   c.has_surface = true;
+  c.lm_sh = nullptr;
   return SHB_OK;
 }
 
@@ -593,6 +600,10 @@ int shb_bake_lightmap(shb_ctx* h, const shb_bake_cfg* cfg, int32_t dilation, flo
     vis_res = vin ? v_src : v_other;
     m_res = vin ? m_sh : m_other;
   }
+  c.lm_sh = sh_res;
+  c.lm_vis = vis_res;
+  c.lm_mask = m_res;
+  c.lm_texels = n;
   if (sh_out) SHB_TRY(c, hipMemcpyAsync(sh_out, sh_res, n * 27 * 4, hipMemcpyDeviceToHost, st));
   if (vis_out) SHB_TRY(c, hipMemcpyAsync(vis_out, vis_res, n * 4, hipMemcpyDeviceToHost, st));
   if (mask_out) SHB_TRY(c, hipMemcpyAsync(mask_out, m_res, n, hipMemcpyDeviceToHost, st));
@@ -601,10 +612,25 @@ int shb_bake_lightmap(shb_ctx* h, const shb_bake_cfg* cfg, int32_t dilation, flo
   return SHB_OK;
 }
 
+int shb_read_lightmap(shb_ctx* h, float* sh_out, float* vis_out, uint8_t* mask_out) {
+  if (!h) return SHB_ERR_INVALID_ARG;
+  Ctx& c = h->c;
+  c.err.clear();
+  if (!c.lm_sh) return fail(c, SHB_ERR_NO_SURFACE, "no lightmap baked in this context yet");
+  SHB_TRY(c, hipSetDevice(c.device));
+  const size_t n = c.lm_texels;
+  if (sh_out) SHB_TRY(c, hipMemcpyAsync(sh_out, c.lm_sh, n * 27 * 4, hipMemcpyDeviceToHost, c.stream));
+  if (vis_out) SHB_TRY(c, hipMemcpyAsync(vis_out, c.lm_vis, n * 4, hipMemcpyDeviceToHost, c.stream));
+  if (mask_out) SHB_TRY(c, hipMemcpyAsync(mask_out, c.lm_mask, n, hipMemcpyDeviceToHost, c.stream));
+  SHB_TRY(c, hipStreamSynchronize(c.stream));
+  return SHB_OK;
+}
+
 int shb_downsample(shb_ctx* h, int32_t width, int32_t height, int32_t scale, int32_t channels,
                    const float* in, float* out) {
   if (!h) return SHB_ERR_INVALID_ARG;
   Ctx& c = h->c;
+  c.lm_sh = nullptr;
   if (width <= 0 || height <= 0 || scale <= 0 || channels <= 0 || !in || !out)
     return fail(c, SHB_ERR_INVALID_ARG, "bad downsample arguments");
   SHB_TRY(c, hipSetDevice(c.device));
@@ -623,6 +649,7 @@ int shb_downsample_mask(shb_ctx* h, int32_t width, int32_t height, int32_t scale
                         const uint8_t* in, uint8_t* out) {
   if (!h) return SHB_ERR_INVALID_ARG;
   Ctx& c = h->c;
+  c.lm_sh = nullptr;
   if (width <= 0 || height <= 0 || scale <= 0 || !in || !out)
     return fail(c, SHB_ERR_INVALID_ARG, "bad downsample_mask arguments");
   SHB_TRY(c, hipSetDevice(c.device));
@@ -641,6 +668,7 @@ int shb_dilate(shb_ctx* h, int32_t width, int32_t height, int32_t channels, floa
                uint8_t* mask, int32_t passes) {
   if (!h) return SHB_ERR_INVALID_ARG;
   Ctx& c = h->c;
+  c.lm_sh = nullptr;
   if (width < 0 || height < 0 || channels <= 0 || passes < 0)
     return fail(c, SHB_ERR_INVALID_ARG, "bad dilate arguments");
   size_t n = (size_t)width * height;

@@ -42,6 +42,11 @@ struct Ctx {
   int num_cus = 0;
   std::string err;
   shb_tuning tuning{256, 1 << 23, 0, 8};
+  // device-resident result of the last shb_bake_lightmap (base resolution)
+  const float* lm_sh = nullptr;
+  const float* lm_vis = nullptr;
+  const uint8_t* lm_mask = nullptr;
+  size_t lm_texels = 0;
 
   // scene
   bool has_scene = false;

@@ -106,7 +106,7 @@ assert HIT_DTYPE.itemsize == 56
 
 EXPORTED = [
     "shb_create", "shb_destroy", "shb_last_error", "shb_version", "shb_upload_scene",
-    "shb_rasterize", "shb_set_surface", "shb_validity_mask", "shb_bake", "shb_bake_lightmap",
+    "shb_rasterize", "shb_set_surface", "shb_validity_mask", "shb_bake", "shb_bake_lightmap", "shb_read_lightmap",
     "shb_downsample", "shb_downsample_mask", "shb_dilate", "shb_trace_rays",
     "shb_bake_batched", "shb_autotune", "shb_set_tuning", "shb_get_tuning", "shb_set_params",
     "shb_bake_adjoint",
@@ -133,6 +133,7 @@ def _load():
     lib.shb_bake.argtypes = [vp, C.POINTER(ShbBakeCfg), _fp, _fp, C.POINTER(ShbBakeStats)]
     lib.shb_bake_lightmap.argtypes = [vp, C.POINTER(ShbBakeCfg), C.c_int32, _fp, _fp, u8p,
                                       C.POINTER(ShbBakeStats)]
+    lib.shb_read_lightmap.argtypes = [vp, _fp, _fp, u8p]
     lib.shb_downsample.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _fp, _fp]
     lib.shb_downsample_mask.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, u8p, u8p]
     lib.shb_dilate.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, _fp, u8p, C.c_int32]
@@ -249,16 +250,27 @@ class Baker:
         self._chk(lib.shb_bake(self.h, C.byref(cfg), _f(sh), _f(vis), C.byref(st)))
         return sh, vis, st.as_dict()
 
-    def bake_lightmap(self, samples, bounces, dilation, seed=12345, flags=0):
+    def bake_lightmap(self, samples, bounces, dilation, seed=12345, flags=0, to_host=True):
+        """Bake + downsample + dilation. With to_host=False the result stays
+        in device memory (fetch it with read_lightmap) and None is returned
+        for the arrays."""
         n = self.width * self.height
-        sh = np.zeros((n, 27), np.float32)
-        vis = np.zeros(n, np.float32)
-        mask = np.zeros(n, np.uint8)
+        sh = np.zeros((n, 27), np.float32) if to_host else None
+        vis = np.zeros(n, np.float32) if to_host else None
+        mask = np.zeros(n, np.uint8) if to_host else None
         st = ShbBakeStats()
         cfg = self.cfg(samples, bounces, seed, flags)
         self._chk(lib.shb_bake_lightmap(self.h, C.byref(cfg), dilation, _f(sh), _f(vis),
                                         _u8(mask), C.byref(st)))
         return sh, vis, mask, st.as_dict()
+
+    def read_lightmap(self):
+        n = self.width * self.height
+        sh = np.zeros((n, 27), np.float32)
+        vis = np.zeros(n, np.float32)
+        mask = np.zeros(n, np.uint8)
+        self._chk(lib.shb_read_lightmap(self.h, _f(sh), _f(vis), _u8(mask)))
+        return sh, vis, mask
 
     # -- image ops
     def downsample(self, img, width, height, scale):

@@ -18,6 +18,8 @@ def pytest_configure(config):
     if not os.path.exists(os.path.join(ROOT, "sh-baker_amd", "shb", "libshb_hip.so")):
         subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "sh-baker_amd"), "-j8"],
                        check=True)
+    if not os.path.exists(os.path.join(ROOT, "tests", "cpp", "shb_baker_test")):
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "tests", "cpp")], check=True)
 
 
 @pytest.fixture(scope="session")

@@ -1,0 +1,304 @@
+// The reference's own unit tests for the bake path (src/baker_test.cpp,
+// src/occlusion_test.cpp, src/rasterizer_test.cpp, src/dilation_test.cpp),
+// restated against the C++ mirror include/shb_baker.hpp: same scenes, same
+// expectations, so a reference user sees the same contract hold on the GPU.
+// GTest is not available here; the small harness below keeps the
+// TEST / EXPECT_* spelling. Exit code 77 when no GPU is present.
+#include <cmath>
+#include <cstdio>
+#include <functional>
+#include <string>
+#include <vector>
+
+#include "../../include/shb_baker.hpp"
+
+namespace {
+struct Case {
+  const char* name;
+  std::function<void()> fn;
+};
+std::vector<Case>& registry() {
+  static std::vector<Case> r;
+  return r;
+}
+struct Reg {
+  Reg(const char* n, std::function<void()> f) { registry().push_back({n, std::move(f)}); }
+};
+int g_failures = 0;
+bool g_case_failed = false;
+void report(const char* file, int line, const std::string& what) {
+  std::printf("  %s:%d: %s\n", file, line, what.c_str());
+  g_case_failed = true;
+}
+}  // namespace
+
+#define TEST(suite, name)                                                  \
+  static void suite##_##name();                                            \
+  static Reg reg_##suite##_##name(#suite "." #name, suite##_##name);       \
+  static void suite##_##name()
+#define CHECK_OP(a, op, b, fatal)                                                         \
+  do {                                                                                    \
+    auto va = (a);                                                                        \
+    auto vb = (b);                                                                        \
+    if (!(va op vb)) {                                                                    \
+      report(__FILE__, __LINE__,                                                          \
+             std::string(#a " " #op " " #b " failed: ") + std::to_string((double)va) +   \
+                 " vs " + std::to_string((double)vb));                                    \
+      if (fatal) return;                                                                  \
+    }                                                                                     \
+  } while (0)
+#define EXPECT_EQ(a, b) CHECK_OP(a, ==, b, false)
+#define ASSERT_EQ(a, b) CHECK_OP(a, ==, b, true)
+#define EXPECT_GT(a, b) CHECK_OP(a, >, b, false)
+#define EXPECT_GE(a, b) CHECK_OP(a, >=, b, false)
+#define EXPECT_NEAR(a, b, tol) CHECK_OP(std::fabs((double)(a) - (double)(b)), <=, (double)(tol), false)
+#define EXPECT_FLOAT_EQ(a, b) \
+  CHECK_OP(std::fabs((double)(a) - (double)(b)), <=, 4.0 * 1.1920929e-7 * std::fabs((double)(b)), false)
+#define EXPECT_TRUE(c) CHECK_OP((bool)(c), ==, true, false)
+#define EXPECT_FALSE(c) CHECK_OP((bool)(c), ==, false, false)
+#define ASSERT_TRUE(c) CHECK_OP((bool)(c), ==, true, true)
+
+using namespace shb_baker;
+
+static Geometry unit_quad(bool tangents) {
+  Geometry quad;
+  quad.material_id = 0;
+  quad.vertices = {{-1, -1, 0}, {1, -1, 0}, {1, 1, 0}, {-1, 1, 0}};
+  quad.normals = {{0, 0, 1}, {0, 0, 1}, {0, 0, 1}, {0, 0, 1}};
+  quad.texture_uvs = {{0, 0}, {1, 0}, {1, 1}, {0, 1}};
+  quad.lightmap_uvs = {{0, 0}, {1, 0}, {1, 1}, {0, 1}};
+  if (tangents) quad.tangents = {{1, 0, 0, 1}, {1, 0, 0, 1}, {1, 0, 0, 1}, {1, 0, 0, 1}};
+  quad.indices = {0, 1, 2, 0, 2, 3};
+  return quad;
+}
+
+// baker_test.cpp:10-65
+TEST(BakerTest, BakeSimpleQuad) {
+  Scene scene;
+  scene.geometries.push_back(unit_quad(false));
+  Material mat;
+  mat.name = "white";
+  mat.albedo.pixel_data = {255, 255, 255, 255};
+  mat.albedo.width = mat.albedo.height = 1;
+  mat.albedo.channels = 4;
+  scene.materials.push_back(mat);
+  Light light;
+  light.type = Light::Type::Directional;
+  light.direction = Vector3f(0, 0, -1);
+  scene.lights.push_back(light);
+
+  std::vector<SurfacePoint> surface_points(16);
+  RasterConfig raster_config;
+  raster_config.width = raster_config.height = 4;
+  surface_points[5].material_id = 0;
+  surface_points[5].position = Vector3f(-0.25f, -0.25f, 0.0f);
+  surface_points[5].normal = Vector3f(0.0f, 0.0f, 1.0f);
+  surface_points[5].tangent = Vector4f(1.0f, 0.0f, 0.0f, 1.0f);
+  BakeConfig config;
+  config.samples = 64;
+  config.bounces = 0;
+  BakeResult output = BakeSHLightMap(scene, surface_points, raster_config, config);
+  ASSERT_EQ(output.sh_texture.pixels.size(), (size_t)16);
+  SHCoeffs result = output.sh_texture.pixels[5];
+  EXPECT_GT(result.coeffs[0].x(), 0.1f);
+  // closed form: every sample adds Y(0,0,1), 64 x (1/64)
+  EXPECT_NEAR(result.coeffs[0].x(), 0.282095f, 1e-6f);
+  EXPECT_NEAR(result.coeffs[2].y(), 0.488603f, 1e-6f);
+  EXPECT_NEAR(result.coeffs[6].z(), 0.630784f, 1e-6f);
+  EXPECT_EQ(output.environment_visibility_texture.pixel_data[5], 1.0f);
+  EXPECT_EQ(output.sh_texture.pixels[0].coeffs[0].x(), 0.0f);
+}
+
+// baker_test.cpp:67-88
+TEST(BakerTest, DownsampleSHTexture) {
+  SHTexture input;
+  input.width = input.height = 2;
+  input.pixels.resize(4);
+  for (auto& p : input.pixels)
+    for (auto& c : p.coeffs) c = Vector3f(1.0f, 1.0f, 1.0f);
+  SHTexture output = DownsampleSHTexture(input, 2);
+  EXPECT_EQ(output.width, 1);
+  EXPECT_EQ(output.height, 1);
+  ASSERT_EQ(output.pixels.size(), (size_t)1);
+  for (int c = 0; c < 9; ++c) EXPECT_FLOAT_EQ(output.pixels[0].coeffs[c].x(), 1.0f);
+}
+
+// occlusion_test.cpp:18-62
+TEST(OcclusionTest, TriangleIntersection) {
+  Scene scene;
+  Geometry geo;
+  geo.vertices = {{-1.0f, -1.0f, 0.0f}, {1.0f, -1.0f, 0.0f}, {0.0f, 1.0f, 0.0f}};
+  geo.indices = {0, 1, 2};
+  scene.geometries.push_back(geo);
+  BvhScene bvh(scene);
+  ASSERT_TRUE(bvh.valid());
+  Ray ray_hit;
+  ray_hit.origin = Vector3f(0.0f, 0.0f, 1.0f);
+  ray_hit.direction = Vector3f(0.0f, 0.0f, -1.0f);
+  auto hit = FindOcclusion(bvh, ray_hit);
+  ASSERT_TRUE(hit.has_value());
+  EXPECT_NEAR(hit->position.z(), 0.0f, 1e-4f);
+  EXPECT_NEAR(hit->normal.z(), 1.0f, 1e-4f);
+  Ray ray_miss;
+  ray_miss.origin = Vector3f(2.0f, 0.0f, 1.0f);
+  ray_miss.direction = Vector3f(0.0f, 0.0f, -1.0f);
+  EXPECT_FALSE(FindOcclusion(bvh, ray_miss));
+  Ray ray_away;
+  ray_away.origin = Vector3f(0.0f, 0.0f, 1.0f);
+  ray_away.direction = Vector3f(0.0f, 0.0f, 1.0f);
+  EXPECT_FALSE(FindOcclusion(bvh, ray_away));
+}
+
+// occlusion_test.cpp:64-115
+TEST(OcclusionTest, RayClipping) {
+  Scene scene;
+  Geometry geo;
+  geo.vertices = {{-1.0f, -1.0f, 0.0f}, {1.0f, -1.0f, 0.0f}, {0.0f, 1.0f, 0.0f}};
+  geo.indices = {0, 1, 2};
+  scene.geometries.push_back(geo);
+  BvhScene bvh(scene);
+  ASSERT_TRUE(bvh.valid());
+  Ray ray;
+  ray.origin = Vector3f(0.0f, 0.0f, 1.0f);
+  ray.direction = Vector3f(0.0f, 0.0f, -1.0f);
+  ray.tfar = 0.5f;  // stops before the triangle at t = 1
+  EXPECT_FALSE(FindOcclusion(bvh, ray));
+  ray.tnear = 1.2f;  // starts behind it
+  ray.tfar = std::numeric_limits<float>::infinity();
+  EXPECT_FALSE(FindOcclusion(bvh, ray));
+  ray.tnear = 0.1f;
+  ray.tfar = 10.0f;
+  auto hit = FindOcclusion(bvh, ray);
+  ASSERT_TRUE(hit.has_value());
+  EXPECT_NEAR(hit->position.z(), 0.0f, 1e-4f);
+}
+
+// occlusion_test.cpp:117-184
+TEST(OcclusionTest, MultiGeometryMaterialTest) {
+  Scene scene;
+  Geometry geo_a, geo_b;
+  geo_a.vertices = {{-3, -1, 0}, {-1, -1, 0}, {-3, 1, 0}, {-1, 1, 0}};
+  geo_a.indices = {0, 1, 2, 1, 3, 2};
+  geo_a.material_id = 10;
+  geo_b.vertices = {{1, -1, 0}, {3, -1, 0}, {1, 1, 0}, {3, 1, 0}};
+  geo_b.indices = {0, 1, 2, 1, 3, 2};
+  geo_b.material_id = 20;
+  scene.geometries.push_back(geo_a);
+  scene.geometries.push_back(geo_b);
+  BvhScene bvh(scene);
+  ASSERT_TRUE(bvh.valid());
+  Ray ray;
+  ray.origin = Vector3f(2.8f, 0.8f, 5.0f);
+  ray.direction = Vector3f(0.0f, 0.0f, -1.0f);
+  auto hit = FindOcclusion(bvh, ray);
+  ASSERT_TRUE(hit.has_value());
+  EXPECT_EQ(hit->material_id, 20);
+  EXPECT_NEAR(hit->position.x(), 2.8f, 1e-4f);
+  EXPECT_NEAR(hit->position.y(), 0.8f, 1e-4f);
+  EXPECT_NEAR(hit->position.z(), 0.0f, 1e-4f);
+}
+
+// rasterizer_test.cpp:7-43
+TEST(RasterizerTest, RasterizeQuad) {
+  Scene scene;
+  scene.geometries.push_back(unit_quad(true));
+  RasterConfig config;
+  config.width = config.height = 4;
+  std::vector<SurfacePoint> result = RasterizeScene(scene, config);
+  ASSERT_EQ(result.size(), (size_t)16);
+  for (int i = 0; i < 16; ++i) EXPECT_GE(result[i].material_id, 0);
+  EXPECT_NEAR(result[5].position.x(), -0.25f, 0.001f);
+  EXPECT_NEAR(result[5].position.y(), -0.25f, 0.001f);
+  EXPECT_NEAR(result[5].normal.z(), 1.0f, 0.001f);
+}
+
+// rasterizer_test.cpp:45-72
+TEST(RasterizerTest, RasterizeQuadSupersampled) {
+  Scene scene;
+  scene.geometries.push_back(unit_quad(true));
+  RasterConfig config;
+  config.width = config.height = 4;
+  config.supersample_scale = 2;
+  std::vector<SurfacePoint> result = RasterizeScene(scene, config);
+  ASSERT_EQ(result.size(), (size_t)64);
+  for (int i = 0; i < 64; ++i) EXPECT_GE(result[i].material_id, 0);
+}
+
+// rasterizer_test.cpp:74-87
+TEST(RasterizerTest, ValidityMask) {
+  std::vector<SurfacePoint> points(3);
+  points[0].material_id = 0;
+  points[1].material_id = -1;
+  points[2].material_id = 0;
+  std::vector<uint8_t> mask = CreateValidityMask(points);
+  ASSERT_EQ(mask.size(), (size_t)3);
+  EXPECT_EQ(mask[0], 1);
+  EXPECT_EQ(mask[1], 0);
+  EXPECT_EQ(mask[2], 1);
+}
+
+// rasterizer_test.cpp:89-126
+TEST(RasterizerTest, DownsampleValidityMask) {
+  std::vector<uint8_t> points(16, 0);
+  points[2] = 1;
+  points[8] = points[9] = points[12] = points[13] = 1;
+  points[15] = 1;
+  std::vector<uint8_t> mask = DownsampleValidityMask(points, 2, 2, 2);
+  ASSERT_EQ(mask.size(), (size_t)4);
+  EXPECT_EQ(mask[0], 0);
+  EXPECT_EQ(mask[1], 1);
+  EXPECT_EQ(mask[2], 1);
+  EXPECT_EQ(mask[3], 1);
+}
+
+// dilation_test.cpp:11-33
+TEST(DilationTest, SimpleFill) {
+  int w = 3, h = 3;
+  std::vector<SHCoeffs> pixels(w * h);
+  std::vector<uint8_t> mask(w * h, 0);
+  int center = 1 * w + 1;
+  mask[center] = 1;
+  pixels[center].coeffs[0] = Vector3f(1.0f, 0.0f, 0.0f);
+  Dilate(w, h, pixels, mask, 1);
+  EXPECT_EQ(mask[1], 1);
+  EXPECT_NEAR(pixels[1].coeffs[0].x(), 1.0f, 0.001f);
+  EXPECT_EQ(mask[0], 1);
+  EXPECT_NEAR(pixels[0].coeffs[0].x(), 1.0f, 0.001f);
+}
+
+// dilation_test.cpp:35-58
+TEST(DilationTest, AverageNeighbors) {
+  int w = 3, h = 3;
+  std::vector<SHCoeffs> pixels(w * h);
+  std::vector<uint8_t> mask(w * h, 0);
+  int left = 1 * w + 0, right = 1 * w + 2, center = 1 * w + 1;
+  mask[left] = 1;
+  pixels[left].coeffs[0] = Vector3f(1.0f, 0.0f, 0.0f);
+  mask[right] = 1;
+  pixels[right].coeffs[0] = Vector3f(0.0f, 0.0f, 1.0f);
+  Dilate(w, h, pixels, mask, 1);
+  EXPECT_EQ(mask[center], 1);
+  EXPECT_NEAR(pixels[center].coeffs[0].x(), 0.5f, 0.001f);
+  EXPECT_NEAR(pixels[center].coeffs[0].z(), 0.5f, 0.001f);
+}
+
+int main() {
+  shb_ctx* probe = nullptr;
+  if (shb_create(&probe, 0) != SHB_OK) {
+    std::printf("SKIP: no GPU (%s)\n", shb_version());
+    return 77;
+  }
+  shb_destroy(probe);
+  int passed = 0;
+  for (const Case& c : registry()) {
+    g_case_failed = false;
+    std::printf("[ RUN  ] %s\n", c.name);
+    std::fflush(stdout);
+    c.fn();
+    std::printf("[ %s ] %s\n", g_case_failed ? "FAIL" : " OK ", c.name);
+    std::fflush(stdout);
+    if (g_case_failed) ++g_failures; else ++passed;
+  }
+  std::printf("%d passed, %d failed\n", passed, g_failures);
+  return g_failures ? 1 : 0;
+}

@@ -126,6 +126,22 @@ def test_lightmap_pipeline_bit_exact(baker, room, ss, dil):
     assert np.array_equal(vis.view(np.uint32), e_vis.reshape(vis.shape).view(np.uint32))
 
 
+def test_lightmap_device_resident_read_back(baker, room):
+    s, _ = room
+    baker.upload_scene(s)
+    baker.rasterize(32, 32, 2, want=False)
+    sh, vis, mask, _ = baker.bake_lightmap(8, 2, 1, seed=4)
+    none = baker.bake_lightmap(8, 2, 1, seed=4, to_host=False)
+    assert none[0] is None
+    sh2, vis2, mask2 = baker.read_lightmap()
+    assert np.array_equal(sh.view(np.uint32), sh2.view(np.uint32))
+    assert np.array_equal(vis.view(np.uint32), vis2.view(np.uint32))
+    np.testing.assert_array_equal(mask, mask2)
+    baker.rasterize(32, 32, 2, want=False)  # invalidates the device result
+    with pytest.raises(shb.ShbError):
+        baker.read_lightmap()
+
+
 def test_bake_errors(baker):
     s = scenes.box_room_scene(64, 64)
     fresh = shb.Baker(0)
