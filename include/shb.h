@@ -169,7 +169,9 @@ typedef struct shb_bake_stats {
 typedef struct shb_tuning {
   int32_t block_size;       /* threads per block of the traversal kernels   */
   int32_t batch_paths;      /* path slots resident per wavefront batch      */
-  int32_t reserved;         /* must be 0                                    */
+  int32_t queue_chunk;      /* 0: one ray per thread; 1..64: persistent
+                               traversal, a wave refills once this many of
+                               its lanes are idle                          */
   int32_t bvh_width;        /* children per BVH node, 2..8 (rebuilds)       */
 } shb_tuning;
 

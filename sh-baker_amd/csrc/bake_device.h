@@ -101,6 +101,22 @@ struct NeeResult {
 // for each light draw u, W += w, keep light i when u*W < w -- the same
 // w_i / sum(w) selection law as std::discrete_distribution. Returns false
 // when nothing is selected, sum(w) < 1e-6 or joint pdf < 1e-3.
+// Point on an area-light triangle at folded barycentrics (u1, u2)
+// (light.cpp:34-68): position, area pdf and emission texture coordinates.
+__device__ __forceinline__ void area_point(const AreaTri& at, float u1, float u2, f3& target,
+                                           float& ap, float& tu, float& tv) {
+  float w = 1.0f - u1 - u2;
+  f3 v0 = {at.v0[0], at.v0[1], at.v0[2]};
+  f3 v1 = {at.v1[0], at.v1[1], at.v1[2]};
+  f3 v2 = {at.v2[0], at.v2[1], at.v2[2]};
+  target = w * v0 + u1 * v1 + u2 * v2;
+  ap = at.pdf;
+  if (at.has_uv != 0.f) {
+    tu = w * at.uv[0] + u1 * at.uv[2] + u2 * at.uv[4];
+    tv = w * at.uv[1] + u1 * at.uv[3] + u2 * at.uv[5];
+  }
+}
+
 template <bool VERTEX>
 __device__ __forceinline__ bool nee_sample(const SceneView& sc, f3 P, f3 N,
                                            f3 brdf, bool front, Rng& rng,
@@ -111,7 +127,9 @@ __device__ __forceinline__ bool nee_sample(const SceneView& sc, f3 P, f3 N,
   f3 prad = {0.f, 0.f, 0.f}, pdir = {0.f, 0.f, 0.f};
   const int nl = sc.num_lights;
   for (int l = 0; l < nl; ++l) {
-    const LightDev& L = sc.lights[l];
+    // every lane visits the same light: a wave-uniform index lets the light
+    // record come through the scalar cache
+    const LightDev L = uniform_load(sc.lights, (uint32_t)l);
     f3 rad = {0.f, 0.f, 0.f}, sdir;
     float tfar, ap = 1.0f;
     const int type = L.type;
@@ -140,27 +158,20 @@ __device__ __forceinline__ bool nee_sample(const SceneView& sc, f3 P, f3 N,
           uint32_t tri = rng.index(L.num_tris);
           float u1 = rng.uniform();
           float u2 = rng.uniform();
-          const AreaTri& at = sc.area_tris[L.first_tri + tri];
           if (u1 + u2 > 1.0f) {
             u1 = 1.0f - u1;
             u2 = 1.0f - u2;
           }
-          float w = 1.0f - u1 - u2;
-          f3 v0 = {at.v0[0], at.v0[1], at.v0[2]};
-          f3 v1 = {at.v1[0], at.v1[1], at.v1[2]};
-          f3 v2 = {at.v2[0], at.v2[1], at.v2[2]};
-          target = w * v0 + u1 * v1 + u2 * v2;
-          ap = at.pdf;
-          if (L.emit_const) {
+          // single-triangle emitters (tri is 0) read a wave-uniform record
+          float tu = 0.f, tv = 0.f;
+          if (L.num_tris == 1)
+            area_point(uniform_load(sc.area_tris, L.first_tri), u1, u2, target, ap, tu, tv);
+          else
+            area_point(sc.area_tris[L.first_tri + tri], u1, u2, target, ap, tu, tv);
+          if (L.emit_const)
             emit = f3{L.emit[0], L.emit[1], L.emit[2]};
-          } else {
-            float tu = 0.f, tv = 0.f;
-            if (at.has_uv != 0.f) {
-              tu = w * at.uv[0] + u1 * at.uv[2] + u2 * at.uv[4];
-              tv = w * at.uv[1] + u1 * at.uv[3] + u2 * at.uv[5];
-            }
+          else
             emit = get_emission(sc, L.material, tu, tv);
-          }
         }
       } else {
         target = f3{L.pos[0], L.pos[1], L.pos[2]};
@@ -189,8 +200,9 @@ __device__ __forceinline__ bool nee_sample(const SceneView& sc, f3 P, f3 N,
         rad = cmul(rad, cos_n * bv);
       }
     }
-    const float* ls = sc.light_scale + 3 * l;
-    rad = cmul(rad, f3{ls[0], ls[1], ls[2]});
+    const uint32_t l3 = 3u * (uint32_t)l;
+    rad = cmul(rad, f3{uniform_load(sc.light_scale, l3), uniform_load(sc.light_scale, l3 + 1),
+                       uniform_load(sc.light_scale, l3 + 2)});
     float w = maxc(rad);
     float u = rng.uniform();
     W += w;

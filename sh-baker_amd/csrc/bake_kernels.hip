@@ -10,6 +10,8 @@
 // Mirrors BakeSHLightMap / Trace / EvaluateLightSamples /
 // AccumulateIncomingLightSamples (reference src/baker.cpp:70-304,
 // src/light.cpp:82-293).
+#include <algorithm>
+
 #include "../../include/shb.h"
 #include "bake_device.h"
 #include "kernels.h"
@@ -77,41 +79,31 @@ struct BakeShadowEpilogue {
 
 // Traversal stack in LDS: entry-major, lane-minor (conflict-free per wave).
 struct LdsStack {
-  uint2* mem;  // the kernel's __shared__ array
+  uint2* mem;  // the kernel's dynamic __shared__ array
   int tid;
   int stride;
+  int entries;
   __device__ uint2 get(int sp) const { return mem[sp * stride + tid]; }
   __device__ void set(int sp, uint2 v) const { mem[sp * stride + tid] = v; }
+  __device__ int cap() const { return entries; }
 };
 
-// One ray per thread over a queue whose length is read on the device, so
-// the grid is sized for the queue's capacity and surplus threads exit.
-template <bool ANY, bool COUNT, class Epi>
-__global__ void __launch_bounds__(256) k_trace(TraceArgs a, Epi epi) {
-  __shared__ uint2 stack_mem[kStackLds * 256];
-  const uint32_t count = a.count_ptr ? *a.count_ptr : a.count_const;
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= count) return;
-  const uint32_t slot = a.queue ? a.queue[i] : i;
+// Ray setup shared by the traversal kernels.
+__device__ __forceinline__ void trace_begin(const TraceArgs& a, uint32_t slot, TravRay& r,
+                                            TravState& s) {
   const float4 o = a.rays[2 * (size_t)slot];
   const float4 d = a.rays[2 * (size_t)slot + 1];
-  TravRay r;
   trav_setup(r, o, d);
-  TravState s;
   trav_init(s, r, d.w);
   // empty scene, or a ray without a usable direction / origin: miss
   if (!a.has_nodes || !ray_is_finite(o, d)) s.gbits = 0;
-  LdsStack stack{stack_mem, (int)threadIdx.x, (int)blockDim.x};
-  bool overflow = false;
-  bool capped = false;
-  uint32_t steps = 0;
-  while (!trav_step<ANY, COUNT>(a.scene, r, s, stack, overflow)) {
-    // a ray visits each node at most once: more steps means a corrupt tree
-    if (++steps > a.max_steps) {
-      capped = true;
-      break;
-    }
-  }
+}
+
+// Error report, optional visit counts and the epilogue of a finished ray.
+template <bool COUNT, class Epi>
+__device__ __forceinline__ void trace_finish(const TraceArgs& a, const Epi& epi, uint32_t slot,
+                                             uint32_t i, TravState& s, bool overflow,
+                                             bool capped, uint32_t steps) {
   if (overflow || capped) {
     // error[0]: bit 0 stack overflow, bit 1 step cap; the first failing ray
     // records slot + 1, its stack depth and its step count in error[1..3]
@@ -134,6 +126,117 @@ __global__ void __launch_bounds__(256) k_trace(TraceArgs a, Epi epi) {
     }
   }
   epi(slot, s);
+}
+
+// One ray per thread over a queue whose length is read on the device, so
+// the grid is sized for the queue's capacity and surplus threads exit.
+template <bool ANY, bool COUNT, class Epi>
+__global__ void __launch_bounds__(256) k_trace(TraceArgs a, Epi epi) {
+  extern __shared__ uint2 stack_mem[];  // a.stack_entries * blockDim.x
+  const uint32_t count = a.count_ptr ? *a.count_ptr : a.count_const;
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const uint32_t slot = a.queue ? a.queue[i] : i;
+  TravRay r;
+  TravState s;
+  trace_begin(a, slot, r, s);
+  LdsStack stack{stack_mem, (int)threadIdx.x, (int)blockDim.x, (int)a.stack_entries};
+  bool overflow = false;
+  bool capped = false;
+  uint32_t steps = 0;
+  while (!trav_step<ANY, COUNT>(a.scene, r, s, stack, overflow)) {
+    // a ray visits each node at most once: more steps means a corrupt tree
+    if (++steps > a.max_steps) {
+      capped = true;
+      break;
+    }
+  }
+  trace_finish<COUNT>(a, epi, slot, i, s, overflow, capped, steps);
+}
+
+// Persistent form: every wave keeps up to 64 rays in flight. When at least
+// a.refill lanes are idle they take the next entries of the wave's pool, a
+// range of kFetch queue positions claimed with one atomic on a.head, so lanes
+// do not wait for the wave's longest ray and most refills need no atomic.
+// Exit: once a claim reaches the queue's end the wave is `drained` (a.head
+// only grows); it leaves when drained, its pool is empty and no lane holds a
+// ray. Every ray ends by the step cap at the latest.
+constexpr uint32_t kFetch = 128;  // >= 64: one claim serves every idle lane
+
+template <bool ANY, bool COUNT, class Epi>
+__global__ void __launch_bounds__(256) k_trace_persistent(TraceArgs a, Epi epi) {
+  extern __shared__ uint2 stack_mem[];  // a.stack_entries * blockDim.x
+  const uint32_t count = a.count_ptr ? *a.count_ptr : a.count_const;
+  const uint32_t lane = lane_id();
+  const uint64_t lanes_below = (1ull << lane) - 1ull;
+  LdsStack stack{stack_mem, (int)threadIdx.x, (int)blockDim.x, (int)a.stack_entries};
+  TravRay r;
+  TravState s;
+  uint32_t slot = 0, qi = 0, steps = 0;
+  bool has = false, overflow = false;
+  // wave-uniform pool [pool_next, pool_end) of claimed queue positions
+  uint32_t pool_next = 0, pool_end = 0;
+  bool drained = false;
+  while (true) {
+    const uint64_t idle = __ballot(!has);
+    const uint32_t n_idle = (uint32_t)__popcll(idle);
+    const bool pool_empty = pool_next >= pool_end;
+    if (n_idle >= a.refill && !(drained && pool_empty)) {
+      const uint32_t take_old = min(pool_end - pool_next, n_idle);
+      const uint32_t old_base = pool_next;
+      pool_next += take_old;
+      uint32_t new_base = 0;
+      bool fresh = false;  // a claim of this pass holds positions < count
+      if (take_old < n_idle && !drained) {
+        const int leader = __ffsll((unsigned long long)idle) - 1;
+        if ((int)lane == leader) new_base = atomicAdd(a.head, kFetch);
+        new_base = __shfl(new_base, leader);
+        if (new_base >= count) {
+          drained = true;
+          pool_next = pool_end = 0;
+        } else {
+          if (new_base + kFetch >= count) drained = true;
+          pool_end = min(new_base + kFetch, count);
+          pool_next = min(new_base + (n_idle - take_old), pool_end);
+          fresh = true;
+        }
+      }
+      if (!has) {
+        const uint32_t rank = (uint32_t)__popcll(idle & lanes_below);
+        bool got = false;
+        if (rank < take_old) {
+          qi = old_base + rank;
+          got = true;
+        } else if (fresh) {
+          qi = new_base + (rank - take_old);
+          got = qi < count;
+        }
+        if (got) {
+          slot = a.queue ? a.queue[qi] : qi;
+          trace_begin(a, slot, r, s);
+          has = true;
+          overflow = false;
+          steps = 0;
+        }
+      }
+    }
+    if (__ballot(has) == 0ull) {
+      if (drained && pool_next >= pool_end) break;
+      continue;  // all idle, more work claimable: the next pass refills
+    }
+    if (has) {
+      bool done = trav_step<ANY, COUNT>(a.scene, r, s, stack, overflow);
+      bool capped = false;
+      if (!done && ++steps > a.max_steps) {
+        capped = true;
+        done = true;
+      }
+      if (done) {
+        trace_finish<COUNT>(a, epi, slot, qi, s, overflow, capped, steps);
+        has = false;
+      }
+    }
+  }
 }
 
 // ------------------------------------------------------------- fixed rays
@@ -237,8 +340,23 @@ template <bool ANY, bool COUNT, class Epi>
 static hipError_t launch_trace_t(const TraceArgs& a, const Epi& e, uint32_t capacity, int block,
                                  hipStream_t st) {
   if (capacity == 0) return hipSuccess;
+  if (a.stack_entries < 1 || a.stack_entries > (uint32_t)kStackLds) return hipErrorInvalidValue;
+  const size_t lds = (size_t)a.stack_entries * block * sizeof(uint2);
   uint32_t grid = (capacity + block - 1) / block;
-  hipLaunchKernelGGL((k_trace<ANY, COUNT, Epi>), dim3(grid), dim3(block), 0, st, a, e);
+  if (a.head && a.refill >= 1 && a.refill <= 64) {
+    // persistent: as many blocks as fit on the device at once (occupancy of
+    // this kernel with its LDS stack), no more than the queue could use
+    int per_cu = 0;
+    auto kern = k_trace_persistent<ANY, COUNT, Epi>;
+    hipError_t oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, block, lds);
+    if (oe != hipSuccess || per_cu < 1) per_cu = 1;
+    uint32_t resident = (uint32_t)std::max(1, a.num_cus) * (uint32_t)per_cu;
+    grid = std::min(grid, resident);
+    hipLaunchKernelGGL((k_trace_persistent<ANY, COUNT, Epi>), dim3(grid), dim3(block), lds, st,
+                       a, e);
+  } else {
+    hipLaunchKernelGGL((k_trace<ANY, COUNT, Epi>), dim3(grid), dim3(block), lds, st, a, e);
+  }
   return hipGetLastError();
 }
 

@@ -16,6 +16,10 @@ struct TraceArgs {
   uint32_t count_const;       // count when count_ptr is null
   int has_nodes;              // 0: empty scene
   uint32_t max_steps;         // per-ray traversal step cap (node count + 1)
+  uint32_t stack_entries;     // LDS stack entries per lane (tree depth - 1)
+  uint32_t* head;             // persistent form: queue position counter (zeroed)
+  uint32_t refill;            // persistent form: idle lanes that trigger a refill
+  int num_cus;                // persistent form: compute units of the device
   uint32_t* error;            // [0] bits: 1 stack overflow, 2 step cap; [1..3] first ray
   uint32_t* ray_counts;       // optional: node visits, triangle tests per slot
   unsigned long long* totals; // optional: 64 pairs of summed (nodes, tris)

@@ -69,6 +69,10 @@ TraceArgs trace_args(Ctx& c) {
   a.scene = c.view();
   a.has_nodes = c.bvh.nodes.empty() ? 0 : 1;
   a.max_steps = (uint32_t)c.bvh.nodes.size() + 1;
+  a.stack_entries = (uint32_t)std::max(1, std::min(c.bvh.max_depth - 1, kStackLds));
+  a.refill = (uint32_t)c.tuning.queue_chunk;
+  a.num_cus = c.num_cus;
+  a.head = nullptr;  // set per launch when the persistent form is used
   a.error = c.d_work.as<uint32_t>();
   return a;
 }
@@ -87,8 +91,9 @@ int check_overflow(Ctx& c) {
 }
 
 // Counter layout per batch (uint32): extension-queue counts by depth, shadow
-// counts by depth, work counters of the extension and shadow traversals.
-constexpr int kCntExt = 0, kCntSh = 20, kCntStride = 40;
+// counts by depth, queue heads of the persistent extension and shadow
+// traversals by depth.
+constexpr int kCntExt = 0, kCntSh = 20, kCntHeadExt = 40, kCntHeadSh = 56, kCntStride = 72;
 
 // SHB_DEBUG_STAGES=1: synchronise after every bake stage and log its time to
 // stderr (diagnostics; off by default).
@@ -236,6 +241,7 @@ int bake_device(Ctx& c, const shb_bake_cfg* cfg, uint32_t job, BakeRun* run) {
       ta.count_ptr = d == 0 ? nullptr : cnt + kCntExt + d;
       ta.count_const = b.texels_in_batch * spp;
       ta.totals = count ? totals : nullptr;
+      ta.head = ta.refill ? cnt + kCntHeadExt + d : nullptr;
       trace_ev(0);
       SHB_TRY(c, launch_trace_hits(ta, b.hit, count, b.texels_in_batch * spp, c.tuning.block_size, st));
       trace_ev(0);
@@ -250,6 +256,7 @@ int bake_device(Ctx& c, const shb_bake_cfg* cfg, uint32_t job, BakeRun* run) {
       sa.queue = b.shadow_queue;
       sa.count_ptr = cnt + kCntSh + d;
       sa.totals = count ? totals_sh : nullptr;
+      sa.head = sa.refill ? cnt + kCntHeadSh + d : nullptr;
       trace_ev(1);
       SHB_TRY(c, launch_trace_bake_shadow(sa, b, count, 2 * b.texels_in_batch * spp,
                                           c.tuning.block_size, st));
@@ -710,8 +717,11 @@ int shb_trace_rays(shb_ctx* h, const shb_ray* rays, uint32_t n, int32_t any_hit,
   const bool count = node_visits || tri_tests;
   if (count) SHB_TRY(c, c.d_raycounts.reserve((size_t)n * 8));
   SHB_TRY(c, hipMemcpyAsync(c.d_rays.ptr, rays, (size_t)n * sizeof(shb_ray), hipMemcpyHostToDevice, st));
+  SHB_TRY(c, c.d_counters.reserve(64));
   SHB_TRY(c, launch_fill_u32(c.d_work.as<uint32_t>(), 0u, kWorkBytes / 4, st));
+  SHB_TRY(c, launch_fill_u32(c.d_counters.as<uint32_t>(), 0u, 16, st));
   TraceArgs ta = trace_args(c);
+  if (ta.refill) ta.head = c.d_counters.as<uint32_t>();
   ta.rays = c.d_rays.as<float4>();
   ta.queue = nullptr;
   ta.count_ptr = nullptr;
@@ -751,8 +761,10 @@ int shb_set_tuning(shb_ctx* h, const shb_tuning* t) {
   Ctx& c = h->c;
   if (t->block_size != 64 && t->block_size != 128 && t->block_size != 256)
     return fail(c, SHB_ERR_INVALID_ARG, "block_size must be 64, 128 or 256");
-  if (t->batch_paths < 1024) return fail(c, SHB_ERR_INVALID_ARG, "batch_paths must be >= 1024");
-  if (t->reserved != 0) return fail(c, SHB_ERR_INVALID_ARG, "reserved must be 0");
+  if (t->batch_paths < 1024 || t->batch_paths > (1 << 26))
+    return fail(c, SHB_ERR_INVALID_ARG, "batch_paths must be in [1024, 2^26]");
+  if (t->queue_chunk < 0 || t->queue_chunk > 64)
+    return fail(c, SHB_ERR_INVALID_ARG, "queue_chunk must be in [0, 64]");
   if (t->bvh_width < 2 || t->bvh_width > 8)
     return fail(c, SHB_ERR_INVALID_ARG, "bvh_width must be in [2, 8]");
   bool rebuild = c.has_scene && t->bvh_width != c.tuning.bvh_width;

@@ -185,6 +185,33 @@ SHB_HD Rng site_rng(uint64_t seed, uint32_t texel, uint32_t sample,
   return r;
 }
 
+// A value the caller knows to be equal on every active lane, made visibly
+// uniform to the compiler (scalar registers, scalar-cache loads).
+//
+// uniform_load(p, i): p[i] for a wave-uniform i from memory no kernel writes
+// while it runs (scene tables), read through the constant address space so
+// it can be a scalar load.
+#if defined(SHB_HOST_ONLY)
+SHB_HD uint32_t uniform_index(uint32_t v) { return v; }
+SHB_HD int uniform_index(int v) { return v; }
+template <class T>
+SHB_HD T uniform_load(const T* p, uint32_t i) { return p[i]; }
+#else
+template <class T>
+__device__ __forceinline__ T uniform_load(const T* p, uint32_t i) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef const __attribute__((address_space(4))) T* cptr;
+  return ((cptr)p)[__builtin_amdgcn_readfirstlane((int)i)];
+#else
+  return p[i];
+#endif
+}
+__device__ __forceinline__ uint32_t uniform_index(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+}
+__device__ __forceinline__ int uniform_index(int v) { return __builtin_amdgcn_readfirstlane(v); }
+#endif
+
 // ------------------------------------------------------------ math
 // Reductions keep Eigen's fixed-size order e0 + (e1 + e2).
 struct f3 {

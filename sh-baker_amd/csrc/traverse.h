@@ -138,8 +138,9 @@ __device__ __forceinline__ void trav_init(TravState& s, const TravRay& r, float 
 
 // One traversal step: visit one node and test its triangles. Returns true
 // when the ray is finished (stack empty, or any-hit found).
-// Stack: any type with get(int) / set(int, uint2) over kStackLds entries
-// (on the GPU a slice of an LDS array, on the host a local array).
+// Stack: any type with get(int) / set(int, uint2) / cap() (entries; at most
+// kStackLds) — on the GPU a slice of the kernel's dynamic LDS, sized to the
+// uploaded tree's depth, on the host a local array.
 template <bool ANY, bool COUNT, class Stack>
 __device__ __forceinline__ bool trav_step(const SceneView& sc, const TravRay& r,
                                           TravState& s, Stack& stack, bool& overflow) {
@@ -157,7 +158,7 @@ __device__ __forceinline__ bool trav_step(const SceneView& sc, const TravRay& r,
   uint32_t im = (s.gbits >> 8) & 0xffu;
   uint32_t ni = s.gbase + __builtin_popcount(im & ((1u << slot) - 1u));
   if ((s.gbits & 0xffu) != 0u) {
-    if (s.sp >= kStackLds) {  // deeper than the BVH the upload accepted
+    if (s.sp >= stack.cap()) {  // deeper than the BVH the upload accepted
       overflow = true;
       return true;
     }

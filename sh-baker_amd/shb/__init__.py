@@ -94,7 +94,7 @@ class ShbBakeStats(C.Structure):
 
 class ShbTuning(C.Structure):
     _fields_ = [("block_size", C.c_int32), ("batch_paths", C.c_int32),
-                ("reserved", C.c_int32), ("bvh_width", C.c_int32)]
+                ("queue_chunk", C.c_int32), ("bvh_width", C.c_int32)]
 
 
 RAY_DTYPE = np.dtype([("origin", "<f4", 3), ("tnear", "<f4"), ("direction", "<f4", 3),

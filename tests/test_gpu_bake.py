@@ -97,10 +97,29 @@ def test_bake_batch_size_invariant(baker, room):
     baker.upload_scene(s)
     baker.rasterize(48, 48, 1, want=False)
     a = baker.bake(16, 3, seed=9)[0]
+    old = baker.tuning()
     baker.set_tuning(batch_paths=4096)
-    b = baker.bake(16, 3, seed=9)[0]
-    baker.set_tuning(batch_paths=1 << 23)
+    try:
+        b = baker.bake(16, 3, seed=9)[0]
+    finally:
+        baker.set_tuning(**old)
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_bake_persistent_traversal_bit_identical(baker, room):
+    s, _ = room
+    baker.upload_scene(s)
+    baker.rasterize(48, 48, 1, want=False)
+    old = baker.tuning()
+    try:
+        baker.set_tuning(queue_chunk=0, block_size=256)
+        a, av, _ = baker.bake(16, 3, seed=21)
+        baker.set_tuning(queue_chunk=16, block_size=64)
+        b, bv, _ = baker.bake(16, 3, seed=21)
+    finally:
+        baker.set_tuning(**old)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    assert np.array_equal(av.view(np.uint32), bv.view(np.uint32))
 
 
 @pytest.mark.parametrize("ss,dil", [(2, 2), (1, 3), (3, 0)])

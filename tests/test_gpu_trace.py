@@ -152,3 +152,25 @@ def test_visit_counts(baker):
     rays = seeded_rays(s, 5000, 8)
     hits, nv, tt = baker.trace_rays(rays, counts=True)
     assert (nv >= 1).all() and nv.mean() < 200 and tt.mean() < 400
+
+
+@pytest.mark.parametrize("chunk,block", [(0, 256), (1, 256), (8, 64), (32, 64), (64, 256)])
+def test_persistent_traversal_matches(baker, chunk, block):
+    # the persistent dynamic-fetch form returns exactly what the one-ray-per-
+    # thread form returns (same closest hits, same occlusion)
+    s = scenes.scaled_synthetic_scene(20000, 256, 9)
+    baker.upload_scene(s)
+    rays = seeded_rays(s, 30011, 11)
+    ref = orc.Oracle(s).trace(rays, mode=0)
+    old = baker.tuning()
+    baker.set_tuning(queue_chunk=chunk, block_size=block)
+    try:
+        compare_exact(baker.trace_rays(rays), ref)
+        rays["tfar"] = np.random.default_rng(5).uniform(0.01, 30.0, rays.size)
+        ha = baker.trace_rays(rays, any_hit=True)
+        ho = orc.Oracle(s).trace(rays, mode=2)
+        np.testing.assert_array_equal(ha["prim_id"] >= 0, ho["prim_id"] >= 0)
+        hc, nv, tt = baker.trace_rays(rays[:5000], counts=True)
+        assert (nv >= 1).all()
+    finally:
+        baker.set_tuning(**old)
