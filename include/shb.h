@@ -133,8 +133,12 @@ typedef struct shb_surface_point {
 
 /* ------------------------------------------------------------------ bake */
 enum {
-  SHB_BAKE_COUNT_VISITS = 1u << 0  /* also count BVH node visits / triangle
+  SHB_BAKE_COUNT_VISITS = 1u << 0, /* also count BVH node visits / triangle
                                       tests (slower; for measurement only) */
+  SHB_BAKE_USE_GRAPH = 1u << 1     /* capture the launch sequence in a hipGraph
+                                      and replay it while the plan, buffers and
+                                      tuning are unchanged (per-launch trace
+                                      timings are then not reported)          */
 };
 
 /* Reference `BakeConfig` (src/baker.h:10-14) plus the seed of the

@@ -88,15 +88,22 @@ struct LdsStack {
   __device__ int cap() const { return entries; }
 };
 
-// Ray setup shared by the traversal kernels.
+// Ray setup shared by the traversal kernels: the ray starts at its job's
+// BVH root (a batch of several jobs traverses their concatenated trees).
 __device__ __forceinline__ void trace_begin(const TraceArgs& a, uint32_t slot, TravRay& r,
                                             TravState& s) {
   const float4 o = a.rays[2 * (size_t)slot];
   const float4 d = a.rays[2 * (size_t)slot + 1];
   trav_setup(r, o, d);
   trav_init(s, r, d.w);
+  uint32_t root = a.root0;
+  if (a.nseg > 1) {
+    const uint32_t p = slot >= a.P ? slot - a.P : slot;  // texel shadow rays sit at P + p
+    root = a.map->root[find_segment(a.map->slot_base, a.nseg, p)];
+  }
+  s.gbase = root;
   // empty scene, or a ray without a usable direction / origin: miss
-  if (!a.has_nodes || !ray_is_finite(o, d)) s.gbits = 0;
+  if (!a.has_nodes || root == kNoRoot || !ray_is_finite(o, d)) s.gbits = 0;
 }
 
 // Error report, optional visit counts and the epilogue of a finished ray.
@@ -291,10 +298,14 @@ __global__ void k_hit_details(SceneView sc, const float4* rays, const float4* hi
 // ------------------------------------------------------------ bake stages
 // Primary ray + texel NEE for every path slot of the batch.
 __global__ void k_generate(BakeArgs a) {
-  const uint32_t n = a.texels_in_batch * a.spp;
+  const uint32_t n = a.P;
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
     uint32_t p = base + threadIdx.x;
-    bool shadow = p < n && generate_path(a, p);
+    bool shadow = false;
+    if (p < n) {
+      const SlotRef sr = slot_ref(a, p);
+      with_job(a, sr.job, [&](const JobDev& J) { shadow = generate_path(a, J, sr, p); });
+    }
     uint32_t at = wave_append(a.shadow_count, shadow);
     if (shadow) a.shadow_queue[at] = a.P + p;
   }
@@ -304,14 +315,16 @@ __global__ void k_generate(BakeArgs a) {
 __global__ void k_shade(BakeArgs a, int depth, const uint32_t* in_queue,
                         const uint32_t* in_count, uint32_t* out_queue,
                         uint32_t* out_count) {
-  const uint32_t n = in_queue ? *in_count : a.texels_in_batch * a.spp;
+  const uint32_t n = in_queue ? *in_count : a.P;
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
     uint32_t i = base + threadIdx.x;
     bool shadow = false, extend = false;
     uint32_t p = 0;
     if (i < n) {
       p = in_queue ? in_queue[i] : i;
-      shade_path(a, depth, p, shadow, extend);
+      const SlotRef sr = slot_ref(a, p);
+      with_job(a, sr.job,
+               [&](const JobDev& J) { shade_path(a, J, sr, depth, p, shadow, extend); });
     }
     uint32_t at = wave_append(a.shadow_count, shadow);
     if (shadow) a.shadow_queue[at] = p;
@@ -320,9 +333,36 @@ __global__ void k_shade(BakeArgs a, int depth, const uint32_t* in_queue,
   }
 }
 
+// One thread per texel of the batch.
 __global__ void k_project(BakeArgs a) {
-  uint32_t tl = blockIdx.x * blockDim.x + threadIdx.x;
-  if (tl < a.texels_in_batch) project_texel(a, tl);
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.T) return;
+  uint32_t job, rank, slot0, stride;
+  if (a.nseg <= 1) {
+    job = a.job0;
+    rank = a.texel_begin0 + t;
+    slot0 = t;
+    stride = a.T0;
+  } else {
+    const BatchMap& m = *a.map;
+    const uint32_t s = find_segment(m.tex_base, a.nseg, t);
+    const uint32_t tl = t - m.tex_base[s];
+    job = m.job[s];
+    rank = m.texel_begin[s] + tl;
+    slot0 = m.slot_base[s] + tl;
+    stride = m.tex_base[s + 1] - m.tex_base[s];
+  }
+  with_job(a, job, [&](const JobDev& J) { project_texel(a, J, rank, slot0, stride); });
+}
+
+__global__ void k_relocate_nodes(const CWNode* src, CWNode* dst, uint32_t n, uint32_t node_off,
+                                 uint32_t tri_off) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  CWNode nd = src[i];
+  nd.child_base += node_off;
+  nd.tri_base += tri_off;
+  dst[i] = nd;
 }
 
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n) {
@@ -401,8 +441,16 @@ hipError_t launch_shade(const BakeArgs& b, int depth, const uint32_t* in_q,
   return hipGetLastError();
 }
 
+hipError_t launch_relocate_nodes(const CWNode* src, CWNode* dst, uint32_t n, uint32_t node_off,
+                                 uint32_t tri_off, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_relocate_nodes, dim3((n + 255) / 256), dim3(256), 0, st, src, dst, n,
+                     node_off, tri_off);
+  return hipGetLastError();
+}
+
 hipError_t launch_project(const BakeArgs& b, hipStream_t st) {
-  uint32_t T = b.texels_in_batch;
+  uint32_t T = b.T;
   if (T == 0) return hipSuccess;
   hipLaunchKernelGGL(k_project, dim3((T + 127) / 128), dim3(128), 0, st, b);
   return hipGetLastError();

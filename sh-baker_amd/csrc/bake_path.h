@@ -1,6 +1,11 @@
 // Per-path bodies of the wavefront bake stages (generate, shade, project),
 // shared by the HIP kernels and by CPU builds that check them. References
 // are to DaviesX/sh-baker.
+//
+// A batch may hold several jobs (lightmaps); a path slot finds its job,
+// texel and sample through the batch's BatchMap. When every active lane of
+// a wave works on the same job, the job record is read as a wave-uniform
+// value (scalar loads); otherwise per lane.
 #ifndef SHB_BAKE_PATH_H_
 #define SHB_BAKE_PATH_H_
 
@@ -9,21 +14,63 @@
 
 namespace shb {
 
+// Where path slot p of the batch lives.
+struct SlotRef {
+  uint32_t job;
+  uint32_t rank;  // texel rank in the job's texel list
+  uint32_t smp;
+};
+
+__device__ __forceinline__ SlotRef slot_ref(const BakeArgs& a, uint32_t p) {
+  SlotRef r;
+  if (a.nseg <= 1) {
+    r.job = a.job0;
+    r.rank = a.texel_begin0 + p % a.T0;
+    r.smp = p / a.T0;
+    return r;
+  }
+  const BatchMap& m = *a.map;
+  const uint32_t s = find_segment(m.slot_base, a.nseg, p);
+  const uint32_t Ts = m.tex_base[s + 1] - m.tex_base[s];
+  const uint32_t rel = p - m.slot_base[s];
+  r.job = m.job[s];
+  r.rank = m.texel_begin[s] + rel % Ts;
+  r.smp = rel / Ts;
+  return r;
+}
+
+// Calls f(job record) with the record read wave-uniformly when all active
+// lanes share `job` (the common case: segments are large and contiguous).
+template <class F>
+__device__ __forceinline__ void with_job(const BakeArgs& a, uint32_t job, F&& f) {
+#if defined(SHB_HOST_ONLY)
+  f(a.jobs[job]);
+#else
+  const uint32_t j0 = uniform_index(job);
+  if (__ballot(job != j0) == 0ull) {
+    const JobDev J = uniform_load(a.jobs, j0);
+    f(J);
+  } else {
+    const JobDev J = a.jobs[job];
+    f(J);
+  }
+#endif
+}
+
 // Primary ray and texel NEE of path slot p (baker.cpp:245-276,
 // light.cpp:191-293). Returns true when a texel shadow ray was made (stored
 // at shadow slot P + p).
-__device__ __forceinline__ bool generate_path(const BakeArgs& a, uint32_t p) {
-  const uint32_t T = a.texels_in_batch;
-  uint32_t tl = p % T, smp = p / T;
-  uint32_t texel = a.texel_list[a.texel_begin + tl];
-  const float4* sp = a.surface + 3 * (size_t)texel;
+__device__ __forceinline__ bool generate_path(const BakeArgs& a, const JobDev& J,
+                                              const SlotRef& sr, uint32_t p) {
+  uint32_t texel = J.texel_list[sr.rank];
+  const float4* sp = J.surface + 3 * (size_t)texel;
   float4 q0 = sp[0], q1 = sp[1], q2 = sp[2];
   f3 P = {q0.x, q0.y, q0.z};
   f3 N = {q0.w, q1.x, q1.y};
   f3 T3 = {q1.z, q1.w, q2.x};
   float Tw = q2.y;
   f3 origin = P + N * 0.001f;
-  Rng rng = site_rng(a.seed, texel, smp, 0, 1, kSiteTexel, a.job);
+  Rng rng = site_rng(J.seed, texel, sr.smp, 0, 1, kSiteTexel, J.stream);
   float u1 = rng.uniform(), u2 = rng.uniform();
   f3 dl = hemisphere(u1, u2);
   f3 bit = cross(N, T3) * Tw;
@@ -35,7 +82,7 @@ __device__ __forceinline__ bool generate_path(const BakeArgs& a, uint32_t p) {
   a.li[p] = make_float4(0.f, 0.f, 0.f, 0.f);
   NeeResult nr;
   f3 zero = {0.f, 0.f, 0.f};
-  if (nee_sample<false>(a.scene, P, N, zero, false, rng, nr)) {
+  if (nee_sample<false>(J.scene, P, N, zero, false, rng, nr)) {
     a.nee0[p] = make_float4(nr.L.x, nr.L.y, nr.L.z, 0.f);
     a.nee0_dir[p] = make_float4(nr.dir.x, nr.dir.y, nr.dir.z, 0.f);
     size_t so = (size_t)a.P + p;
@@ -50,25 +97,24 @@ __device__ __forceinline__ bool generate_path(const BakeArgs& a, uint32_t p) {
 // Shading of path slot p's closest hit at depth `depth` (baker.cpp:94-146):
 // sky, emission at depth 0, vertex NEE (shadow slot p), cosine-weighted
 // continuation. Sets shadow / extend when a ray was made.
-__device__ __forceinline__ void shade_path(const BakeArgs& a, int depth, uint32_t p,
-                                           bool& shadow, bool& extend) {
+__device__ __forceinline__ void shade_path(const BakeArgs& a, const JobDev& J, const SlotRef& sr,
+                                           int depth, uint32_t p, bool& shadow, bool& extend) {
   shadow = extend = false;
-  const uint32_t T = a.texels_in_batch;
+  const SceneView& sc = J.scene;
   float4 h = a.hit[p];
   int id = (int)as_uint(h.y);
   if (id < 0) {
     if (depth == 0) a.li[p].w = 1.0f;  // sky seen directly (baker.cpp:83-91)
     return;
   }
-  uint32_t tl = p % T, smp = p / T;
-  uint32_t texel = a.texel_list[a.texel_begin + tl];
+  uint32_t texel = J.texel_list[sr.rank];
   float4 ro = a.ray[2 * (size_t)p], rd = a.ray[2 * (size_t)p + 1];
   f3 dir = f3_of(rd);
   f3 pos = f3_of(ro) + dir * h.x;
-  int4 info = a.scene.tri_info[id];
+  int4 info = sc.tri_info[id];
   const int mat = info.x;
-  const float4* sr = a.scene.shade + 4 * (size_t)id;
-  float4 s0 = sr[0], s1 = sr[1], s2 = sr[2], s3 = sr[3];
+  const float4* srec = sc.shade + 4 * (size_t)id;
+  float4 s0 = srec[0], s1 = srec[1], s2 = srec[2], s3 = srec[3];
   uint32_t fl = as_uint(s3.w);
   float u = h.z, v = h.w, w = 1.0f - u - v;
   f3 nrm;
@@ -85,10 +131,10 @@ __device__ __forceinline__ void shade_path(const BakeArgs& a, int depth, uint32_
   f3 thr = f3_of(th4);
   uint32_t branch = as_uint(th4.w);
   f3 add = {0.f, 0.f, 0.f};
-  if (depth == 0 && a.scene.mats[mat].emission > 0.0f) {
-    f3 em = get_emission(a.scene, mat, tu, tv);
+  if (depth == 0 && sc.mats[mat].emission > 0.0f) {
+    f3 em = get_emission(sc, mat, tu, tv);
     if (info.y >= 0) {
-      const float* ls = a.scene.light_scale + 3 * info.y;
+      const float* ls = sc.light_scale + 3 * info.y;
       em = cmul(em, f3{ls[0], ls[1], ls[2]});
     }
     add = cmul(thr, em);
@@ -96,10 +142,10 @@ __device__ __forceinline__ void shade_path(const BakeArgs& a, int depth, uint32_
   f3 hit_pos = pos + nrm * 0.001f;
   f3 wo = -dir;
   bool front = dot(nrm, wo) > 0.0f;
-  f3 brdf = lambert(a.scene, mat, tu, tv);
-  Rng rng = site_rng(a.seed, texel, smp, (uint32_t)depth, branch, kSiteVertex, a.job);
+  f3 brdf = lambert(sc, mat, tu, tv);
+  Rng rng = site_rng(J.seed, texel, sr.smp, (uint32_t)depth, branch, kSiteVertex, J.stream);
   NeeResult nr;
-  if (nee_sample<true>(a.scene, hit_pos, nrm, brdf, front, rng, nr)) {
+  if (nee_sample<true>(sc, hit_pos, nrm, brdf, front, rng, nr)) {
     f3 c = cmul(thr, nr.L);
     a.scontrib[p] = make_float4(c.x, c.y, c.z, 0.f);
     a.sray[2 * (size_t)p] = make_float4(hit_pos.x, hit_pos.y, hit_pos.z, kRayEps);
@@ -117,7 +163,7 @@ __device__ __forceinline__ void shade_path(const BakeArgs& a, int depth, uint32_
   f3 sd;
   float pdf;
   sample_lambert(nrm, r1, r2, sd, pdf);
-  if (!(pdf < 1e-3f) && depth < (int)a.bounces) {
+  if (!(pdf < 1e-3f) && depth < (int)J.bounces) {
     // EvalMaterial is zero unless both directions are above the normal and
     // the albedo is non-zero; a zero BRDF ends the path (every deeper term
     // is zero) instead of tracing it.
@@ -132,12 +178,12 @@ __device__ __forceinline__ void shade_path(const BakeArgs& a, int depth, uint32_
   }
 }
 
-// Folds texel tl's samples into SH9 (baker.cpp:284-294) with Neumaier-
+// Folds batch texel t's samples into SH9 (baker.cpp:284-294) with Neumaier-
 // compensated FP32 sums, in the reference order per sample: texel NEE term
 // along its unnormalised direction, then 2*pi*Li along the primary ray.
-__device__ __forceinline__ void project_texel(const BakeArgs& a, uint32_t tl) {
-  const uint32_t T = a.texels_in_batch;
-  uint32_t texel = a.texel_list[a.texel_begin + tl];
+__device__ __forceinline__ void project_texel(const BakeArgs& a, const JobDev& J, uint32_t rank,
+                                              uint32_t slot0, uint32_t stride) {
+  uint32_t texel = J.texel_list[rank];
   float sum[27], comp[27];
 #pragma unroll
   for (int k = 0; k < 27; ++k) sum[k] = comp[k] = 0.f;
@@ -151,8 +197,8 @@ __device__ __forceinline__ void project_texel(const BakeArgs& a, uint32_t tl) {
       comp[k] += (x - t) + sum[k];
     sum[k] = t;
   };
-  for (uint32_t smp = 0; smp < a.spp; ++smp) {
-    uint32_t p = smp * T + tl;
+  for (uint32_t smp = 0; smp < J.spp; ++smp) {
+    uint32_t p = slot0 + smp * stride;
     float4 n0 = a.nee0[p];
     float Y[9];
     if (n0.w != 0.f) {
@@ -178,11 +224,11 @@ __device__ __forceinline__ void project_texel(const BakeArgs& a, uint32_t tl) {
       }
     }
   }
-  const float inv = 1.0f / (float)a.spp;
-  float* o = a.sh_out + 27 * (size_t)texel;
+  const float inv = 1.0f / (float)J.spp;
+  float* o = J.sh_out + 27 * (size_t)texel;
 #pragma unroll
   for (int k = 0; k < 27; ++k) o[k] = (sum[k] + comp[k]) * inv;
-  a.vis_out[texel] = vis * inv;
+  J.vis_out[texel] = vis * inv;
 }
 
 }  // namespace shb

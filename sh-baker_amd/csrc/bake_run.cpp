@@ -1,0 +1,416 @@
+// Wavefront bake driver (see bake_run.h). Reference: BakeSHLightMap,
+// src/baker.cpp:186-304 (one lightmap); the batched form bakes several
+// lightmaps through the same batches.
+#include "bake_run.h"
+
+#include <cstring>
+#include <string>
+
+namespace shb {
+
+int prepare_job(JobIn& j) {
+  Ctx& c = *j.c;
+  const shb_bake_cfg* cfg = &j.cfg;
+  c.lm_sh = nullptr;  // the device buffers below are about to be reused
+  if (!c.has_scene) return fail(c, SHB_ERR_NO_SCENE, "no scene uploaded");
+  if (!c.has_surface)
+    return fail(c, SHB_ERR_NO_SURFACE, "no surface: call shb_rasterize or shb_set_surface");
+  if (cfg->samples < 1 || cfg->samples > kMaxSamples)
+    return fail(c, SHB_ERR_INVALID_ARG, "samples must be in [1, 65535]");
+  if (cfg->bounces < 0 || cfg->bounces > kMaxBounces)
+    return fail(c, SHB_ERR_INVALID_ARG, "bounces must be in [0, 15]");
+  if (cfg->num_light_samples != 1)
+    return fail(c, SHB_ERR_UNSUPPORTED, "num_light_samples must be 1");
+  if (c.hs.bad_material)
+    return fail(c, SHB_ERR_INVALID_ARG,
+                "a geometry's material_id does not name a material (the bake shades every hit)");
+  if (c.hs.has_transparency)
+    return fail(c, SHB_ERR_UNSUPPORTED,
+                "materials with alpha < 1 (path splitting) are not supported by the GPU bake");
+  SHB_TRY(c, hipSetDevice(c.device));
+  const size_t npx = (size_t)c.width * c.ss * c.height * c.ss;
+  hipStream_t st = c.stream;
+  SHB_TRY(c, c.d_sh.reserve(std::max<size_t>(npx * 27 * sizeof(float), 16)));
+  SHB_TRY(c, c.d_vis.reserve(std::max<size_t>(npx * sizeof(float), 16)));
+  SHB_TRY(c, launch_fill_f32(c.d_sh.as<float>(), 0.f, npx * 27, st));
+  SHB_TRY(c, launch_fill_f32(c.d_vis.as<float>(), 0.f, npx, st));
+  SHB_TRY(c, c.d_mask.reserve(std::max<size_t>(npx, 16)));
+  SHB_TRY(c, c.d_list.reserve(std::max<size_t>(npx * 4 + 16, 64)));
+  SHB_TRY(c, c.d_tmp.reserve(((size_t)compact_blocks((uint32_t)npx) + 1) * 4 + 64));
+  SHB_TRY(c, launch_validity_mask(c.d_surface.as<float4>(), npx, c.d_mask.as<uint8_t>(), st));
+  SHB_TRY(c, launch_compact_valid(c.d_mask.as<uint8_t>(), (uint32_t)npx, c.d_list.as<uint32_t>(),
+                                  c.d_tmp.as<uint32_t>(), st));
+  uint32_t V = 0;
+  SHB_TRY(c, hipMemcpyAsync(&V, c.d_list.as<uint32_t>() + npx, 4, hipMemcpyDeviceToHost, st));
+  SHB_TRY(c, hipStreamSynchronize(st));
+  j.valid = V;
+  return SHB_OK;
+}
+
+namespace {
+
+struct Seg {
+  uint32_t job, begin, T, spp;
+};
+
+// Greedy packing of every job's valid texels into batches of at most
+// batch_paths path slots (a texel's samples never split across batches).
+std::vector<std::vector<Seg>> plan_batches(const std::vector<JobIn>& jobs, uint64_t cap) {
+  std::vector<std::vector<Seg>> out;
+  std::vector<Seg> cur;
+  uint64_t used = 0;
+  auto flush = [&]() {
+    if (!cur.empty()) out.push_back(cur);
+    cur.clear();
+    used = 0;
+  };
+  for (uint32_t j = 0; j < jobs.size(); ++j) {
+    const uint32_t spp = (uint32_t)jobs[j].cfg.samples;
+    uint32_t left = jobs[j].valid, begin = 0;
+    while (left > 0) {
+      if (cur.size() == (size_t)kMaxJobs) flush();
+      uint64_t room = cap > used ? cap - used : 0;
+      uint32_t t = (uint32_t)std::min<uint64_t>(left, room / spp);
+      if (t == 0) {
+        if (!cur.empty()) {
+          flush();
+          continue;
+        }
+        t = 1;  // one texel's samples exceed the batch size
+      }
+      cur.push_back({j, begin, t, spp});
+      used += (uint64_t)t * spp;
+      begin += t;
+      left -= t;
+    }
+  }
+  flush();
+  return out;
+}
+
+// Launch recorder: launches, or (dry run) appends each launch's arguments
+// to a key that identifies the recorded sequence.
+struct Rec {
+  bool dry = false;
+  std::string* key = nullptr;
+  template <class T>
+  void add(int kind, const T& v) {
+    key->append(reinterpret_cast<const char*>(&kind), sizeof(kind));
+    key->append(reinterpret_cast<const char*>(&v), sizeof(T));
+  }
+};
+
+}  // namespace
+
+int run_bake(Ctx& x, std::vector<JobIn>& jobs, bool graph, BakeRun* run) {
+  SHB_TRY(x, hipSetDevice(x.device));
+  hipStream_t st = x.stream;
+  const uint32_t nj = (uint32_t)jobs.size();
+  const bool count = (jobs[0].cfg.flags & SHB_BAKE_COUNT_VISITS) != 0;
+  run->valid = 0;
+  for (const JobIn& j : jobs) run->valid += j.valid;
+
+  // traversal arrays: one job uses its own BVH; several jobs get their trees
+  // concatenated (child and triangle bases shifted), each ray starting at its
+  // job's root
+  std::vector<uint32_t> roots(nj, kNoRoot);
+  const CWNode* tnodes = nullptr;
+  const float4* ttris = nullptr;
+  size_t total_nodes = 0, total_tris = 0;
+  int depth = 1;
+  for (const JobIn& j : jobs) {
+    total_nodes += j.c->bvh.nodes.size();
+    total_tris += j.c->bvh.tris.size() / 3;
+    depth = std::max(depth, j.c->bvh.max_depth);
+  }
+  if (nj == 1) {
+    tnodes = jobs[0].c->d_nodes.as<CWNode>();
+    ttris = jobs[0].c->d_tris.as<float4>();
+    if (!jobs[0].c->bvh.nodes.empty()) roots[0] = 0;
+  } else {
+    SHB_TRY(x, x.d_cnodes.reserve(std::max<size_t>(total_nodes * sizeof(CWNode), 16)));
+    SHB_TRY(x, x.d_ctris.reserve(std::max<size_t>(total_tris * 3 * sizeof(float4), 16)));
+    size_t no = 0, to = 0;
+    for (uint32_t k = 0; k < nj; ++k) {
+      const Ctx& c = *jobs[k].c;
+      const size_t nn = c.bvh.nodes.size(), nt = c.bvh.tris.size() / 3;
+      if (nn) {
+        roots[k] = (uint32_t)no;
+        SHB_TRY(x, launch_relocate_nodes(c.d_nodes.as<CWNode>(), x.d_cnodes.as<CWNode>() + no,
+                                         (uint32_t)nn, (uint32_t)no, (uint32_t)to, st));
+        SHB_TRY(x, hipMemcpyAsync(x.d_ctris.as<float4>() + 3 * to, c.d_tris.ptr,
+                                  nt * 3 * sizeof(float4), hipMemcpyDeviceToDevice, st));
+      }
+      no += nn;
+      to += nt;
+    }
+    tnodes = x.d_cnodes.as<CWNode>();
+    ttris = x.d_ctris.as<float4>();
+  }
+
+  // job table
+  std::vector<JobDev> jt(nj);
+  for (uint32_t k = 0; k < nj; ++k) {
+    Ctx& c = *jobs[k].c;
+    JobDev& J = jt[k];
+    std::memset(&J, 0, sizeof(J));
+    J.scene = c.view();
+    J.surface = c.d_surface.as<float4>();
+    J.texel_list = c.d_list.as<uint32_t>();
+    J.sh_out = c.d_sh.as<float>();
+    J.vis_out = c.d_vis.as<float>();
+    J.seed = jobs[k].cfg.seed;
+    J.spp = (uint32_t)jobs[k].cfg.samples;
+    J.bounces = (uint32_t)jobs[k].cfg.bounces;
+    J.stream = jobs[k].stream;
+  }
+  SHB_TRY(x, x.d_jobs.reserve(std::max<size_t>(nj * sizeof(JobDev), 16)));
+  SHB_TRY(x, hipMemcpyAsync(x.d_jobs.ptr, jt.data(), nj * sizeof(JobDev), hipMemcpyHostToDevice,
+                            st));
+
+  // batches and their slot maps
+  const auto batches = plan_batches(jobs, (uint64_t)x.tuning.batch_paths);
+  const uint32_t nb = (uint32_t)batches.size();
+  std::vector<BatchMap> maps(std::max<uint32_t>(nb, 1));
+  std::vector<uint32_t> maxB(nb, 0);
+  uint32_t Pmax = 0;
+  for (uint32_t bi = 0; bi < nb; ++bi) {
+    BatchMap& m = maps[bi];
+    std::memset(&m, 0, sizeof(m));
+    m.nseg = (uint32_t)batches[bi].size();
+    for (uint32_t s = 0; s < m.nseg; ++s) {
+      const Seg& g = batches[bi][s];
+      m.slot_base[s + 1] = m.slot_base[s] + g.T * g.spp;
+      m.tex_base[s + 1] = m.tex_base[s] + g.T;
+      m.job[s] = g.job;
+      m.texel_begin[s] = g.begin;
+      m.root[s] = roots[g.job];
+      maxB[bi] = std::max(maxB[bi], (uint32_t)jobs[g.job].cfg.bounces);
+    }
+    Pmax = std::max(Pmax, m.slot_base[m.nseg]);
+  }
+  SHB_TRY(x, x.d_maps.reserve(maps.size() * sizeof(BatchMap)));
+  SHB_TRY(x, hipMemcpyAsync(x.d_maps.ptr, maps.data(), maps.size() * sizeof(BatchMap),
+                            hipMemcpyHostToDevice, st));
+
+  // path state: 13 float4 per slot; queues 4 uint32 per slot
+  SHB_TRY(x, x.d_paths.reserve(std::max<size_t>((size_t)Pmax * 13 * sizeof(float4), 16)));
+  SHB_TRY(x, x.d_queues.reserve(std::max<size_t>((size_t)Pmax * 4 * sizeof(uint32_t), 16)));
+  SHB_TRY(x, x.d_counters.reserve(std::max<size_t>((size_t)nb * kCntStride * 4, 64)));
+  SHB_TRY(x, x.d_work.reserve(kWorkBytes));
+  float4* base = x.d_paths.as<float4>();
+  uint32_t* q = x.d_queues.as<uint32_t>();
+  unsigned long long* totals = reinterpret_cast<unsigned long long*>(x.d_work.as<uint32_t>() + 4);
+  unsigned long long* totals_sh = totals + 128;
+  const int sgrid = std::max(1, std::min<int>((int)((Pmax + 255) / 256), x.num_cus * 16));
+
+  TraceArgs tbase{};
+  tbase.scene.nodes = tnodes;
+  tbase.scene.tris = ttris;
+  tbase.scene.has_nodes = total_nodes ? 1 : 0;
+  tbase.has_nodes = total_nodes ? 1 : 0;
+  tbase.max_steps = (uint32_t)total_nodes + 1;
+  tbase.stack_entries = (uint32_t)std::max(1, std::min(depth - 1, kStackLds));
+  tbase.refill = (uint32_t)x.tuning.queue_chunk;
+  tbase.num_cus = x.num_cus;
+  tbase.error = x.d_work.as<uint32_t>();
+
+  // event pairs around each traversal launch; kind 0 extension, 1 shadow
+  std::vector<hipEvent_t> tev;
+  std::vector<int> tkind;
+  const bool time_traces = !graph;
+  auto trace_ev = [&](int kind) {
+    if (!time_traces) return;
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) e = nullptr;
+    tev.push_back(e);
+    tkind.push_back(kind);
+    if (e) (void)hipEventRecord(e, st);
+  };
+
+  // the launch sequence of the whole bake
+  auto record = [&](Rec& r) -> int {
+    auto fill = [&](uint32_t* p, size_t n) -> hipError_t {
+      if (r.dry) {
+        r.add(1, p);
+        r.add(1, n);
+        return hipSuccess;
+      }
+      return launch_fill_u32(p, 0u, n, st);
+    };
+    SHB_TRY(x, fill(x.d_counters.as<uint32_t>(), (size_t)nb * kCntStride));
+    SHB_TRY(x, fill(x.d_work.as<uint32_t>(), kWorkBytes / 4));
+    for (uint32_t bi = 0; bi < nb; ++bi) {
+      const BatchMap& m = maps[bi];
+      uint32_t* cnt = x.d_counters.as<uint32_t>() + (size_t)bi * kCntStride;
+      const uint32_t P = m.slot_base[m.nseg];
+      BakeArgs b{};
+      b.jobs = x.d_jobs.as<JobDev>();
+      b.map = x.d_maps.as<BatchMap>() + bi;
+      b.nseg = m.nseg;
+      b.job0 = m.job[0];
+      b.texel_begin0 = m.texel_begin[0];
+      b.T0 = m.tex_base[1] - m.tex_base[0];
+      b.T = m.tex_base[m.nseg];
+      b.P = P;
+      b.ray = base;
+      b.hit = base + 2 * (size_t)P;
+      b.thr = base + 3 * (size_t)P;
+      b.li = base + 4 * (size_t)P;
+      b.nee0 = base + 5 * (size_t)P;
+      b.nee0_dir = base + 6 * (size_t)P;
+      b.dir0 = base + 7 * (size_t)P;
+      b.sray = base + 8 * (size_t)P;
+      b.scontrib = base + 12 * (size_t)P;
+      uint32_t* qext[2] = {q, q + P};
+      b.shadow_queue = q + 2 * (size_t)P;
+      b.shadow_count = cnt + kCntSh;
+      TraceArgs tb = tbase;
+      tb.map = b.map;
+      tb.nseg = m.nseg;
+      tb.root0 = m.root[0];
+      tb.P = P;
+      if (r.dry) r.add(2, b);
+      else SHB_TRY(x, launch_generate(b, sgrid, st));
+      if (!r.dry) SHB_TRY(x, stage_done(st, "generate", bi, -1));
+      for (uint32_t d = 0; d <= maxB[bi]; ++d) {
+        TraceArgs ta = tb;
+        ta.rays = b.ray;
+        ta.queue = d == 0 ? nullptr : qext[d & 1];
+        ta.count_ptr = d == 0 ? nullptr : cnt + kCntExt + d;
+        ta.count_const = P;
+        ta.totals = count ? totals : nullptr;
+        ta.head = ta.refill ? cnt + kCntHeadExt + d : nullptr;
+        if (r.dry) {
+          r.add(3, ta);
+        } else {
+          trace_ev(0);
+          SHB_TRY(x, launch_trace_hits(ta, b.hit, count, P, x.tuning.block_size, st));
+          trace_ev(0);
+          ++run->launches_ext;
+          SHB_TRY(x, stage_done(st, "extend", bi, (int)d));
+        }
+        b.shadow_count = cnt + kCntSh + d;
+        if (r.dry) {
+          r.add(4, b);
+          r.add(4, d);
+        } else {
+          SHB_TRY(x, launch_shade(b, (int)d, d == 0 ? nullptr : qext[d & 1], cnt + kCntExt + d,
+                                  qext[(d + 1) & 1], cnt + kCntExt + d + 1, sgrid, st));
+          SHB_TRY(x, stage_done(st, "shade", bi, (int)d));
+        }
+        TraceArgs sa = tb;
+        sa.rays = b.sray;
+        sa.queue = b.shadow_queue;
+        sa.count_ptr = cnt + kCntSh + d;
+        sa.totals = count ? totals_sh : nullptr;
+        sa.head = sa.refill ? cnt + kCntHeadSh + d : nullptr;
+        if (r.dry) {
+          r.add(5, sa);
+        } else {
+          trace_ev(1);
+          SHB_TRY(x, launch_trace_bake_shadow(sa, b, count, 2 * P, x.tuning.block_size, st));
+          trace_ev(1);
+          ++run->launches_sh;
+          SHB_TRY(x, stage_done(st, "shadow", bi, (int)d));
+        }
+      }
+      if (r.dry) r.add(6, b);
+      else {
+        SHB_TRY(x, launch_project(b, st));
+        SHB_TRY(x, stage_done(st, "project", bi, -1));
+      }
+    }
+    return SHB_OK;
+  };
+
+  hipEvent_t e0, e1;
+  SHB_TRY(x, hipEventCreate(&e0));
+  SHB_TRY(x, hipEventCreate(&e1));
+  SHB_TRY(x, hipEventRecord(e0, st));
+  int rc = SHB_OK;
+  if (graph && !debug_stages()) {
+    std::string key;
+    Rec dry;
+    dry.dry = true;
+    dry.key = &key;
+    rc = record(dry);
+    if (rc) return rc;
+    if (!x.graph_exec || key != x.graph_key) {
+      if (x.graph_exec) {
+        (void)hipGraphExecDestroy(x.graph_exec);
+        x.graph_exec = nullptr;
+        x.graph_key.clear();
+      }
+      SHB_TRY(x, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      Rec live;
+      rc = record(live);
+      hipGraph_t g = nullptr;
+      hipError_t ce = hipStreamEndCapture(st, &g);
+      if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      SHB_TRY(x, ce);
+      hipError_t ie = hipGraphInstantiate(&x.graph_exec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      SHB_TRY(x, ie);
+      x.graph_key = key;
+      run->graph_captured = true;
+    }
+    SHB_TRY(x, hipGraphLaunch(x.graph_exec, st));
+    // launch counts as recorded
+    for (uint32_t bi = 0; bi < nb; ++bi) {
+      run->launches_ext += maxB[bi] + 1;
+      run->launches_sh += maxB[bi] + 1;
+    }
+  } else {
+    Rec live;
+    rc = record(live);
+    if (rc) return rc;
+  }
+  SHB_TRY(x, hipEventRecord(e1, st));
+  SHB_TRY(x, hipEventSynchronize(e1));
+  float ms = 0.f;
+  SHB_TRY(x, hipEventElapsedTime(&ms, e0, e1));
+  run->dev_s = ms * 1e-3;
+  double tr = 0, tr_sh = 0;
+  for (size_t i = 0; i + 1 < tev.size(); i += 2) {
+    float m = 0.f;
+    if (tev[i] && tev[i + 1] && hipEventElapsedTime(&m, tev[i], tev[i + 1]) == hipSuccess) {
+      tr += m;
+      if (tkind[i] == 1) tr_sh += m;
+    }
+  }
+  for (hipEvent_t e : tev)
+    if (e) (void)hipEventDestroy(e);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  run->trace_s = tr * 1e-3;
+  run->trace_sh_s = tr_sh * 1e-3;
+  // ray counts
+  std::vector<uint32_t> hc((size_t)nb * kCntStride);
+  if (nb)
+    SHB_TRY(x, hipMemcpy(hc.data(), x.d_counters.ptr, hc.size() * 4, hipMemcpyDeviceToHost));
+  for (uint32_t bi = 0; bi < nb; ++bi) {
+    const uint32_t* k = hc.data() + (size_t)bi * kCntStride;
+    run->rays_ext += maps[bi].slot_base[maps[bi].nseg];
+    for (uint32_t d = 1; d <= maxB[bi]; ++d) run->rays_ext += k[kCntExt + d];
+    for (uint32_t d = 0; d <= maxB[bi]; ++d) run->rays_sh += k[kCntSh + d];
+  }
+  std::vector<uint32_t> hw(kWorkBytes / 4);
+  SHB_TRY(x, hipMemcpy(hw.data(), x.d_work.ptr, kWorkBytes, hipMemcpyDeviceToHost));
+  if (hw[0]) return check_overflow(x, total_nodes + 1);
+  if (count) {
+    const unsigned long long* t2 = reinterpret_cast<const unsigned long long*>(hw.data() + 4);
+    for (int k = 0; k < 64; ++k) {
+      run->node_visits += t2[2 * k] + t2[128 + 2 * k];
+      run->tri_tests += t2[2 * k + 1] + t2[128 + 2 * k + 1];
+      run->node_visits_sh += t2[128 + 2 * k];
+      run->tri_tests_sh += t2[128 + 2 * k + 1];
+    }
+  }
+  return SHB_OK;
+}
+
+}  // namespace shb

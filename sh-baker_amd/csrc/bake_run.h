@@ -1,0 +1,35 @@
+// The wavefront bake driver shared by shb_bake, shb_bake_lightmap,
+// shb_bake_batched and the autotuner (not part of the ABI).
+#ifndef SHB_BAKE_RUN_H_
+#define SHB_BAKE_RUN_H_
+
+#include <vector>
+
+#include "api_util.h"
+
+namespace shb {
+
+// One lightmap of a bake: its context holds the scene, the surface and the
+// outputs (d_sh, d_vis); prepare_job fills its valid-texel list.
+struct JobIn {
+  Ctx* c = nullptr;
+  shb_bake_cfg cfg{};
+  uint32_t stream = 0;  // random-stream id (0 for the plain entry points)
+  uint32_t valid = 0;   // valid texels (set by prepare_job)
+};
+
+// Validates job j's context and config, zeroes its outputs and builds its
+// valid-texel list on its own stream (returns after that stream drained).
+// Errors are reported on the job's context.
+int prepare_job(JobIn& j);
+
+// Runs the bake of every job on x's stream: one set of wavefront batches
+// whose path slots may mix jobs (several jobs traverse their concatenated
+// BVHs). With `graph`, the launch sequence is captured once in a hipGraph
+// and replayed while the plan and buffers are unchanged. Errors are
+// reported on x.
+int run_bake(Ctx& x, std::vector<JobIn>& jobs, bool graph, BakeRun* run);
+
+}  // namespace shb
+
+#endif  // SHB_BAKE_RUN_H_

@@ -1,19 +1,142 @@
-// Batched bake, autotuner and path-replay adjoint entry points.
-#include "shb_ctx.h"
+// Batched bake and autotuner entry points (no reference counterpart: the
+// reference bakes one lightmap per process, src/main.cpp:40-193).
+#include <algorithm>
+#include <set>
+#include <string>
+#include <vector>
+
+#include "bake_run.h"
+
+using namespace shb;
 
 extern "C" {
 
 int shb_bake_batched(shb_ctx* const* ctxs, int32_t num_jobs, const shb_bake_cfg* cfgs,
                      float* const* sh_outs, float* const* vis_outs, shb_bake_stats* stats) {
   if (!ctxs || num_jobs <= 0 || !ctxs[0]) return SHB_ERR_INVALID_ARG;
-  ctxs[0]->c.err = "shb_bake_batched: not available in this build yet";
-  return SHB_ERR_UNSUPPORTED;
+  Ctx& x = ctxs[0]->c;
+  x.err.clear();
+  if (!cfgs) return fail(x, SHB_ERR_INVALID_ARG, "cfgs is NULL");
+  if (num_jobs > kMaxJobs)
+    return fail(x, SHB_ERR_LIMIT, "at most " + std::to_string(kMaxJobs) + " jobs per batched bake");
+  const double t0 = now_s();
+  std::set<const shb_ctx*> seen;
+  std::vector<JobIn> jobs((size_t)num_jobs);
+  for (int32_t i = 0; i < num_jobs; ++i) {
+    if (!ctxs[i]) return fail(x, SHB_ERR_INVALID_ARG, "job " + std::to_string(i) + ": ctx is NULL");
+    if (!seen.insert(ctxs[i]).second)
+      return fail(x, SHB_ERR_INVALID_ARG,
+                  "job " + std::to_string(i) + ": a context may appear once (it holds the outputs)");
+    if (ctxs[i]->c.device != x.device)
+      return fail(x, SHB_ERR_INVALID_ARG,
+                  "job " + std::to_string(i) + ": every context must be on the same device");
+    if ((cfgs[i].flags & SHB_BAKE_COUNT_VISITS) != (cfgs[0].flags & SHB_BAKE_COUNT_VISITS))
+      return fail(x, SHB_ERR_INVALID_ARG, "SHB_BAKE_COUNT_VISITS must be set on all jobs or none");
+    jobs[i].c = &ctxs[i]->c;
+    jobs[i].cfg = cfgs[i];
+    jobs[i].stream = 0;  // same stream as a plain bake of the job: identical results
+  }
+  for (int32_t i = 0; i < num_jobs; ++i) {
+    int rc = prepare_job(jobs[i]);
+    if (rc) return fail(x, rc, "job " + std::to_string(i) + ": " + jobs[i].c->err);
+  }
+  BakeRun run;
+  int rc = run_bake(x, jobs, true, &run);
+  if (rc) return rc;
+  for (int32_t i = 0; i < num_jobs; ++i) {
+    Ctx& c = *jobs[i].c;
+    const size_t npx = (size_t)c.width * c.ss * c.height * c.ss;
+    if (sh_outs && sh_outs[i])
+      SHB_TRY(x, hipMemcpyAsync(sh_outs[i], c.d_sh.ptr, npx * 27 * 4, hipMemcpyDeviceToHost,
+                                x.stream));
+    if (vis_outs && vis_outs[i])
+      SHB_TRY(x, hipMemcpyAsync(vis_outs[i], c.d_vis.ptr, npx * 4, hipMemcpyDeviceToHost,
+                                x.stream));
+  }
+  SHB_TRY(x, hipStreamSynchronize(x.stream));
+  if (stats) {
+    fill_stats(stats, run, 0, t0);
+    uint64_t paths = 0;
+    for (const JobIn& j : jobs) paths += (uint64_t)j.valid * (uint64_t)j.cfg.samples;
+    stats->paths = paths;
+  }
+  return SHB_OK;
 }
 
-int shb_autotune(shb_ctx* ctx, const shb_bake_cfg* cfg, shb_tuning* best) {
-  if (!ctx) return SHB_ERR_INVALID_ARG;
-  ctx->c.err = "shb_autotune: not available in this build yet";
-  return SHB_ERR_UNSUPPORTED;
+int shb_autotune(shb_ctx* h, const shb_bake_cfg* cfg, shb_tuning* best) {
+  if (!h) return SHB_ERR_INVALID_ARG;
+  Ctx& c = h->c;
+  c.err.clear();
+  if (!cfg) return fail(c, SHB_ERR_INVALID_ARG, "cfg is NULL");
+  // short bakes of the current scene and surface: same bounces, at most 16
+  // samples per texel (the per-path work does not depend on spp)
+  shb_bake_cfg probe = *cfg;
+  probe.samples = std::min(cfg->samples, 16);
+  probe.flags = 0;
+  std::vector<JobIn> jobs(1);
+  jobs[0].c = &c;
+  jobs[0].cfg = probe;
+  int rc = prepare_job(jobs[0]);
+  if (rc) return rc;
+  auto time_it = [&](double* sec) -> int {
+    double b = 1e30;
+    for (int k = 0; k < 2; ++k) {
+      BakeRun run;
+      int r = run_bake(c, jobs, false, &run);
+      if (r) return r;
+      b = std::min(b, run.dev_s);
+    }
+    *sec = b;
+    return SHB_OK;
+  };
+  const shb_tuning start = c.tuning;
+  shb_tuning cur = start;
+  double best_s = 1e30;
+  shb_tuning best_t = start;
+  const int widths[] = {8, 4};
+  const int blocks[] = {64, 256};
+  const int chunks[] = {0, 8, 16, 32};
+  for (int w : widths) {
+    cur.bvh_width = w;
+    rc = shb_set_tuning(h, &cur);
+    if (rc) return rc;
+    for (int b : blocks)
+      for (int q : chunks) {
+        cur.block_size = b;
+        cur.queue_chunk = q;
+        c.tuning = cur;
+        double s = 0;
+        rc = time_it(&s);
+        if (rc) return rc;
+        if (s < best_s) {
+          best_s = s;
+          best_t = cur;
+        }
+      }
+  }
+  // batch size for the chosen launch shape, judged at the caller's spp
+  jobs[0].cfg = *cfg;
+  jobs[0].cfg.flags = 0;
+  cur = best_t;
+  rc = shb_set_tuning(h, &cur);
+  if (rc) return rc;
+  best_s = 1e30;
+  const int batch[] = {1 << 22, 1 << 24, 1 << 25};
+  for (int bp : batch) {
+    cur.batch_paths = bp;
+    c.tuning = cur;
+    double s = 0;
+    rc = time_it(&s);
+    if (rc) return rc;
+    if (s < best_s) {
+      best_s = s;
+      best_t = cur;
+    }
+  }
+  rc = shb_set_tuning(h, &best_t);
+  if (rc) return rc;
+  if (best) *best = best_t;
+  return SHB_OK;
 }
 
 int shb_bake_adjoint(shb_ctx* ctx, const shb_bake_cfg* cfg, const float* dloss_dsh,
@@ -24,3 +147,4 @@ int shb_bake_adjoint(shb_ctx* ctx, const shb_bake_cfg* cfg, const float* dloss_d
 }
 
 }  // extern "C"
+

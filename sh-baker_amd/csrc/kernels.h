@@ -8,8 +8,54 @@
 
 namespace shb {
 
+constexpr int kMaxJobs = 64;  // lightmaps in one batched bake
+
+// One lightmap of a bake: an entry of the device job table.
+struct JobDev {
+  SceneView scene;             // the job's scene tables (shading, lights, materials)
+  const float4* surface;       // 3 float4 per supersampled texel
+  const uint32_t* texel_list;  // the job's valid texels, ascending
+  float* sh_out;               // 27 per supersampled texel
+  float* vis_out;              // 1 per supersampled texel
+  uint64_t seed;
+  uint32_t spp;
+  uint32_t bounces;
+  uint32_t stream;             // random-stream id (the counter's job word)
+  uint32_t pad;
+};
+
+// Path slots of one wavefront batch: segment s holds texels
+// [texel_begin[s], texel_begin[s] + T_s) of job[s] (ranks in its texel
+// list), T_s = tex_base[s+1] - tex_base[s], as slots
+// [slot_base[s], slot_base[s+1]) laid out sample-major (slot = base +
+// sample * T_s + texel), so one texel's samples are T_s slots apart.
+struct BatchMap {
+  uint32_t nseg;
+  uint32_t pad;
+  uint32_t slot_base[kMaxJobs + 1];
+  uint32_t tex_base[kMaxJobs + 1];
+  uint32_t job[kMaxJobs];
+  uint32_t texel_begin[kMaxJobs];
+  uint32_t root[kMaxJobs];     // the job's BVH root in the traversal arrays
+};
+
+// Largest s with base[s] <= v (base[0] = 0, n entries, ascending).
+__device__ __forceinline__ uint32_t find_segment(const uint32_t* base, uint32_t n, uint32_t v) {
+  uint32_t lo = 0, hi = n;
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (base[mid] <= v)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+constexpr uint32_t kNoRoot = 0xffffffffu;  // a job without triangles
+
 struct TraceArgs {
-  SceneView scene;
+  SceneView scene;            // nodes / tris used (the batch's traversal arrays)
   const float4* rays;         // 2 float4 per ray slot: (o, tnear), (d, tfar)
   const uint32_t* queue;      // ray slots to trace, or null for 0..count-1
   const uint32_t* count_ptr;  // device count of queue entries, or null
@@ -23,19 +69,23 @@ struct TraceArgs {
   uint32_t* error;            // [0] bits: 1 stack overflow, 2 step cap; [1..3] first ray
   uint32_t* ray_counts;       // optional: node visits, triangle tests per slot
   unsigned long long* totals; // optional: 64 pairs of summed (nodes, tris)
+  // root node per ray: root0 when nseg <= 1, else map->root[] of the
+  // segment of (slot mod P)
+  const BatchMap* map;
+  uint32_t nseg;
+  uint32_t root0;
+  uint32_t P;
 };
 
 struct BakeArgs {
-  SceneView scene;
-  const float4* surface;      // 3 float4 per supersampled texel
-  const uint32_t* texel_list; // valid texels, ascending
-  uint32_t texel_begin;       // first rank of this batch in texel_list
-  uint32_t texels_in_batch;   // T
-  uint32_t spp;
-  uint32_t bounces;
-  uint32_t P;                 // path slots (>= T * spp)
-  uint32_t job;
-  uint64_t seed;
+  const JobDev* jobs;         // device job table
+  const BatchMap* map;        // this batch's slot map (device)
+  uint32_t nseg;              // segments (1: the fields below describe it)
+  uint32_t job0;              // nseg == 1: its job, first texel rank, texels
+  uint32_t texel_begin0;
+  uint32_t T0;
+  uint32_t T;                 // texels in the batch
+  uint32_t P;                 // path slots in the batch
   float4* ray;                // 2 per slot: extension ray
   float4* hit;                // 1 per slot: (t, id, u, v)
   float4* thr;                // throughput, branch id in w
@@ -47,8 +97,6 @@ struct BakeArgs {
   float4* scontrib;           // vertex NEE contribution per slot
   uint32_t* shadow_queue;
   uint32_t* shadow_count;
-  float* sh_out;              // 27 per supersampled texel
-  float* vis_out;             // 1 per supersampled texel
 };
 
 #ifndef SHB_HOST_ONLY
@@ -68,6 +116,10 @@ hipError_t launch_shade(const BakeArgs& b, int depth, const uint32_t* in_q,
                         hipStream_t st);
 hipError_t launch_project(const BakeArgs& b, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t st);
+// Copies n BVH nodes, shifting child and triangle bases by node_off / tri_off
+// (concatenated trees of a batched bake).
+hipError_t launch_relocate_nodes(const CWNode* src, CWNode* dst, uint32_t n, uint32_t node_off,
+                                 uint32_t tri_off, hipStream_t st);
 hipError_t launch_fill_f32(float* p, float v, size_t n, hipStream_t st);
 
 #endif  // SHB_HOST_ONLY

@@ -9,6 +9,8 @@
 #include <string>
 #include <vector>
 
+#include "api_util.h"
+#include "bake_run.h"
 #include "scene_host.h"
 #include "shb_ctx.h"
 
@@ -16,40 +18,11 @@ using namespace shb;
 
 namespace {
 
-#define SHB_TRY(ctx, expr)                                                   \
-  do {                                                                       \
-    hipError_t _e = (expr);                                                  \
-    if (_e != hipSuccess) {                                                  \
-      (ctx).err = std::string(#expr) + ": " + hipGetErrorString(_e);         \
-      return _e == hipErrorOutOfMemory ? SHB_ERR_OOM : SHB_ERR_DEVICE;       \
-    }                                                                        \
-  } while (0)
-
-int fail(Ctx& c, int code, const std::string& msg) {
-  c.err = msg;
-  return code;
-}
-
 int build_scene(Ctx& c, const shb_scene_desc* d) {
   std::string err;
   int rc = build_host_scene(d, &c.hs, &err);
   if (rc) c.err = err;
   return rc;
-}
-
-template <class T>
-hipError_t upload(DevBuf& b, const std::vector<T>& v, hipStream_t st) {
-  size_t n = std::max<size_t>(v.size() * sizeof(T), 16);
-  hipError_t e = b.reserve(n);
-  if (e != hipSuccess) return e;
-  if (!v.empty())
-    return hipMemcpyAsync(b.ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st);
-  return hipSuccess;
-}
-
-double now_s() {
-  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
-      .count();
 }
 
 int upload_bvh(Ctx& c) {
@@ -64,273 +37,19 @@ int upload_bvh(Ctx& c) {
   return SHB_OK;
 }
 
-TraceArgs trace_args(Ctx& c) {
-  TraceArgs a{};
-  a.scene = c.view();
-  a.has_nodes = c.bvh.nodes.empty() ? 0 : 1;
-  a.max_steps = (uint32_t)c.bvh.nodes.size() + 1;
-  a.stack_entries = (uint32_t)std::max(1, std::min(c.bvh.max_depth - 1, kStackLds));
-  a.refill = (uint32_t)c.tuning.queue_chunk;
-  a.num_cus = c.num_cus;
-  a.head = nullptr;  // set per launch when the persistent form is used
-  a.error = c.d_work.as<uint32_t>();
-  return a;
-}
+}  // namespace
 
-int check_overflow(Ctx& c) {
-  uint32_t e[4] = {0, 0, 0, 0};
-  SHB_TRY(c, hipMemcpyAsync(e, c.d_work.ptr, sizeof(e), hipMemcpyDeviceToHost, c.stream));
-  SHB_TRY(c, hipStreamSynchronize(c.stream));
-  if (e[0] == 0) return SHB_OK;
-  std::string what = (e[0] & 1u) ? "traversal stack overflow" : "";
-  if (e[0] & 2u) what += std::string(what.empty() ? "" : " and ") + "traversal step cap reached";
-  return fail(c, SHB_ERR_LIMIT,
-              what + " (first ray slot " + std::to_string((long long)e[1] - 1) + ", stack depth " +
-                  std::to_string(e[2]) + ", " + std::to_string(e[3]) + " steps, cap " +
-                  std::to_string(c.bvh.nodes.size() + 1) + ")");
-}
-
-// Counter layout per batch (uint32): extension-queue counts by depth, shadow
-// counts by depth, queue heads of the persistent extension and shadow
-// traversals by depth.
-constexpr int kCntExt = 0, kCntSh = 20, kCntHeadExt = 40, kCntHeadSh = 56, kCntStride = 72;
-
-// SHB_DEBUG_STAGES=1: synchronise after every bake stage and log its time to
-// stderr (diagnostics; off by default).
-bool debug_stages() {
-  static int v = -1;
-  if (v < 0) {
-    const char* e = getenv("SHB_DEBUG_STAGES");
-    v = (e && e[0] == '1') ? 1 : 0;
-  }
-  return v == 1;
-}
-
-hipError_t stage_done(hipStream_t st, const char* name, uint32_t batch, int depth) {
-  if (!debug_stages()) return hipSuccess;
-  static double last = 0;
-  hipError_t e = hipStreamSynchronize(st);
-  double t = std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
-  fprintf(stderr, "[shb] batch %u depth %d %-8s %.3f ms (%s)\n", batch, depth, name,
-          last > 0 ? (t - last) * 1e3 : 0.0, hipGetErrorString(e));
-  fflush(stderr);
-  last = t;
-  return e;
-}
-
-// d_work layout: [0] error bits, [1..3] first failing ray (slot + 1, stack
-// depth, steps); then 64 pairs of u64 visit totals of the extension
-// traversals, then 64 pairs of the shadow traversals.
-constexpr size_t kWorkBytes = 16 + 2 * 64 * 16;
-
-struct BakeRun {
-  uint64_t rays_ext = 0, rays_sh = 0, valid = 0, node_visits = 0, tri_tests = 0;
-  uint64_t node_visits_sh = 0, tri_tests_sh = 0;
-  double dev_s = 0, trace_s = 0, trace_sh_s = 0;
-  uint32_t launches_ext = 0, launches_sh = 0;
-};
-
-// The bake into c.d_sh / c.d_vis (supersampled), with per-batch kernels:
-// generate, then per depth: extend, shade, shadow; then project.
-int bake_device(Ctx& c, const shb_bake_cfg* cfg, uint32_t job, BakeRun* run) {
-  c.lm_sh = nullptr;  // the device buffers below are about to be reused
-  if (!c.has_scene) return fail(c, SHB_ERR_NO_SCENE, "no scene uploaded");
-  if (!c.has_surface) return fail(c, SHB_ERR_NO_SURFACE, "no surface: call shb_rasterize or shb_set_surface");
+namespace {
+// The plain bake of one context: its own job, random stream 0.
+int bake_one(Ctx& c, const shb_bake_cfg* cfg, BakeRun* run) {
   if (!cfg) return fail(c, SHB_ERR_INVALID_ARG, "cfg is NULL");
-  if (cfg->samples < 1 || cfg->samples > kMaxSamples)
-    return fail(c, SHB_ERR_INVALID_ARG, "samples must be in [1, 65535]");
-  if (cfg->bounces < 0 || cfg->bounces > kMaxBounces)
-    return fail(c, SHB_ERR_INVALID_ARG, "bounces must be in [0, 15]");
-  if (cfg->num_light_samples != 1)
-    return fail(c, SHB_ERR_UNSUPPORTED, "num_light_samples must be 1");
-  if (c.hs.bad_material)
-    return fail(c, SHB_ERR_INVALID_ARG,
-                "a geometry's material_id does not name a material (the bake shades every hit)");
-  if (c.hs.has_transparency)
-    return fail(c, SHB_ERR_UNSUPPORTED,
-                "materials with alpha < 1 (path splitting) are not supported by the GPU bake");
-  const uint32_t W = (uint32_t)(c.width * c.ss), H = (uint32_t)(c.height * c.ss);
-  const size_t npx = (size_t)W * H;
-  const uint32_t spp = (uint32_t)cfg->samples, B = (uint32_t)cfg->bounces;
-  hipStream_t st = c.stream;
-  hipEvent_t e0, e1;
-  SHB_TRY(c, hipEventCreate(&e0));
-  SHB_TRY(c, hipEventCreate(&e1));
-  SHB_TRY(c, hipEventRecord(e0, st));
-  // outputs
-  SHB_TRY(c, c.d_sh.reserve(std::max<size_t>(npx * 27 * sizeof(float), 16)));
-  SHB_TRY(c, c.d_vis.reserve(std::max<size_t>(npx * sizeof(float), 16)));
-  SHB_TRY(c, launch_fill_f32(c.d_sh.as<float>(), 0.f, npx * 27, st));
-  SHB_TRY(c, launch_fill_f32(c.d_vis.as<float>(), 0.f, npx, st));
-  // valid texel list
-  SHB_TRY(c, c.d_mask.reserve(std::max<size_t>(npx, 16)));
-  SHB_TRY(c, c.d_list.reserve(std::max<size_t>(npx * 4 + 16, 64)));
-  SHB_TRY(c, c.d_tmp.reserve(((size_t)compact_blocks((uint32_t)npx) + 1) * 4 + 64));
-  SHB_TRY(c, launch_validity_mask(c.d_surface.as<float4>(), npx, c.d_mask.as<uint8_t>(), st));
-  uint32_t* d_nvalid = c.d_list.as<uint32_t>() + npx;
-  SHB_TRY(c, launch_compact_valid(c.d_mask.as<uint8_t>(), (uint32_t)npx, c.d_list.as<uint32_t>(),
-                                  c.d_tmp.as<uint32_t>(), st));
-  uint32_t V = 0;
-  SHB_TRY(c, stage_done(st, "compact", 0, -1));
-  SHB_TRY(c, hipMemcpyAsync(&V, d_nvalid, 4, hipMemcpyDeviceToHost, st));
-  SHB_TRY(c, hipStreamSynchronize(st));
-  run->valid = V;
-  const bool count = (cfg->flags & SHB_BAKE_COUNT_VISITS) != 0;
-  // batches
-  uint32_t T = (uint32_t)std::max<int64_t>(1, (int64_t)c.tuning.batch_paths / spp);
-  T = std::min<uint32_t>(T, std::max<uint32_t>(V, 1));
-  const uint32_t P = T * spp;
-  const uint32_t nb = V == 0 ? 0 : (V + T - 1) / T;
-  // path state: 13 float4 per slot; queues 4 uint32 per slot
-  SHB_TRY(c, c.d_paths.reserve((size_t)P * 13 * sizeof(float4)));
-  SHB_TRY(c, c.d_queues.reserve((size_t)P * 4 * sizeof(uint32_t)));
-  SHB_TRY(c, c.d_counters.reserve(std::max<size_t>((size_t)nb * kCntStride * 4, 64)));
-  SHB_TRY(c, c.d_work.reserve(kWorkBytes));
-  SHB_TRY(c, launch_fill_u32(c.d_counters.as<uint32_t>(), 0u, (size_t)nb * kCntStride, st));
-  SHB_TRY(c, launch_fill_u32(c.d_work.as<uint32_t>(), 0u, kWorkBytes / 4, st));
-  float4* base = c.d_paths.as<float4>();
-  BakeArgs b{};
-  b.scene = c.view();
-  b.surface = c.d_surface.as<float4>();
-  b.texel_list = c.d_list.as<uint32_t>();
-  b.spp = spp;
-  b.bounces = B;
-  b.P = P;
-  b.job = job;
-  b.seed = cfg->seed;
-  b.ray = base;
-  b.hit = base + 2 * (size_t)P;
-  b.thr = base + 3 * (size_t)P;
-  b.li = base + 4 * (size_t)P;
-  b.nee0 = base + 5 * (size_t)P;
-  b.nee0_dir = base + 6 * (size_t)P;
-  b.dir0 = base + 7 * (size_t)P;
-  b.sray = base + 8 * (size_t)P;
-  b.scontrib = base + 12 * (size_t)P;
-  uint32_t* q = c.d_queues.as<uint32_t>();
-  uint32_t* qext[2] = {q, q + P};
-  b.shadow_queue = q + 2 * (size_t)P;
-  b.sh_out = c.d_sh.as<float>();
-  b.vis_out = c.d_vis.as<float>();
-  unsigned long long* totals = reinterpret_cast<unsigned long long*>(c.d_work.as<uint32_t>() + 4);
-  unsigned long long* totals_sh = totals + 128;
-  // event pairs around each traversal launch; kind 0 extension, 1 shadow
-  std::vector<hipEvent_t> tev;
-  std::vector<int> tkind;
-  auto trace_ev = [&](int kind) -> hipEvent_t {
-    hipEvent_t e;
-    if (hipEventCreate(&e) != hipSuccess) e = nullptr;
-    tev.push_back(e);
-    tkind.push_back(kind);
-    if (e) (void)hipEventRecord(e, st);
-    return e;
-  };
-  const int sgrid = std::max(1, std::min<int>((P + 255) / 256, c.num_cus * 16));
-  (void)sgrid;
-  for (uint32_t bi = 0; bi < nb; ++bi) {
-    uint32_t* cnt = c.d_counters.as<uint32_t>() + (size_t)bi * kCntStride;
-    b.texel_begin = bi * T;
-    b.texels_in_batch = std::min(T, V - bi * T);
-    b.shadow_count = cnt + kCntSh;
-    SHB_TRY(c, launch_generate(b, sgrid, st));
-    SHB_TRY(c, stage_done(st, "generate", bi, -1));
-    for (uint32_t d = 0; d <= B; ++d) {
-      TraceArgs ta = trace_args(c);
-      ta.rays = b.ray;
-      ta.queue = d == 0 ? nullptr : qext[d & 1];
-      ta.count_ptr = d == 0 ? nullptr : cnt + kCntExt + d;
-      ta.count_const = b.texels_in_batch * spp;
-      ta.totals = count ? totals : nullptr;
-      ta.head = ta.refill ? cnt + kCntHeadExt + d : nullptr;
-      trace_ev(0);
-      SHB_TRY(c, launch_trace_hits(ta, b.hit, count, b.texels_in_batch * spp, c.tuning.block_size, st));
-      trace_ev(0);
-      ++run->launches_ext;
-      SHB_TRY(c, stage_done(st, "extend", bi, (int)d));
-      b.shadow_count = cnt + kCntSh + d;
-      SHB_TRY(c, launch_shade(b, (int)d, d == 0 ? nullptr : qext[d & 1], cnt + kCntExt + d,
-                              qext[(d + 1) & 1], cnt + kCntExt + d + 1, sgrid, st));
-      SHB_TRY(c, stage_done(st, "shade", bi, (int)d));
-      TraceArgs sa = trace_args(c);
-      sa.rays = b.sray;
-      sa.queue = b.shadow_queue;
-      sa.count_ptr = cnt + kCntSh + d;
-      sa.totals = count ? totals_sh : nullptr;
-      sa.head = sa.refill ? cnt + kCntHeadSh + d : nullptr;
-      trace_ev(1);
-      SHB_TRY(c, launch_trace_bake_shadow(sa, b, count, 2 * b.texels_in_batch * spp,
-                                          c.tuning.block_size, st));
-      trace_ev(1);
-      ++run->launches_sh;
-      SHB_TRY(c, stage_done(st, "shadow", bi, (int)d));
-    }
-    SHB_TRY(c, launch_project(b, st));
-    SHB_TRY(c, stage_done(st, "project", bi, -1));
-  }
-  SHB_TRY(c, hipEventRecord(e1, st));
-  SHB_TRY(c, hipEventSynchronize(e1));
-  float ms = 0.f;
-  SHB_TRY(c, hipEventElapsedTime(&ms, e0, e1));
-  run->dev_s = ms * 1e-3;
-  double tr = 0, tr_sh = 0;
-  for (size_t i = 0; i + 1 < tev.size(); i += 2) {
-    float m = 0.f;
-    if (tev[i] && tev[i + 1] && hipEventElapsedTime(&m, tev[i], tev[i + 1]) == hipSuccess) {
-      tr += m;
-      if (tkind[i] == 1) tr_sh += m;
-    }
-  }
-  for (hipEvent_t e : tev)
-    if (e) (void)hipEventDestroy(e);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  run->trace_s = tr * 1e-3;
-  run->trace_sh_s = tr_sh * 1e-3;
-  // ray counts
-  std::vector<uint32_t> hc((size_t)nb * kCntStride);
-  if (nb) {
-    SHB_TRY(c, hipMemcpy(hc.data(), c.d_counters.ptr, hc.size() * 4, hipMemcpyDeviceToHost));
-  }
-  for (uint32_t bi = 0; bi < nb; ++bi) {
-    const uint32_t* k = hc.data() + (size_t)bi * kCntStride;
-    uint32_t tb = std::min(T, V - bi * T);
-    run->rays_ext += (uint64_t)tb * spp;
-    for (uint32_t d = 1; d <= B; ++d) run->rays_ext += k[kCntExt + d];
-    for (uint32_t d = 0; d <= B; ++d) run->rays_sh += k[kCntSh + d];
-  }
-  std::vector<uint32_t> hw(kWorkBytes / 4);
-  SHB_TRY(c, hipMemcpy(hw.data(), c.d_work.ptr, kWorkBytes, hipMemcpyDeviceToHost));
-  if (hw[0]) return check_overflow(c);
-  if (count) {
-    const unsigned long long* t2 = reinterpret_cast<const unsigned long long*>(hw.data() + 4);
-    for (int k = 0; k < 64; ++k) {
-      run->node_visits += t2[2 * k] + t2[128 + 2 * k];
-      run->tri_tests += t2[2 * k + 1] + t2[128 + 2 * k + 1];
-      run->node_visits_sh += t2[128 + 2 * k];
-      run->tri_tests_sh += t2[128 + 2 * k + 1];
-    }
-  }
-  return SHB_OK;
+  std::vector<JobIn> jobs(1);
+  jobs[0].c = &c;
+  jobs[0].cfg = *cfg;
+  int rc = prepare_job(jobs[0]);
+  if (rc) return rc;
+  return run_bake(c, jobs, (cfg->flags & SHB_BAKE_USE_GRAPH) != 0, run);
 }
-
-void fill_stats(shb_bake_stats* s, const BakeRun& r, uint32_t spp, double t0) {
-  if (!s) return;
-  s->valid_texels = r.valid;
-  s->paths = r.valid * spp;
-  s->rays_extension = r.rays_ext;
-  s->rays_shadow = r.rays_sh;
-  s->node_visits = r.node_visits;
-  s->tri_tests = r.tri_tests;
-  s->seconds_total = now_s() - t0;
-  s->seconds_device = r.dev_s;
-  s->seconds_trace = r.trace_s;
-  s->node_visits_shadow = r.node_visits_sh;
-  s->tri_tests_shadow = r.tri_tests_sh;
-  s->seconds_trace_shadow = r.trace_sh_s;
-  s->launches_extension = r.launches_ext;
-  s->launches_shadow = r.launches_sh;
-}
-
 }  // namespace
 
 // =====================================================================
@@ -377,7 +96,10 @@ void shb_destroy(shb_ctx* h) {
                     &c.d_counters, &c.d_work,    &c.d_list,     &c.d_mask,
                     &c.d_tmp,    &c.d_sh,       &c.d_vis,     &c.d_img_a,    &c.d_img_b,
                     &c.d_mask_a, &c.d_mask_b,   &c.d_rays,    &c.d_hits,     &c.d_hit_out,
-                    &c.d_raycounts, &c.d_adj};
+                    &c.d_raycounts, &c.d_adj,   &c.d_jobs,    &c.d_maps,     &c.d_cnodes,
+                    &c.d_ctris};
+  if (c.graph_exec) (void)hipGraphExecDestroy(c.graph_exec);
+  c.graph_exec = nullptr;
   for (DevBuf* b : bufs) b->release();
   if (c.stream) (void)hipStreamDestroy(c.stream);
   delete h;
@@ -542,7 +264,7 @@ int shb_bake(shb_ctx* h, const shb_bake_cfg* cfg, float* sh_out, float* vis_out,
   double t0 = now_s();
   SHB_TRY(c, hipSetDevice(c.device));
   BakeRun run;
-  int rc = bake_device(c, cfg, 0, &run);
+  int rc = bake_one(c, cfg, &run);
   if (rc) return rc;
   size_t npx = (size_t)c.width * c.ss * c.height * c.ss;
   if (sh_out)
@@ -563,7 +285,7 @@ int shb_bake_lightmap(shb_ctx* h, const shb_bake_cfg* cfg, int32_t dilation, flo
   double t0 = now_s();
   SHB_TRY(c, hipSetDevice(c.device));
   BakeRun run;
-  int rc = bake_device(c, cfg, 0, &run);
+  int rc = bake_one(c, cfg, &run);
   if (rc) return rc;
   hipStream_t st = c.stream;
   const int w = c.width, hgt = c.height, ss = c.ss;
@@ -753,7 +475,7 @@ int shb_trace_rays(shb_ctx* h, const shb_ray* rays, uint32_t n, int32_t any_hit,
       if (tri_tests) tri_tests[i] = rc2[2 * i + 1];
     }
   }
-  return check_overflow(c);
+  return check_overflow(c, c.bvh.nodes.size() + 1);
 }
 
 int shb_set_tuning(shb_ctx* h, const shb_tuning* t) {

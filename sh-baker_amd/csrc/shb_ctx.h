@@ -70,6 +70,11 @@ struct Ctx {
   DevBuf d_img_a, d_img_b, d_mask_a, d_mask_b;
   DevBuf d_rays, d_hits, d_hit_out, d_raycounts;
   DevBuf d_adj;               // adjoint accumulators
+  // batched / graph bakes run on this context: job table, batch maps,
+  // concatenated BVHs of several jobs, the captured launch sequence
+  DevBuf d_jobs, d_maps, d_cnodes, d_ctris;
+  hipGraphExec_t graph_exec = nullptr;
+  std::string graph_key;
 
   SceneView view() const {
     SceneView v;

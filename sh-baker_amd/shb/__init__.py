@@ -27,6 +27,7 @@ ERRORS = {
 LIGHT_POINT, LIGHT_DIRECTIONAL, LIGHT_SPOT, LIGHT_AREA = 0, 1, 2, 3
 GEOM_HAS_NORMALS, GEOM_HAS_TEXTURE_UVS, GEOM_HAS_LIGHTMAP_UVS, GEOM_HAS_TANGENTS = 1, 2, 4, 8
 BAKE_COUNT_VISITS = 1
+BAKE_USE_GRAPH = 2
 
 
 class ShbGeometry(C.Structure):
@@ -272,6 +273,14 @@ class Baker:
         self._chk(lib.shb_read_lightmap(self.h, _f(sh), _f(vis), _u8(mask)))
         return sh, vis, mask
 
+    def autotune(self, samples, bounces, seed=12345):
+        """Times short bakes of the current scene/surface over launch shapes,
+        BVH widths and batch sizes; applies and returns the fastest tuning."""
+        t = ShbTuning()
+        cfg = self.cfg(samples, bounces, seed)
+        self._chk(lib.shb_autotune(self.h, C.byref(cfg), C.byref(t)))
+        return {k: getattr(t, k) for k, _ in t._fields_}
+
     # -- image ops
     def downsample(self, img, width, height, scale):
         img = np.ascontiguousarray(img, np.float32)
@@ -361,3 +370,29 @@ def bake_batched(bakers, cfgs):
     if rc != SHB_OK:
         raise ShbError(rc, lib.shb_last_error(bakers[0].h).decode())
     return outs_sh, outs_vis, st.as_dict()
+
+
+def bake_batched(bakers, samples, bounces, seeds=None, flags=0, want=True):
+    """Bakes several Bakers (each with its own scene and surface, all on one
+    device) in one batched run (shared wavefront batches, hipGraph replay).
+    samples / bounces / seeds: one value per job or a scalar. Returns
+    ([sh], [vis], stats) like Baker.bake per job."""
+    n = len(bakers)
+    as_list = lambda v: list(v) if isinstance(v, (list, tuple, np.ndarray)) else [v] * n
+    samples, bounces = as_list(samples), as_list(bounces)
+    seeds = as_list(12345 if seeds is None else seeds)
+    cfgs = (ShbBakeCfg * n)(*[Baker.cfg(int(samples[i]), int(bounces[i]), int(seeds[i]), flags)
+                              for i in range(n)])
+    ctxs = (C.c_void_p * n)(*[b.h for b in bakers])
+    shs, viss = [], []
+    for b in bakers:
+        m = b.width * b.ss * b.height * b.ss
+        shs.append(np.zeros((m, 27), np.float32) if want else None)
+        viss.append(np.zeros(m, np.float32) if want else None)
+    sh_p = (_fp * n)(*[_f(a) for a in shs]) if want else None
+    vis_p = (_fp * n)(*[_f(a) for a in viss]) if want else None
+    st = ShbBakeStats()
+    rc = lib.shb_bake_batched(ctxs, n, cfgs, sh_p, vis_p, C.byref(st))
+    if rc != SHB_OK:
+        raise ShbError(rc, lib.shb_last_error(bakers[0].h).decode())
+    return shs, viss, st.as_dict()

@@ -287,7 +287,8 @@ class SceneBuilder:
     def pack(self, width, height, padding=2):
         """Shelf-packs every chart (tallest first) at one texel density, the
         largest found by bisection that fits, with `padding` texels around
-        each chart. Raises ValueError when the charts cannot fit at all."""
+        each chart. When the charts cannot fit as whole texels, places them in
+        a grid of sub-texel cells instead."""
         ext = np.array(self.charts, np.float64).reshape(-1, 2)
         order = np.argsort(-ext[:, 1], kind="stable")
 
@@ -310,9 +311,24 @@ class SceneBuilder:
             return pos
 
         lo = 0.0
+        self.pad_used = padding
         if place(lo) is None:
-            raise ValueError(f"{len(ext)} charts do not fit a {width}x{height} atlas "
-                             f"with {padding} texels of padding")
+            # too many charts for whole texels: a grid of equal cells (possibly
+            # smaller than a texel), every chart at one density so the largest
+            # fits its cell; charts below a texel rasterize through the
+            # centroid fallback, as in the reference rasterizer
+            n = len(ext)
+            gx = int(math.ceil(math.sqrt(n * width / height)))
+            gy = int(math.ceil(n / gx))
+            cw, chh = width / gx, height / gy
+            d = 0.9 * min(cw / max(ext[:, 0].max(), 1e-12), chh / max(ext[:, 1].max(), 1e-12))
+            pos = np.zeros((n, 2), np.float64)
+            for k, ci in enumerate(order):
+                pos[ci] = ((k % gx) * cw + 0.05 * cw, (k // gx) * chh + 0.05 * chh)
+            self.pad_used = 0
+            self.density = d
+            self.chart_pos = pos
+            return d
         area = float((ext[:, 0] * ext[:, 1]).sum())
         hi = 2.0 * math.sqrt(width * height / max(area, 1e-12))
         while place(hi) is not None:
@@ -336,7 +352,7 @@ class SceneBuilder:
         s.texture_uvs = np.concatenate(self.tuv).astype(np.float32)
         loc = np.concatenate(self.local)
         ch = np.concatenate(self.chart_of)
-        px = self.chart_pos[ch] + padding + loc * self.density
+        px = self.chart_pos[ch] + self.pad_used + loc * self.density
         s.lightmap_uvs = np.stack([px[:, 0] / width, px[:, 1] / height], 1).astype(np.float32)
         s.indices = np.concatenate(self.idx).astype(np.uint32)
         s.geometries = self.geoms

@@ -1,0 +1,109 @@
+"""Batched bake (shb_bake_batched), hipGraph replay and the autotuner.
+
+The batched form bakes several lightmaps through shared wavefront batches
+over their concatenated BVHs; the random stream is keyed by (seed, texel,
+sample, depth), so every job must come out bit-identical to a plain bake of
+the same context, whatever the batch packing, launch shapes or replay."""
+import numpy as np
+import pytest
+
+import shb
+from shb import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def bits(a):
+    return np.ascontiguousarray(a).view(np.uint32)
+
+
+@pytest.fixture(scope="module")
+def jobs():
+    specs = [  # (scene, atlas, ss, spp, bounces, seed)
+        (scenes.box_room_scene(48, 48), 48, 1, 16, 3, 7),
+        (scenes.scaled_synthetic_scene(3000, 64, 2), 64, 1, 8, 1, 8),
+        (scenes.box_room_scene(32, 32), 32, 2, 4, 5, 9),
+        (scenes.scaled_synthetic_scene(12000, 96, 4), 96, 1, 32, 2, 10),
+    ]
+    out = []
+    for s, res, ss, spp, B, seed in specs:
+        b = shb.Baker(0)
+        b.upload_scene(s)
+        b.rasterize(res, res, ss, want=False)
+        out.append((b, spp, B, seed))
+    yield out
+    for b, *_ in out:
+        b.close()
+
+
+def plain(jobs):
+    return [b.bake(spp, B, seed=seed)[:2] for b, spp, B, seed in jobs]
+
+
+def test_batched_bit_identical_to_plain(jobs):
+    ref = plain(jobs)
+    bakers = [j[0] for j in jobs]
+    sh, vis, st = shb.bake_batched(bakers, [j[1] for j in jobs], [j[2] for j in jobs],
+                                   [j[3] for j in jobs])
+    for k in range(len(jobs)):
+        assert np.array_equal(bits(sh[k]), bits(ref[k][0])), k
+        assert np.array_equal(bits(vis[k]), bits(ref[k][1])), k
+    assert st["valid_texels"] == sum(int((b.validity_mask() > 0).sum()) for b in bakers)
+
+
+def test_batched_small_batches_mix_jobs_and_replay(jobs):
+    # a tiny batch size splits jobs over many batches whose segments mix jobs;
+    # the second call replays the captured graph
+    ref = plain(jobs)
+    bakers = [j[0] for j in jobs]
+    old = bakers[0].tuning()
+    bakers[0].set_tuning(batch_paths=3000)
+    try:
+        for _ in range(2):
+            sh, vis, _ = shb.bake_batched(bakers, [j[1] for j in jobs], [j[2] for j in jobs],
+                                          [j[3] for j in jobs])
+            for k in range(len(jobs)):
+                assert np.array_equal(bits(sh[k]), bits(ref[k][0])), k
+                assert np.array_equal(bits(vis[k]), bits(ref[k][1])), k
+        # a new seed changes the recorded launches: the graph is captured again
+        sh2, _, _ = shb.bake_batched(bakers, [j[1] for j in jobs], [j[2] for j in jobs],
+                                     [j[3] + 1 for j in jobs])
+        assert not np.array_equal(bits(sh2[0]), bits(ref[0][0]))
+        r2 = bakers[0].bake(jobs[0][1], jobs[0][2], seed=jobs[0][3] + 1)[0]
+        assert np.array_equal(bits(sh2[0]), bits(r2))
+    finally:
+        bakers[0].set_tuning(**old)
+
+
+def test_plain_graph_flag_replays_identically(jobs):
+    b, spp, B, seed = jobs[0]
+    ref = b.bake(spp, B, seed=seed)[0]
+    for _ in range(3):
+        g = b.bake(spp, B, seed=seed, flags=shb.BAKE_USE_GRAPH)[0]
+        assert np.array_equal(bits(g), bits(ref))
+
+
+def test_batched_errors(jobs):
+    b = jobs[0][0]
+    with pytest.raises(shb.ShbError) as e:
+        shb.bake_batched([b, b], 4, 1)
+    assert e.value.code == -1
+    fresh = shb.Baker(0)
+    with pytest.raises(shb.ShbError) as e:
+        shb.bake_batched([b, fresh], 4, 1)
+    assert "job 1" in str(e.value)
+    fresh.close()
+
+
+def test_autotune_keeps_results(jobs):
+    b, spp, B, seed = jobs[3]
+    ref = b.bake(spp, B, seed=seed)[0]
+    old = b.tuning()
+    try:
+        t = b.autotune(spp, B, seed=seed)
+        assert t["block_size"] in (64, 128, 256) and 2 <= t["bvh_width"] <= 8
+        assert b.tuning() == t
+        # the tuning changes launch shapes and the BVH, never the result
+        assert np.array_equal(bits(b.bake(spp, B, seed=seed)[0]), bits(ref))
+    finally:
+        b.set_tuning(**old)
