@@ -15,6 +15,7 @@ workload.
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -62,6 +63,92 @@ def copy_probe(mib):
     print(json.dumps({"copy_probe_bytes_each_way": n * 4, "copies": 3}), flush=True)
 
 
+C3_SIZES = (64, 128, 256, 512, 1024, 2048)
+C3_SPP = (32, 64, 128, 256)
+
+
+def c3_jobs(n_jobs=16, seed=0x5EED0003):
+    """The 16-job mixed batch: seeded synthetic rooms of 1k-500k triangles
+    (log-uniform), lightmaps of 64^2..2048^2, 32..256 spp, 1..5 bounces,
+    supersample scale 1."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n_jobs):
+        tris = int(round(math.exp(rng.uniform(math.log(1e3), math.log(5e5)))))
+        res = int(rng.choice(C3_SIZES))
+        spp = int(rng.choice(C3_SPP))
+        bounces = int(rng.integers(1, 6))
+        out.append(dict(tris=tris, res=res, spp=spp, bounces=bounces, seed=int(seed + k)))
+    return out
+
+
+def small_jobs(n_jobs=48, seed=0x5EED0013):
+    """Many small lightmaps: 64^2..256^2, 1k-20k triangles, 16..64 spp,
+    1..3 bounces (the regime a batched launch is for)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n_jobs):
+        out.append(dict(tris=int(round(math.exp(rng.uniform(math.log(1e3), math.log(2e4))))),
+                        res=int(rng.choice((64, 128, 256))), spp=int(rng.choice((16, 32, 64))),
+                        bounces=int(rng.integers(1, 4)), seed=int(seed + k)))
+    return out
+
+
+def bench_batched(shb, scenes, specs, reps=3):
+    """Batched vs plain on a set of jobs, both with results left in device
+    memory, after autotuning the largest job's shape for all jobs."""
+    bakers = []
+    t0 = time.time()
+    for sp in specs:
+        s = scenes.scaled_synthetic_scene(sp["tris"], sp["res"], sp["seed"])
+        b = shb.Baker(0)
+        b.upload_scene(s)
+        b.rasterize(sp["res"], sp["res"], 1, want=False)
+        sp["valid_texels"] = int(b.validity_mask().sum())
+        sp["tris"] = s.num_triangles
+        bakers.append(b)
+    log(f"batched: {len(specs)} scenes built and rasterized in {time.time() - t0:.1f}s")
+    big = max(range(len(specs)), key=lambda k: specs[k]["valid_texels"] * specs[k]["spp"])
+    tuned = bakers[big].autotune(specs[big]["spp"], specs[big]["bounces"])
+    for b in bakers:
+        b.set_tuning(**tuned)
+    spp = [sp["spp"] for sp in specs]
+    bnc = [sp["bounces"] for sp in specs]
+    seeds = [sp["seed"] for sp in specs]
+
+    def plain_once():
+        rays = 0
+        for b, sp in zip(bakers, specs):
+            _, _, st = b.bake(sp["spp"], sp["bounces"], seed=sp["seed"], want=False)
+            rays += st["rays_extension"] + st["rays_shadow"]
+        return rays
+
+    def batched_once():
+        _, _, st = shb.bake_batched(bakers, spp, bnc, seeds, want=False)
+        return st["rays_extension"] + st["rays_shadow"]
+
+    plain_once()
+    batched_once()
+    tp, tb = [], []
+    for _ in range(reps):
+        ts = time.perf_counter()
+        rays_p = plain_once()
+        tp.append(time.perf_counter() - ts)
+        ts = time.perf_counter()
+        rays_b = batched_once()
+        tb.append(time.perf_counter() - ts)
+    tp, tb = float(np.median(tp)), float(np.median(tb))
+    for b in bakers:
+        b.close()
+    res = {"jobs": len(specs), "plain_seconds": round(tp, 4), "batched_seconds": round(tb, 4),
+           "speedup": round(tp / tb, 3), "rays": rays_b, "rays_equal": rays_b == rays_p,
+           "batched_mrays_s": round(rays_b / tb / 1e6, 1), "tuning": tuned,
+           "job_specs": [{k: sp[k] for k in ("tris", "res", "spp", "bounces", "valid_texels")}
+                         for sp in specs]}
+    log(f"batched: {json.dumps({k: v for k, v in res.items() if k != 'job_specs'})}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +164,8 @@ def main():
                     help="target CPU-baseline sample duration (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-count", action="store_true")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the 16-job batched-vs-plain measurement")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device copy of this size (counter calibration)")
     args = ap.parse_args()
@@ -176,6 +265,11 @@ def main():
                     "kernel": dom}
         log(f"counted bake: {json.dumps(kernels)}")
 
+    batched = small = None
+    if not args.no_batched:
+        batched = bench_batched(shb, scenes, c3_jobs())
+        small = bench_batched(shb, scenes, small_jobs())
+
     cpu = None
     if args.cpu_seconds > 0:
         from oracle import oracle as orc
@@ -230,6 +324,8 @@ def main():
                    "rays_per_step": rays // args.steps},
         "roofline": roofline,
         "roofline_kernels": kernels,
+        "batched_16_jobs": batched,
+        "batched_48_small_jobs": small,
         "cpu_baseline": cpu,
     }
     if cpu:

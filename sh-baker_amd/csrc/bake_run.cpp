@@ -8,7 +8,7 @@
 
 namespace shb {
 
-int prepare_job(JobIn& j) {
+int prepare_job_async(JobIn& j, hipStream_t stream, uint32_t* count_dev) {
   Ctx& c = *j.c;
   const shb_bake_cfg* cfg = &j.cfg;
   c.lm_sh = nullptr;  // the device buffers below are about to be reused
@@ -29,7 +29,7 @@ int prepare_job(JobIn& j) {
                 "materials with alpha < 1 (path splitting) are not supported by the GPU bake");
   SHB_TRY(c, hipSetDevice(c.device));
   const size_t npx = (size_t)c.width * c.ss * c.height * c.ss;
-  hipStream_t st = c.stream;
+  hipStream_t st = stream ? stream : c.stream;
   SHB_TRY(c, c.d_sh.reserve(std::max<size_t>(npx * 27 * sizeof(float), 16)));
   SHB_TRY(c, c.d_vis.reserve(std::max<size_t>(npx * sizeof(float), 16)));
   SHB_TRY(c, launch_fill_f32(c.d_sh.as<float>(), 0.f, npx * 27, st));
@@ -40,9 +40,22 @@ int prepare_job(JobIn& j) {
   SHB_TRY(c, launch_validity_mask(c.d_surface.as<float4>(), npx, c.d_mask.as<uint8_t>(), st));
   SHB_TRY(c, launch_compact_valid(c.d_mask.as<uint8_t>(), (uint32_t)npx, c.d_list.as<uint32_t>(),
                                   c.d_tmp.as<uint32_t>(), st));
+  SHB_TRY(c, hipMemcpyAsync(count_dev, c.d_list.as<uint32_t>() + npx, 4,
+                            hipMemcpyDeviceToDevice, st));
+  return SHB_OK;
+}
+
+int prepare_job(JobIn& j) {
+  Ctx& c = *j.c;
+  SHB_TRY(c, hipSetDevice(c.device));
+  SHB_TRY(c, c.d_work.reserve(kWorkBytes));
+  // the count goes through the error word's slot 1, read before any bake
+  uint32_t* cnt = c.d_work.as<uint32_t>() + 1;
+  int rc = prepare_job_async(j, c.stream, cnt);
+  if (rc) return rc;
   uint32_t V = 0;
-  SHB_TRY(c, hipMemcpyAsync(&V, c.d_list.as<uint32_t>() + npx, 4, hipMemcpyDeviceToHost, st));
-  SHB_TRY(c, hipStreamSynchronize(st));
+  SHB_TRY(c, hipMemcpyAsync(&V, cnt, 4, hipMemcpyDeviceToHost, c.stream));
+  SHB_TRY(c, hipStreamSynchronize(c.stream));
   j.valid = V;
   return SHB_OK;
 }

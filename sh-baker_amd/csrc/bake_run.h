@@ -19,8 +19,12 @@ struct JobIn {
 };
 
 // Validates job j's context and config, zeroes its outputs and builds its
-// valid-texel list on its own stream (returns after that stream drained).
-// Errors are reported on the job's context.
+// valid-texel list on stream `st` (the job context's own stream when null);
+// the valid-texel count lands in *count_dev. Nothing waits: the caller
+// synchronises `st` and copies the counts. Errors go to the job's context.
+int prepare_job_async(JobIn& j, hipStream_t st, uint32_t* count_dev);
+
+// prepare_job_async on the job's own stream, then waits and reads j.valid.
 int prepare_job(JobIn& j);
 
 // Runs the bake of every job on x's stream: one set of wavefront batches

@@ -36,10 +36,20 @@ int shb_bake_batched(shb_ctx* const* ctxs, int32_t num_jobs, const shb_bake_cfg*
     jobs[i].cfg = cfgs[i];
     jobs[i].stream = 0;  // same stream as a plain bake of the job: identical results
   }
+  // every job's texel list on the batch stream, one wait for all counts
+  // (work a job context still has queued on its own stream comes first)
+  SHB_TRY(x, hipSetDevice(x.device));
+  for (int32_t i = 0; i < num_jobs; ++i) SHB_TRY(x, hipStreamSynchronize(jobs[i].c->stream));
+  SHB_TRY(x, x.d_vcount.reserve((size_t)num_jobs * 4 + 16));
   for (int32_t i = 0; i < num_jobs; ++i) {
-    int rc = prepare_job(jobs[i]);
+    int rc = prepare_job_async(jobs[i], x.stream, x.d_vcount.as<uint32_t>() + i);
     if (rc) return fail(x, rc, "job " + std::to_string(i) + ": " + jobs[i].c->err);
   }
+  std::vector<uint32_t> counts((size_t)num_jobs);
+  SHB_TRY(x, hipMemcpyAsync(counts.data(), x.d_vcount.ptr, (size_t)num_jobs * 4,
+                            hipMemcpyDeviceToHost, x.stream));
+  SHB_TRY(x, hipStreamSynchronize(x.stream));
+  for (int32_t i = 0; i < num_jobs; ++i) jobs[i].valid = counts[i];
   BakeRun run;
   int rc = run_bake(x, jobs, true, &run);
   if (rc) return rc;

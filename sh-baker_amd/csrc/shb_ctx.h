@@ -72,7 +72,7 @@ struct Ctx {
   DevBuf d_adj;               // adjoint accumulators
   // batched / graph bakes run on this context: job table, batch maps,
   // concatenated BVHs of several jobs, the captured launch sequence
-  DevBuf d_jobs, d_maps, d_cnodes, d_ctris;
+  DevBuf d_jobs, d_maps, d_cnodes, d_ctris, d_vcount;
   hipGraphExec_t graph_exec = nullptr;
   std::string graph_key;
 

@@ -242,10 +242,12 @@ class Baker:
     def cfg(samples, bounces, seed=12345, flags=0, num_light_samples=1):
         return ShbBakeCfg(samples, bounces, num_light_samples, flags, seed)
 
-    def bake(self, samples, bounces, seed=12345, flags=0, num_light_samples=1):
+    def bake(self, samples, bounces, seed=12345, flags=0, num_light_samples=1, want=True):
+        """The supersampled SH9 bake; with want=False the result stays in
+        device memory and None is returned for the arrays."""
         n = self.width * self.ss * self.height * self.ss
-        sh = np.zeros((n, 27), np.float32)
-        vis = np.zeros(n, np.float32)
+        sh = np.zeros((n, 27), np.float32) if want else None
+        vis = np.zeros(n, np.float32) if want else None
         st = ShbBakeStats()
         cfg = self.cfg(samples, bounces, seed, flags, num_light_samples)
         self._chk(lib.shb_bake(self.h, C.byref(cfg), _f(sh), _f(vis), C.byref(st)))
