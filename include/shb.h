@@ -304,15 +304,19 @@ int shb_get_tuning(const shb_ctx* ctx, shb_tuning* tuning);
 
 /* ------------------------------------------- differentiation (no reference) */
 /* Per-light RGB radiance scales and per-material RGB albedo scales (all 1 by
- * default, which reproduces the reference bake). Either pointer may be NULL. */
+ * default, which reproduces the reference bake; finite and >= 0). Either
+ * pointer may be NULL. */
 int shb_set_params(shb_ctx* ctx, const float* light_scale,
                    const float* albedo_scale);
 
 /* Path-replay adjoint: given dL/dSH at the supersampled bake resolution
  * (same layout as shb_bake's sh_out), writes dL/d(light_scale) (3 per light)
- * and dL/d(albedo_scale) (3 per material). Light-selection probabilities are
- * held fixed (detached); every path is re-traced from its counter-based seed,
- * so memory stays O(texels). No reference counterpart. */
+ * and dL/d(albedo_scale) (3 per material) of shb_bake with the same cfg.
+ * The bake's light selection uses radiance without these scales, so the
+ * selection law does not depend on them; every path is re-traced from its
+ * counter-based random numbers (memory O(texels + paths in flight)) and the
+ * result is the exact derivative of the estimator for those numbers. Every
+ * scale must be > 0. No reference counterpart. */
 int shb_bake_adjoint(shb_ctx* ctx, const shb_bake_cfg* cfg,
                      const float* dloss_dsh, float* dloss_dlight,
                      float* dloss_dalbedo, shb_bake_stats* stats);

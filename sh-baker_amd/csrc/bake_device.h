@@ -39,9 +39,12 @@ __device__ __forceinline__ f3 get_emission(const SceneView& sc, int m, float u, 
 }
 
 // Lambert BRDF value albedo/pi (material.cpp:160-161) with the material's
-// albedo scale (1 reproduces the reference).
-__device__ __forceinline__ f3 lambert(const SceneView& sc, int m, float u, float v) {
+// albedo scale (1 reproduces the reference); `unscaled` receives the value
+// without the scale.
+__device__ __forceinline__ f3 lambert(const SceneView& sc, int m, float u, float v,
+                                      f3* unscaled = nullptr) {
   f3 a = get_albedo(sc, m, u, v);
+  if (unscaled) *unscaled = a / (float)kPi;
   const float* s = sc.albedo_scale + 3 * m;
   a = cmul(a, f3{s[0], s[1], s[2]});
   return a / (float)kPi;
@@ -94,13 +97,6 @@ struct NeeResult {
   int light;
 };
 
-// One NEE sample over all lights (light.cpp:82-189 at a path vertex when
-// VERTEX, light.cpp:191-293 at a texel otherwise). Unshadowed radiance and a
-// pending shadow ray per light (light.h:34-203; area lights sampled as in
-// light.cpp:16-78), weight = maxCoeff, selection by a streaming reservoir:
-// for each light draw u, W += w, keep light i when u*W < w -- the same
-// w_i / sum(w) selection law as std::discrete_distribution. Returns false
-// when nothing is selected, sum(w) < 1e-6 or joint pdf < 1e-3.
 // Point on an area-light triangle at folded barycentrics (u1, u2)
 // (light.cpp:34-68): position, area pdf and emission texture coordinates.
 __device__ __forceinline__ void area_point(const AreaTri& at, float u1, float u2, f3& target,
@@ -117,10 +113,20 @@ __device__ __forceinline__ void area_point(const AreaTri& at, float u1, float u2
   }
 }
 
+// One NEE sample over all lights (light.cpp:82-189 at a path vertex when
+// VERTEX, light.cpp:191-293 at a texel otherwise). Unshadowed radiance and a
+// pending shadow ray per light (light.h:34-203; area lights sampled as in
+// light.cpp:16-78), weight = maxCoeff, selection by a streaming reservoir:
+// for each light draw u, W += w, keep light i when u*W < w -- the same
+// w_i / sum(w) selection law as std::discrete_distribution. Returns false
+// when nothing is selected, sum(w) < 1e-6 or joint pdf < 1e-3.
+// Selection weights use the radiance without the light / albedo scales of
+// shb_set_params, so the selection law does not depend on the fitted
+// parameters (with every scale 1 this is the reference's law exactly); the
+// returned radiance carries the scales.
 template <bool VERTEX>
-__device__ __forceinline__ bool nee_sample(const SceneView& sc, f3 P, f3 N,
-                                           f3 brdf, bool front, Rng& rng,
-                                           NeeResult& out) {
+__device__ __forceinline__ bool nee_sample(const SceneView& sc, f3 P, f3 N, f3 brdf, f3 brdf_w,
+                                           bool front, Rng& rng, NeeResult& out) {
   float W = 0.f;
   int pick = -1;
   float pw = 0.f, pap = 1.f, ptfar = 0.f;
@@ -131,21 +137,14 @@ __device__ __forceinline__ bool nee_sample(const SceneView& sc, f3 P, f3 N,
     // record come through the scalar cache
     const LightDev L = uniform_load(sc.lights, (uint32_t)l);
     f3 rad = {0.f, 0.f, 0.f}, sdir;
-    float tfar, ap = 1.0f;
+    float tfar, ap = 1.0f, cos_n = 0.f;
     const int type = L.type;
     if (type == kLightDirectional) {
       f3 ld = {L.dir[0], L.dir[1], L.dir[2]};
       sdir = -ld;
       tfar = 1.0e10f;
-      float cos_n = dot(N, -ld);
-      if (!(cos_n < 0.f)) {
-        rad = f3{L.ic[0], L.ic[1], L.ic[2]};
-        if (VERTEX) {
-          bool ok = front && dot(N, -ld) > 0.0f;
-          f3 bv = ok ? brdf : f3{0.f, 0.f, 0.f};
-          rad = cmul(rad, cos_n * bv);
-        }
-      }
+      cos_n = dot(N, -ld);
+      if (!(cos_n < 0.f)) rad = f3{L.ic[0], L.ic[1], L.ic[2]};
     } else {
       f3 target;
       f3 emit = {0.f, 0.f, 0.f};
@@ -182,7 +181,7 @@ __device__ __forceinline__ bool nee_sample(const SceneView& sc, f3 P, f3 N,
       sdir = target - P;
       tfar = dist - 0.001f;
       Lv = Lv / dist;
-      float cos_n = dot(N, Lv);
+      cos_n = dot(N, Lv);
       if (type == kLightSpot) {
         f3 ld = {L.dir[0], L.dir[1], L.dir[2]};
         float cos_l = dot(ld, -Lv);
@@ -194,16 +193,21 @@ __device__ __forceinline__ bool nee_sample(const SceneView& sc, f3 P, f3 N,
       } else {
         if (!(cos_n < 0.f)) rad = emit / (dist * dist);
       }
-      if (VERTEX && !(cos_n < 0.f)) {
-        bool ok = front && cos_n > 0.0f;
-        f3 bv = ok ? brdf : f3{0.f, 0.f, 0.f};
-        rad = cmul(rad, cos_n * bv);
-      }
+    }
+    // BRDF * cosine at a path vertex: scaled albedo for the contribution,
+    // unscaled for the selection weight
+    f3 radw = rad;
+    if (VERTEX && !(cos_n < 0.f)) {
+      bool ok = front && cos_n > 0.0f;
+      f3 bv = ok ? brdf : f3{0.f, 0.f, 0.f};
+      f3 bw = ok ? brdf_w : f3{0.f, 0.f, 0.f};
+      radw = cmul(rad, cos_n * bw);
+      rad = cmul(rad, cos_n * bv);
     }
     const uint32_t l3 = 3u * (uint32_t)l;
     rad = cmul(rad, f3{uniform_load(sc.light_scale, l3), uniform_load(sc.light_scale, l3 + 1),
                        uniform_load(sc.light_scale, l3 + 2)});
-    float w = maxc(rad);
+    float w = maxc(radw);
     float u = rng.uniform();
     W += w;
     if (w > 0.f && u * W < w) {

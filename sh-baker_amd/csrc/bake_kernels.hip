@@ -77,6 +77,22 @@ struct BakeShadowEpilogue {
   }
 };
 
+// Shadow rays of the adjoint replay: a visible vertex NEE ray flags its slot
+// (its gradient is added by k_adjoint_vertex), a visible texel NEE ray
+// marks the texel term as in the forward bake.
+struct BakeShadowAdjEpilogue {
+  uint32_t* vflag;
+  float4* nee0;
+  uint32_t P;
+  __device__ void operator()(uint32_t slot, const TravState& s) const {
+    if (s.best != 0x7fffffff) return;
+    if (slot < P)
+      vflag[slot] = 1u;
+    else
+      nee0[slot - P].w = 1.0f;
+  }
+};
+
 // Traversal stack in LDS: entry-major, lane-minor (conflict-free per wave).
 struct LdsStack {
   uint2* mem;  // the kernel's dynamic __shared__ array
@@ -311,10 +327,16 @@ __global__ void k_generate(BakeArgs a) {
   }
 }
 
-// Hit shading at depth d.
+// Hit shading at depth d (ADJ: the replay form of the adjoint, with the
+// block's gradient sums in dynamic LDS).
+template <bool ADJ>
 __global__ void k_shade(BakeArgs a, int depth, const uint32_t* in_queue,
                         const uint32_t* in_count, uint32_t* out_queue,
                         uint32_t* out_count) {
+  extern __shared__ float grad_lds[];
+  GradAcc acc;
+  if (ADJ)
+    acc.init(a.adj_nparam <= kAdjLdsParams ? grad_lds : nullptr, a.adj_grad, a.adj_nparam);
   const uint32_t n = in_queue ? *in_count : a.P;
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
     uint32_t i = base + threadIdx.x;
@@ -323,36 +345,66 @@ __global__ void k_shade(BakeArgs a, int depth, const uint32_t* in_queue,
     if (i < n) {
       p = in_queue ? in_queue[i] : i;
       const SlotRef sr = slot_ref(a, p);
-      with_job(a, sr.job,
-               [&](const JobDev& J) { shade_path(a, J, sr, depth, p, shadow, extend); });
+      with_job(a, sr.job, [&](const JobDev& J) {
+        shade_path<ADJ>(a, J, sr, depth, p, shadow, extend, &acc);
+      });
     }
     uint32_t at = wave_append(a.shadow_count, shadow);
     if (shadow) a.shadow_queue[at] = p;
     uint32_t at2 = wave_append(out_count, extend);
     if (extend) out_queue[at2] = p;
   }
+  if (ADJ) acc.flush();
 }
 
-// One thread per texel of the batch.
-__global__ void k_project(BakeArgs a) {
-  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= a.T) return;
-  uint32_t job, rank, slot0, stride;
-  if (a.nseg <= 1) {
-    job = a.job0;
-    rank = a.texel_begin0 + t;
-    slot0 = t;
-    stride = a.T0;
-  } else {
-    const BatchMap& m = *a.map;
-    const uint32_t s = find_segment(m.tex_base, a.nseg, t);
-    const uint32_t tl = t - m.tex_base[s];
-    job = m.job[s];
-    rank = m.texel_begin[s] + tl;
-    slot0 = m.slot_base[s] + tl;
-    stride = m.tex_base[s + 1] - m.tex_base[s];
+// Adjoint of the visible vertex NEE terms of one depth: the shadow queue's
+// entries below P (texel NEE rays sit at P + slot).
+__global__ void k_adjoint_vertex(BakeArgs a, int depth, const uint32_t* sh_count) {
+  extern __shared__ float grad_lds[];
+  GradAcc acc;
+  acc.init(a.adj_nparam <= kAdjLdsParams ? grad_lds : nullptr, a.adj_grad, a.adj_nparam);
+  const uint32_t n = *sh_count;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t slot = a.shadow_queue[i];
+    if (slot >= a.P || a.adj_vflag[slot] == 0u) continue;
+    a.adj_vflag[slot] = 0u;
+    const SlotRef sr = slot_ref(a, slot);
+    adjoint_vertex(a, a.jobs[sr.job], depth, slot, &acc);
   }
-  with_job(a, job, [&](const JobDev& J) { project_texel(a, J, rank, slot0, stride); });
+  acc.flush();
+}
+
+// One thread per texel of the batch. The ADJ form ends with one block-wide
+// flush that every thread reaches (a single barrier per wave).
+template <bool ADJ>
+__global__ void k_project(BakeArgs a) {
+  extern __shared__ float grad_lds[];
+  GradAcc acc;
+  if (ADJ)
+    acc.init(a.adj_nparam <= kAdjLdsParams ? grad_lds : nullptr, a.adj_grad, a.adj_nparam);
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < a.T) {
+    uint32_t job, rank, slot0, stride;
+    if (a.nseg <= 1) {
+      job = a.job0;
+      rank = a.texel_begin0 + t;
+      slot0 = t;
+      stride = a.T0;
+    } else {
+      const BatchMap& m = *a.map;
+      const uint32_t s = find_segment(m.tex_base, a.nseg, t);
+      const uint32_t tl = t - m.tex_base[s];
+      job = m.job[s];
+      rank = m.texel_begin[s] + tl;
+      slot0 = m.slot_base[s] + tl;
+      stride = m.tex_base[s + 1] - m.tex_base[s];
+    }
+    if (ADJ)
+      project_texel_adjoint(a, a.jobs[job], rank, slot0, stride, &acc);
+    else
+      with_job(a, job, [&](const JobDev& J) { project_texel(a, J, rank, slot0, stride); });
+  }
+  if (ADJ) acc.flush();
 }
 
 __global__ void k_relocate_nodes(const CWNode* src, CWNode* dst, uint32_t n, uint32_t node_off,
@@ -421,6 +473,13 @@ hipError_t launch_trace_bake_shadow(const TraceArgs& a, const BakeArgs& b, bool 
                : launch_trace_t<true, false>(a, e, capacity, block, st);
 }
 
+hipError_t launch_trace_bake_shadow_adj(const TraceArgs& a, const BakeArgs& b, bool count,
+                                        uint32_t capacity, int block, hipStream_t st) {
+  BakeShadowAdjEpilogue e{b.adj_vflag, b.nee0, b.P};
+  return count ? launch_trace_t<true, true>(a, e, capacity, block, st)
+               : launch_trace_t<true, false>(a, e, capacity, block, st);
+}
+
 hipError_t launch_hit_details(const SceneView& sc, const float4* rays, const float4* hits,
                               uint32_t n, shb_hit* out, hipStream_t st) {
   if (n == 0) return hipSuccess;
@@ -434,10 +493,26 @@ hipError_t launch_generate(const BakeArgs& b, int grid, hipStream_t st) {
   return hipGetLastError();
 }
 
+// Dynamic LDS of an adjoint kernel: the block's gradient sums, when they fit.
+static size_t adj_lds(const BakeArgs& b) {
+  return b.adj_nparam <= kAdjLdsParams ? (size_t)b.adj_nparam * sizeof(float) : 0;
+}
+
 hipError_t launch_shade(const BakeArgs& b, int depth, const uint32_t* in_q,
                         const uint32_t* in_n, uint32_t* out_q, uint32_t* out_n, int grid,
                         hipStream_t st) {
-  hipLaunchKernelGGL(k_shade, dim3(grid), dim3(256), 0, st, b, depth, in_q, in_n, out_q, out_n);
+  if (b.adj_grad)
+    hipLaunchKernelGGL(k_shade<true>, dim3(grid), dim3(256), adj_lds(b), st, b, depth, in_q, in_n,
+                       out_q, out_n);
+  else
+    hipLaunchKernelGGL(k_shade<false>, dim3(grid), dim3(256), 0, st, b, depth, in_q, in_n, out_q,
+                       out_n);
+  return hipGetLastError();
+}
+
+hipError_t launch_adjoint_vertex(const BakeArgs& b, int depth, const uint32_t* sh_count, int grid,
+                                 hipStream_t st) {
+  hipLaunchKernelGGL(k_adjoint_vertex, dim3(grid), dim3(256), adj_lds(b), st, b, depth, sh_count);
   return hipGetLastError();
 }
 
@@ -452,7 +527,10 @@ hipError_t launch_relocate_nodes(const CWNode* src, CWNode* dst, uint32_t n, uin
 hipError_t launch_project(const BakeArgs& b, hipStream_t st) {
   uint32_t T = b.T;
   if (T == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_project, dim3((T + 127) / 128), dim3(128), 0, st, b);
+  if (b.adj_grad)
+    hipLaunchKernelGGL(k_project<true>, dim3((T + 127) / 128), dim3(128), adj_lds(b), st, b);
+  else
+    hipLaunchKernelGGL(k_project<false>, dim3((T + 127) / 128), dim3(128), 0, st, b);
   return hipGetLastError();
 }
 

@@ -78,13 +78,27 @@ __device__ __forceinline__ bool generate_path(const BakeArgs& a, const JobDev& J
   a.ray[2 * (size_t)p] = make_float4(origin.x, origin.y, origin.z, kRayEps);
   a.ray[2 * (size_t)p + 1] = make_float4(dw.x, dw.y, dw.z, INFINITY);
   a.dir0[p] = make_float4(dw.x, dw.y, dw.z, 0.f);
+  if (a.adj_grad) {
+    // d SH[k][c] / d Li_c = 2 pi Y_k(dir0) / spp (baker.cpp:286-292)
+    const float* g = a.adj_dsh + 27 * (size_t)texel;
+    float Y[9];
+    sh_basis(dw, Y);
+    float gr = 0.f, gg = 0.f, gb = 0.f;
+    for (int k = 0; k < 9; ++k) {
+      gr += g[3 * k + 0] * Y[k];
+      gg += g[3 * k + 1] * Y[k];
+      gb += g[3 * k + 2] * Y[k];
+    }
+    const float f = (float)(2.0 * kPi) * (1.0f / (float)J.spp);
+    a.adj_gvec[p] = make_float4(gr * f, gg * f, gb * f, 0.f);
+  }
   a.thr[p] = make_float4(1.f, 1.f, 1.f, as_float(1u));
   a.li[p] = make_float4(0.f, 0.f, 0.f, 0.f);
   NeeResult nr;
   f3 zero = {0.f, 0.f, 0.f};
-  if (nee_sample<false>(J.scene, P, N, zero, false, rng, nr)) {
+  if (nee_sample<false>(J.scene, P, N, zero, zero, false, rng, nr)) {
     a.nee0[p] = make_float4(nr.L.x, nr.L.y, nr.L.z, 0.f);
-    a.nee0_dir[p] = make_float4(nr.dir.x, nr.dir.y, nr.dir.z, 0.f);
+    a.nee0_dir[p] = make_float4(nr.dir.x, nr.dir.y, nr.dir.z, as_float((uint32_t)nr.light));
     size_t so = (size_t)a.P + p;
     a.sray[2 * so] = make_float4(P.x, P.y, P.z, kRayEps);
     a.sray[2 * so + 1] = make_float4(nr.dir.x, nr.dir.y, nr.dir.z, nr.tfar);
@@ -97,8 +111,12 @@ __device__ __forceinline__ bool generate_path(const BakeArgs& a, const JobDev& J
 // Shading of path slot p's closest hit at depth `depth` (baker.cpp:94-146):
 // sky, emission at depth 0, vertex NEE (shadow slot p), cosine-weighted
 // continuation. Sets shadow / extend when a ray was made.
+// ADJ: records the vertex's material for the replay adjoint and adds the
+// gradient of the depth-0 emission term to `acc`.
+template <bool ADJ>
 __device__ __forceinline__ void shade_path(const BakeArgs& a, const JobDev& J, const SlotRef& sr,
-                                           int depth, uint32_t p, bool& shadow, bool& extend) {
+                                           int depth, uint32_t p, bool& shadow, bool& extend,
+                                           GradAcc* acc) {
   shadow = extend = false;
   const SceneView& sc = J.scene;
   float4 h = a.hit[p];
@@ -131,10 +149,19 @@ __device__ __forceinline__ void shade_path(const BakeArgs& a, const JobDev& J, c
   f3 thr = f3_of(th4);
   uint32_t branch = as_uint(th4.w);
   f3 add = {0.f, 0.f, 0.f};
+  if (ADJ) a.adj_hist[(size_t)depth * a.P + p] = (uint16_t)mat;
   if (depth == 0 && sc.mats[mat].emission > 0.0f) {
     f3 em = get_emission(sc, mat, tu, tv);
     if (info.y >= 0) {
       const float* ls = sc.light_scale + 3 * info.y;
+      if (ADJ) {
+        // d/d scale of the emitter: unscaled emission times the projection
+        // weight (the throughput is 1 at depth 0)
+        const float4 g = a.adj_gvec[p];
+        acc->add(3u * info.y + 0, em.x * thr.x * g.x);
+        acc->add(3u * info.y + 1, em.y * thr.y * g.y);
+        acc->add(3u * info.y + 2, em.z * thr.z * g.z);
+      }
       em = cmul(em, f3{ls[0], ls[1], ls[2]});
     }
     add = cmul(thr, em);
@@ -142,12 +169,13 @@ __device__ __forceinline__ void shade_path(const BakeArgs& a, const JobDev& J, c
   f3 hit_pos = pos + nrm * 0.001f;
   f3 wo = -dir;
   bool front = dot(nrm, wo) > 0.0f;
-  f3 brdf = lambert(sc, mat, tu, tv);
+  f3 brdf_w;
+  f3 brdf = lambert(sc, mat, tu, tv, &brdf_w);
   Rng rng = site_rng(J.seed, texel, sr.smp, (uint32_t)depth, branch, kSiteVertex, J.stream);
   NeeResult nr;
-  if (nee_sample<true>(sc, hit_pos, nrm, brdf, front, rng, nr)) {
+  if (nee_sample<true>(sc, hit_pos, nrm, brdf, brdf_w, front, rng, nr)) {
     f3 c = cmul(thr, nr.L);
-    a.scontrib[p] = make_float4(c.x, c.y, c.z, 0.f);
+    a.scontrib[p] = make_float4(c.x, c.y, c.z, as_float((uint32_t)nr.light));
     a.sray[2 * (size_t)p] = make_float4(hit_pos.x, hit_pos.y, hit_pos.z, kRayEps);
     a.sray[2 * (size_t)p + 1] = make_float4(nr.dir.x, nr.dir.y, nr.dir.z, nr.tfar);
     shadow = true;
@@ -175,6 +203,55 @@ __device__ __forceinline__ void shade_path(const BakeArgs& a, const JobDev& J, c
       a.ray[2 * (size_t)p + 1] = make_float4(sd.x, sd.y, sd.z, INFINITY);
       extend = true;
     }
+  }
+}
+
+// Adjoint of a texel's NEE terms: for each visible sample, the light's
+// gradient gains (unscaled radiance) x (1/spp) sum_k dL/dSH[k][c] Y_k(d).
+__device__ __forceinline__ void project_texel_adjoint(const BakeArgs& a, const JobDev& J,
+                                                      uint32_t rank, uint32_t slot0,
+                                                      uint32_t stride, GradAcc* acc) {
+  uint32_t texel = J.texel_list[rank];
+  const float* g = a.adj_dsh + 27 * (size_t)texel;
+  const float inv = 1.0f / (float)J.spp;
+  for (uint32_t smp = 0; smp < J.spp; ++smp) {
+    uint32_t p = slot0 + smp * stride;
+    float4 n0 = a.nee0[p];
+    if (n0.w == 0.f) continue;
+    float4 nd = a.nee0_dir[p];
+    const uint32_t l = as_uint(nd.w);
+    float Y[9];
+    sh_basis(f3_of(nd), Y);
+    float gr = 0.f, gg = 0.f, gb = 0.f;
+    for (int k = 0; k < 9; ++k) {
+      gr += g[3 * k + 0] * Y[k];
+      gg += g[3 * k + 1] * Y[k];
+      gb += g[3 * k + 2] * Y[k];
+    }
+    const float* s = J.scene.light_scale + 3 * l;
+    acc->add(3 * l + 0, n0.x / s[0] * gr * inv);
+    acc->add(3 * l + 1, n0.y / s[1] * gg * inv);
+    acc->add(3 * l + 2, n0.z / s[2] * gb * inv);
+  }
+}
+
+// Adjoint of slot p's visible vertex NEE term at depth d: value
+// (throughput x radiance, per channel) times the projection weight, divided
+// by the light's scale for the light gradient and by each path vertex's
+// albedo scale (vertices 0..d) for the material gradients.
+__device__ __forceinline__ void adjoint_vertex(const BakeArgs& a, const JobDev& J, int depth,
+                                               uint32_t p, GradAcc* acc) {
+  const float4 c = a.scontrib[p];
+  const float4 g = a.adj_gvec[p];
+  const float G[3] = {c.x * g.x, c.y * g.y, c.z * g.z};
+  const uint32_t l = as_uint(c.w);
+  const float* s = J.scene.light_scale + 3 * l;
+  for (int ch = 0; ch < 3; ++ch) acc->add(3 * l + ch, G[ch] / s[ch]);
+  const uint32_t mbase = 3 * a.adj_nlights;
+  for (int k = 0; k <= depth; ++k) {
+    const uint32_t m = a.adj_hist[(size_t)k * a.P + p];
+    const float* as = J.scene.albedo_scale + 3 * m;
+    for (int ch = 0; ch < 3; ++ch) acc->add(mbase + 3 * m + ch, G[ch] / as[ch]);
   }
 }
 

@@ -115,8 +115,9 @@ struct Rec {
 
 }  // namespace
 
-int run_bake(Ctx& x, std::vector<JobIn>& jobs, bool graph, BakeRun* run) {
+int run_bake(Ctx& x, std::vector<JobIn>& jobs, bool graph, BakeRun* run, const AdjIn* adj) {
   SHB_TRY(x, hipSetDevice(x.device));
+  if (adj && jobs.size() != 1) return fail(x, SHB_ERR_INVALID_ARG, "the adjoint runs one job");
   hipStream_t st = x.stream;
   const uint32_t nj = (uint32_t)jobs.size();
   const bool count = (jobs[0].cfg.flags & SHB_BAKE_COUNT_VISITS) != 0;
@@ -206,8 +207,15 @@ int run_bake(Ctx& x, std::vector<JobIn>& jobs, bool graph, BakeRun* run) {
   SHB_TRY(x, hipMemcpyAsync(x.d_maps.ptr, maps.data(), maps.size() * sizeof(BatchMap),
                             hipMemcpyHostToDevice, st));
 
-  // path state: 13 float4 per slot; queues 4 uint32 per slot
-  SHB_TRY(x, x.d_paths.reserve(std::max<size_t>((size_t)Pmax * 13 * sizeof(float4), 16)));
+  // path state: 13 float4 per slot (+1 for the adjoint's projection
+  // weights); queues 4 uint32 per slot
+  const size_t f4_per_slot = adj ? 14 : 13;
+  SHB_TRY(x, x.d_paths.reserve(std::max<size_t>((size_t)Pmax * f4_per_slot * sizeof(float4), 16)));
+  uint32_t maxB_all = 0;
+  for (uint32_t v : maxB) maxB_all = std::max(maxB_all, v);
+  // adjoint: vertex materials (u16 per depth and slot), visible flags
+  const size_t hist_bytes = ((size_t)(maxB_all + 1) * Pmax * 2 + 15) / 16 * 16;
+  if (adj) SHB_TRY(x, x.d_adj.reserve(hist_bytes + (size_t)Pmax * 4 + 16));
   SHB_TRY(x, x.d_queues.reserve(std::max<size_t>((size_t)Pmax * 4 * sizeof(uint32_t), 16)));
   SHB_TRY(x, x.d_counters.reserve(std::max<size_t>((size_t)nb * kCntStride * 4, 64)));
   SHB_TRY(x, x.d_work.reserve(kWorkBytes));
@@ -253,6 +261,8 @@ int run_bake(Ctx& x, std::vector<JobIn>& jobs, bool graph, BakeRun* run) {
     };
     SHB_TRY(x, fill(x.d_counters.as<uint32_t>(), (size_t)nb * kCntStride));
     SHB_TRY(x, fill(x.d_work.as<uint32_t>(), kWorkBytes / 4));
+    if (adj)
+      SHB_TRY(x, fill(reinterpret_cast<uint32_t*>(x.d_adj.as<uint8_t>() + hist_bytes), Pmax));
     for (uint32_t bi = 0; bi < nb; ++bi) {
       const BatchMap& m = maps[bi];
       uint32_t* cnt = x.d_counters.as<uint32_t>() + (size_t)bi * kCntStride;
@@ -278,6 +288,15 @@ int run_bake(Ctx& x, std::vector<JobIn>& jobs, bool graph, BakeRun* run) {
       uint32_t* qext[2] = {q, q + P};
       b.shadow_queue = q + 2 * (size_t)P;
       b.shadow_count = cnt + kCntSh;
+      if (adj) {
+        b.adj_dsh = adj->dsh;
+        b.adj_grad = adj->grad;
+        b.adj_nlights = adj->nlights;
+        b.adj_nparam = adj->nparam;
+        b.adj_gvec = base + 13 * (size_t)P;
+        b.adj_hist = x.d_adj.as<uint16_t>();
+        b.adj_vflag = reinterpret_cast<uint32_t*>(x.d_adj.as<uint8_t>() + hist_bytes);
+      }
       TraceArgs tb = tbase;
       tb.map = b.map;
       tb.nseg = m.nseg;
@@ -322,10 +341,14 @@ int run_bake(Ctx& x, std::vector<JobIn>& jobs, bool graph, BakeRun* run) {
           r.add(5, sa);
         } else {
           trace_ev(1);
-          SHB_TRY(x, launch_trace_bake_shadow(sa, b, count, 2 * P, x.tuning.block_size, st));
+          if (adj)
+            SHB_TRY(x, launch_trace_bake_shadow_adj(sa, b, count, 2 * P, x.tuning.block_size, st));
+          else
+            SHB_TRY(x, launch_trace_bake_shadow(sa, b, count, 2 * P, x.tuning.block_size, st));
           trace_ev(1);
           ++run->launches_sh;
           SHB_TRY(x, stage_done(st, "shadow", bi, (int)d));
+          if (adj) SHB_TRY(x, launch_adjoint_vertex(b, (int)d, cnt + kCntSh + d, sgrid, st));
         }
       }
       if (r.dry) r.add(6, b);

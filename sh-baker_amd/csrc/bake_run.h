@@ -32,7 +32,20 @@ int prepare_job(JobIn& j);
 // BVHs). With `graph`, the launch sequence is captured once in a hipGraph
 // and replayed while the plan and buffers are unchanged. Errors are
 // reported on x.
-int run_bake(Ctx& x, std::vector<JobIn>& jobs, bool graph, BakeRun* run);
+// Path-replay adjoint inputs: dL/dSH on the device (27 floats per
+// supersampled texel) and the zeroed FP64 gradient sums (3 per light, then 3
+// per material).
+struct AdjIn {
+  const float* dsh = nullptr;
+  double* grad = nullptr;
+  uint32_t nlights = 0;
+  uint32_t nparam = 0;
+};
+
+// With `adj` (one job only), the replay adjoint runs instead of the bake:
+// the same paths from the same counters, gradients instead of SH sums.
+int run_bake(Ctx& x, std::vector<JobIn>& jobs, bool graph, BakeRun* run,
+             const AdjIn* adj = nullptr);
 
 }  // namespace shb
 

@@ -149,11 +149,52 @@ int shb_autotune(shb_ctx* h, const shb_bake_cfg* cfg, shb_tuning* best) {
   return SHB_OK;
 }
 
-int shb_bake_adjoint(shb_ctx* ctx, const shb_bake_cfg* cfg, const float* dloss_dsh,
+int shb_bake_adjoint(shb_ctx* h, const shb_bake_cfg* cfg, const float* dloss_dsh,
                      float* dloss_dlight, float* dloss_dalbedo, shb_bake_stats* stats) {
-  if (!ctx) return SHB_ERR_INVALID_ARG;
-  ctx->c.err = "shb_bake_adjoint: not available in this build yet";
-  return SHB_ERR_UNSUPPORTED;
+  if (!h) return SHB_ERR_INVALID_ARG;
+  Ctx& c = h->c;
+  c.err.clear();
+  if (!cfg) return fail(c, SHB_ERR_INVALID_ARG, "cfg is NULL");
+  if (!dloss_dsh) return fail(c, SHB_ERR_INVALID_ARG, "dloss_dsh is NULL");
+  if (!c.has_scene) return fail(c, SHB_ERR_NO_SCENE, "no scene uploaded");
+  for (float v : c.lscale_h)
+    if (!(v > 0.f)) return fail(c, SHB_ERR_INVALID_ARG, "the adjoint needs light scales > 0");
+  for (float v : c.ascale_h)
+    if (!(v > 0.f)) return fail(c, SHB_ERR_INVALID_ARG, "the adjoint needs albedo scales > 0");
+  const double t0 = now_s();
+  std::vector<JobIn> jobs(1);
+  jobs[0].c = &c;
+  jobs[0].cfg = *cfg;
+  jobs[0].cfg.flags &= ~(uint32_t)SHB_BAKE_COUNT_VISITS;
+  int rc = prepare_job(jobs[0]);
+  if (rc) return rc;
+  const size_t npx = (size_t)c.width * c.ss * c.height * c.ss;
+  const uint32_t nl = (uint32_t)c.hs.lights.size(), nm = (uint32_t)c.hs.mats.size();
+  if (nm > 65535) return fail(c, SHB_ERR_LIMIT, "the adjoint supports up to 65535 materials");
+  const uint32_t nparam = 3 * (nl + nm);
+  SHB_TRY(c, c.d_dsh.reserve(std::max<size_t>(npx * 27 * sizeof(float), 16)));
+  SHB_TRY(c, c.d_grad.reserve(std::max<size_t>((size_t)nparam * sizeof(double), 16)));
+  SHB_TRY(c, hipMemcpyAsync(c.d_dsh.ptr, dloss_dsh, npx * 27 * sizeof(float),
+                            hipMemcpyHostToDevice, c.stream));
+  SHB_TRY(c, launch_fill_u32(c.d_grad.as<uint32_t>(), 0u, (size_t)nparam * 2, c.stream));
+  AdjIn adj;
+  adj.dsh = c.d_dsh.as<float>();
+  adj.grad = c.d_grad.as<double>();
+  adj.nlights = nl;
+  adj.nparam = nparam;
+  BakeRun run;
+  rc = run_bake(c, jobs, (cfg->flags & SHB_BAKE_USE_GRAPH) != 0, &run, &adj);
+  if (rc) return rc;
+  std::vector<double> g(nparam);
+  if (nparam)
+    SHB_TRY(c, hipMemcpy(g.data(), c.d_grad.ptr, (size_t)nparam * sizeof(double),
+                         hipMemcpyDeviceToHost));
+  if (dloss_dlight)
+    for (uint32_t i = 0; i < 3 * nl; ++i) dloss_dlight[i] = (float)g[i];
+  if (dloss_dalbedo)
+    for (uint32_t i = 0; i < 3 * nm; ++i) dloss_dalbedo[i] = (float)g[3 * nl + i];
+  fill_stats(stats, run, (uint32_t)cfg->samples, t0);
+  return SHB_OK;
 }
 
 }  // extern "C"

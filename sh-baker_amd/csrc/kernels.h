@@ -97,7 +97,50 @@ struct BakeArgs {
   float4* scontrib;           // vertex NEE contribution per slot
   uint32_t* shadow_queue;
   uint32_t* shadow_count;
+  // path-replay adjoint (single job): dL/dSH per supersampled texel
+  // (27 floats), gradient sums (3 per light, then 3 per material), the
+  // per-slot projection weights, the materials of each path's vertices
+  // (hist[depth * P + slot]) and the visible flags of vertex NEE rays
+  const float* adj_dsh;
+  double* adj_grad;
+  uint32_t adj_nlights;
+  uint32_t adj_nparam;
+  float4* adj_gvec;
+  uint16_t* adj_hist;
+  uint32_t* adj_vflag;
 };
+
+// Gradient accumulation of a block: float sums in LDS (when the parameters
+// fit), flushed once per block into FP64 sums in global memory.
+struct GradAcc {
+  float* lds;     // null: add straight to global memory
+  double* g;
+  uint32_t n;
+#ifndef SHB_HOST_ONLY
+  __device__ void init(float* mem, double* global, uint32_t count) {
+    lds = mem;
+    g = global;
+    n = count;
+    if (lds)
+      for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) lds[i] = 0.f;
+    __syncthreads();
+  }
+  __device__ void add(uint32_t i, float v) {
+    if (v == 0.f) return;
+    if (lds)
+      atomicAdd(&lds[i], v);
+    else
+      atomicAdd(&g[i], (double)v);
+  }
+  __device__ void flush() {
+    __syncthreads();
+    if (lds)
+      for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
+        if (lds[i] != 0.f) atomicAdd(&g[i], (double)lds[i]);
+  }
+#endif
+};
+constexpr uint32_t kAdjLdsParams = 8192;  // LDS accumulators up to 32 KB
 
 #ifndef SHB_HOST_ONLY
 // Traversal launches: `capacity` is the largest possible queue length (the
@@ -110,11 +153,17 @@ hipError_t launch_trace_bake_shadow(const TraceArgs& a, const BakeArgs& b, bool 
                                     uint32_t capacity, int block, hipStream_t st);
 hipError_t launch_hit_details(const SceneView& sc, const float4* rays, const float4* hits,
                               uint32_t n, shb_hit* out, hipStream_t st);
+// With b.adj_grad set, the adjoint forms of generate / shade / project run.
 hipError_t launch_generate(const BakeArgs& b, int grid, hipStream_t st);
 hipError_t launch_shade(const BakeArgs& b, int depth, const uint32_t* in_q,
                         const uint32_t* in_n, uint32_t* out_q, uint32_t* out_n, int grid,
                         hipStream_t st);
 hipError_t launch_project(const BakeArgs& b, hipStream_t st);
+// Adjoint of the visible vertex NEE terms of depth `depth` (shadow queue).
+hipError_t launch_adjoint_vertex(const BakeArgs& b, int depth, const uint32_t* sh_count, int grid,
+                                 hipStream_t st);
+hipError_t launch_trace_bake_shadow_adj(const TraceArgs& a, const BakeArgs& b, bool count,
+                                        uint32_t capacity, int block, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t st);
 // Copies n BVH nodes, shifting child and triangle bases by node_off / tri_off
 // (concatenated trees of a batched bake).

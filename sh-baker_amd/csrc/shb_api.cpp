@@ -97,7 +97,7 @@ void shb_destroy(shb_ctx* h) {
                     &c.d_tmp,    &c.d_sh,       &c.d_vis,     &c.d_img_a,    &c.d_img_b,
                     &c.d_mask_a, &c.d_mask_b,   &c.d_rays,    &c.d_hits,     &c.d_hit_out,
                     &c.d_raycounts, &c.d_adj,   &c.d_jobs,    &c.d_maps,     &c.d_cnodes,
-                    &c.d_ctris,  &c.d_vcount};
+                    &c.d_ctris,  &c.d_vcount,  &c.d_dsh,     &c.d_grad};
   if (c.graph_exec) (void)hipGraphExecDestroy(c.graph_exec);
   c.graph_exec = nullptr;
   for (DevBuf* b : bufs) b->release();
@@ -132,6 +132,8 @@ int shb_upload_scene(shb_ctx* h, const shb_scene_desc* scene) {
   std::vector<float> ones_m(3 * std::max<size_t>(hs.mats.size(), 1), 1.f);
   SHB_TRY(c, upload(c.d_lscale, ones_l, c.stream));
   SHB_TRY(c, upload(c.d_ascale, ones_m, c.stream));
+  c.lscale_h = ones_l;
+  c.ascale_h = ones_m;
   SHB_TRY(c, upload(c.d_luv, hs.luv, c.stream));
   SHB_TRY(c, upload(c.d_wpos, hs.wpos, c.stream));
   SHB_TRY(c, upload(c.d_wnrm, hs.wnrm, c.stream));
@@ -148,12 +150,24 @@ int shb_set_params(shb_ctx* h, const float* light_scale, const float* albedo_sca
   Ctx& c = h->c;
   if (!c.has_scene) return fail(c, SHB_ERR_NO_SCENE, "no scene uploaded");
   SHB_TRY(c, hipSetDevice(c.device));
-  if (light_scale && !c.hs.lights.empty())
-    SHB_TRY(c, hipMemcpyAsync(c.d_lscale.ptr, light_scale, 12 * c.hs.lights.size(),
-                              hipMemcpyHostToDevice, c.stream));
-  if (albedo_scale && !c.hs.mats.empty())
-    SHB_TRY(c, hipMemcpyAsync(c.d_ascale.ptr, albedo_scale, 12 * c.hs.mats.size(),
-                              hipMemcpyHostToDevice, c.stream));
+  const size_t nl = 3 * c.hs.lights.size(), nm = 3 * c.hs.mats.size();
+  auto ok = [](const float* v, size_t n) {
+    for (size_t i = 0; i < n; ++i)
+      if (!(v[i] >= 0.f) || !std::isfinite(v[i])) return false;
+    return true;
+  };
+  if ((light_scale && !ok(light_scale, nl)) || (albedo_scale && !ok(albedo_scale, nm)))
+    return fail(c, SHB_ERR_INVALID_ARG, "scales must be finite and >= 0");
+  if (light_scale && nl) {
+    c.lscale_h.assign(light_scale, light_scale + nl);
+    SHB_TRY(c, hipMemcpyAsync(c.d_lscale.ptr, c.lscale_h.data(), 4 * nl, hipMemcpyHostToDevice,
+                              c.stream));
+  }
+  if (albedo_scale && nm) {
+    c.ascale_h.assign(albedo_scale, albedo_scale + nm);
+    SHB_TRY(c, hipMemcpyAsync(c.d_ascale.ptr, c.ascale_h.data(), 4 * nm, hipMemcpyHostToDevice,
+                              c.stream));
+  }
   SHB_TRY(c, hipStreamSynchronize(c.stream));
   return SHB_OK;
 }

@@ -73,6 +73,9 @@ struct Ctx {
   // batched / graph bakes run on this context: job table, batch maps,
   // concatenated BVHs of several jobs, the captured launch sequence
   DevBuf d_jobs, d_maps, d_cnodes, d_ctris, d_vcount;
+  // parameters of shb_set_params (host copies) and the adjoint's inputs
+  std::vector<float> lscale_h, ascale_h;
+  DevBuf d_dsh, d_grad;
   hipGraphExec_t graph_exec = nullptr;
   std::string graph_key;
 

@@ -331,47 +331,24 @@ class Baker:
         t = ShbTuning(**cur)
         self._chk(lib.shb_set_tuning(self.h, C.byref(t)))
 
-    def autotune(self, samples, bounces, seed=12345):
-        t = ShbTuning()
-        cfg = self.cfg(samples, bounces, seed)
-        self._chk(lib.shb_autotune(self.h, C.byref(cfg), C.byref(t)))
-        return {k: getattr(t, k) for k, _ in t._fields_}
+    def bake_adjoint(self, samples, bounces, dloss_dsh, seed=12345, flags=0):
+        """Path-replay adjoint of bake(): given dL/dSH at the supersampled
+        resolution ((n, 27) like bake's sh), returns dL/d(light_scale)
+        (lights, 3), dL/d(albedo_scale) (materials, 3) and the stats."""
+        n = self.width * self.ss * self.height * self.ss
+        g = np.ascontiguousarray(dloss_dsh, np.float32).reshape(n, 27)
+        nl, nm = len(self.scene.lights), len(self.scene.materials)
+        dl = np.zeros((nl, 3), np.float32)
+        dm = np.zeros((nm, 3), np.float32)
+        st = ShbBakeStats()
+        cfg = self.cfg(samples, bounces, seed, flags)
+        self._chk(lib.shb_bake_adjoint(self.h, C.byref(cfg), _f(g), _f(dl), _f(dm), C.byref(st)))
+        return dl, dm, st.as_dict()
 
     def set_params(self, light_scale=None, albedo_scale=None):
         ls = None if light_scale is None else np.ascontiguousarray(light_scale, np.float32)
         al = None if albedo_scale is None else np.ascontiguousarray(albedo_scale, np.float32)
         self._chk(lib.shb_set_params(self.h, _f(ls), _f(al)))
-
-    def bake_adjoint(self, dloss_dsh, samples, bounces, seed=12345):
-        g = np.ascontiguousarray(dloss_dsh, np.float32)
-        nl = len(self.scene.lights)
-        nm = len(self.scene.materials)
-        dl = np.zeros((max(nl, 1), 3), np.float32)
-        dm = np.zeros((max(nm, 1), 3), np.float32)
-        st = ShbBakeStats()
-        cfg = self.cfg(samples, bounces, seed)
-        self._chk(lib.shb_bake_adjoint(self.h, C.byref(cfg), _f(g), _f(dl), _f(dm), C.byref(st)))
-        return dl[:nl], dm[:nm], st.as_dict()
-
-
-def bake_batched(bakers, cfgs):
-    """Bakes several Baker contexts in one batched launch sequence."""
-    get_lib()
-    n = len(bakers)
-    hs = (C.c_void_p * n)(*[b.h for b in bakers])
-    carr = (ShbBakeCfg * n)(*[Baker.cfg(*c) if isinstance(c, tuple) else c for c in cfgs])
-    outs_sh, outs_vis = [], []
-    for b in bakers:
-        npx = b.width * b.ss * b.height * b.ss
-        outs_sh.append(np.zeros((npx, 27), np.float32))
-        outs_vis.append(np.zeros(npx, np.float32))
-    psh = (_fp * n)(*[_f(a) for a in outs_sh])
-    pvis = (_fp * n)(*[_f(a) for a in outs_vis])
-    st = ShbBakeStats()
-    rc = lib.shb_bake_batched(hs, n, carr, psh, pvis, C.byref(st))
-    if rc != SHB_OK:
-        raise ShbError(rc, lib.shb_last_error(bakers[0].h).decode())
-    return outs_sh, outs_vis, st.as_dict()
 
 
 def bake_batched(bakers, samples, bounces, seeds=None, flags=0, want=True):
