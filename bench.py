@@ -149,6 +149,76 @@ def bench_batched(shb, scenes, specs, reps=3):
     return res
 
 
+def bench_adjoint(shb, scenes, res=512, spp=64, bounces=3, iters=50, reps=3):
+    """Differentiation form: the 262k-triangle room at res^2 (ss 1), spp,
+    bounces; adjoint vs forward time, gradient vs central finite differences
+    under common random numbers, and an Adam fit of 69x3 light and 192x3
+    albedo scales to a target baked with seeded lognormal(0, 0.3) factors."""
+    from shb import fit
+    s = scenes.synthetic_room_scene(res, res)
+    b = shb.Baker(0)
+    b.upload_scene(s)
+    b.rasterize(res, res, 1, want=False)
+    nl, nm = len(s.lights), len(s.materials)
+    seed = 4242
+    true_l, true_a = fit.lognormal_factors(nl, nm, 0.3, seed=7)
+    b.set_params(true_l, true_a)
+    target = b.bake(spp, bounces, seed=seed)[0]
+    ones_l, ones_a = np.ones((nl, 3), np.float32), np.ones((nm, 3), np.float32)
+    b.set_params(ones_l, ones_a)
+    sh = b.bake(spp, bounces, seed=seed)[0]
+    g = (sh.astype(np.float64) - target).astype(np.float32) / sh.shape[0]
+    b.bake_adjoint(spp, bounces, g, seed=seed)  # warm-up
+    tf, ta = [], []
+    for _ in range(reps):
+        ts = time.perf_counter()
+        b.bake(spp, bounces, seed=seed, want=False)
+        tf.append(time.perf_counter() - ts)
+        ts = time.perf_counter()
+        dl, dm, _ = b.bake_adjoint(spp, bounces, g, seed=seed)
+        ta.append(time.perf_counter() - ts)
+    tf, ta = float(np.median(tf)), float(np.median(ta))
+
+    def loss(ls, al):
+        b.set_params(ls, al)
+        d = b.bake(spp, bounces, seed=seed)[0].astype(np.float64) - target
+        return 0.5 * float((d * d).sum()) / sh.shape[0]
+
+    fd_checks = []
+    for kind, grad in (("light", dl), ("albedo", dm)):
+        for flat in np.argsort(-np.abs(grad).ravel())[:3]:
+            i, c = int(flat // 3), int(flat % 3)
+            for h in (1e-3, 1e-2):
+                lp, lm_, ap, am = ones_l.copy(), ones_l.copy(), ones_a.copy(), ones_a.copy()
+                if kind == "light":
+                    lp[i, c], lm_[i, c] = 1 + h, 1 - h
+                    fdv = (loss(lp, ones_a) - loss(lm_, ones_a)) / (2 * h)
+                else:
+                    ap[i, c], am[i, c] = 1 + h, 1 - h
+                    fdv = (loss(ones_l, ap) - loss(ones_l, am)) / (2 * h)
+                fd_checks.append({"param": f"{kind}[{i}][{c}]", "h": h, "adjoint": float(grad[i, c]),
+                                  "fd": fdv, "rel_err": abs(float(grad[i, c]) - fdv) / max(abs(fdv), 1e-30)})
+    b.set_params(ones_l, ones_a)
+    t0 = time.perf_counter()
+    ls, al, losses = fit.fit(b, target, spp, bounces, iters=iters, lr=0.05, seed=seed)
+    t_fit = time.perf_counter() - t0
+    err_l = float(np.abs(np.log(ls / true_l)).mean())
+    err_a = float(np.abs(np.log(al / true_a)).mean())
+    b.close()
+    res_d = {"lightmap": f"{res}x{res}", "spp": spp, "bounces": bounces, "params": 3 * (nl + nm),
+             "forward_seconds": round(tf, 4), "adjoint_seconds": round(ta, 4),
+             "adjoint_forward_ratio": round(ta / tf, 3),
+             "fd_max_rel_err_h1e-3": max(f["rel_err"] for f in fd_checks if f["h"] == 1e-3),
+             "fd_max_rel_err_h1e-2": max(f["rel_err"] for f in fd_checks if f["h"] == 1e-2),
+             "fd_checks": fd_checks,
+             "adam_iters": iters, "loss_first": losses[0], "loss_last": losses[-1],
+             "fit_seconds": round(t_fit, 2),
+             "mean_abs_log_error_lights": round(err_l, 4),
+             "mean_abs_log_error_albedos": round(err_a, 4)}
+    log(f"adjoint: {json.dumps({k: v for k, v in res_d.items() if k != 'fd_checks'})}")
+    return res_d
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -166,6 +236,8 @@ def main():
     ap.add_argument("--no-count", action="store_true")
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the 16-job batched-vs-plain measurement")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the adjoint / fitting measurement")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device copy of this size (counter calibration)")
     args = ap.parse_args()
@@ -265,6 +337,8 @@ def main():
                     "kernel": dom}
         log(f"counted bake: {json.dumps(kernels)}")
 
+    adjoint = None if args.no_adjoint else bench_adjoint(shb, scenes)
+
     batched = small = None
     if not args.no_batched:
         batched = bench_batched(shb, scenes, c3_jobs())
@@ -324,6 +398,8 @@ def main():
                    "rays_per_step": rays // args.steps},
         "roofline": roofline,
         "roofline_kernels": kernels,
+        "adjoint_forward_ratio": adjoint["adjoint_forward_ratio"] if adjoint else None,
+        "adjoint_512": adjoint,
         "batched_16_jobs": batched,
         "batched_48_small_jobs": small,
         "cpu_baseline": cpu,

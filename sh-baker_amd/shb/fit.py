@@ -76,7 +76,7 @@ def fit(baker, target_sh, samples, bounces, iters=50, lr=0.05, seed=12345,
         loss = 0.5 * ((sh - target) ** 2).sum() / n
         loss.backward()
         opt.step()
-        losses.append(float(loss))
+        losses.append(float(loss.detach()))
         if callback:
-            callback(it, float(loss))
+            callback(it, losses[-1])
     return torch.exp(ll).detach().numpy(), torch.exp(la).detach().numpy(), losses
