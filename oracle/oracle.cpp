@@ -627,9 +627,12 @@ struct ShadowRay {
 // Unshadowed incident radiance of one light (light.h:34-203) and its pending
 // shadow ray. with_brdf selects the *Radiance<Brdf> form (cos * BRDF applied)
 // used at path vertices; otherwise the *IncomingRadiance form used at texels.
+// gpu_arith: one division per light, x * (1/d) and (1/d)^2 as the HIP
+// kernels compute them (GPU-stream mode); otherwise the reference's
+// per-component divisions.
 static V3 LightRadiance(const Scene& s, const Light& l, const AreaSample* as,
                         V3 P, V3 N, bool with_brdf, const Mat* mat, V2 uv,
-                        V3 wo, ShadowRay* ray) {
+                        V3 wo, ShadowRay* ray, bool gpu_arith = false) {
   auto brdf = [&](V3 dir) { return EvalMaterial(*mat, uv, N, dir, wo); };
   switch (l.type) {
     case SHB_LIGHT_DIRECTIONAL: {  // light.h:34-64
@@ -652,18 +655,21 @@ static V3 LightRadiance(const Scene& s, const Light& l, const AreaSample* as,
       ray->d = l.pos - P;
       ray->tnear = 0.001f;
       ray->tfar = dist - 0.001f;
-      L = L / dist;
+      const float inv = 1.0f / dist;
+      const float inv2 = inv * inv;
+      L = gpu_arith ? L * inv : L / dist;
       float cos_n = dot(N, L);
       V3 rad;
       if (l.type == SHB_LIGHT_POINT) {
         if (cos_n < 0.f) return V3{};
-        rad = l.intensity * l.color / (dist * dist);
+        rad = gpu_arith ? l.intensity * l.color * inv2 : l.intensity * l.color / (dist * dist);
       } else {
         float cos_l = dot(l.dir, -L);
         float falloff = (cos_l - l.cout) / (l.cin - l.cout);
         falloff = std::clamp(falloff, 0.0f, 1.0f);
         if (cos_n < 0.f) return V3{};
-        rad = l.intensity * l.color / (dist * dist) * falloff;
+        rad = gpu_arith ? l.intensity * l.color * inv2 * falloff
+                        : l.intensity * l.color / (dist * dist) * falloff;
       }
       if (!with_brdf) return rad;
       return cmul(rad, cos_n * brdf(L));
@@ -676,10 +682,11 @@ static V3 LightRadiance(const Scene& s, const Light& l, const AreaSample* as,
       ray->d = as->point - P;
       ray->tnear = 0.001f;
       ray->tfar = dist - 0.001f;
-      L = L / dist;
+      const float inv = 1.0f / dist;
+      L = gpu_arith ? L * inv : L / dist;
       float cos_n = dot(N, L);
       if (cos_n < 0.f) return V3{};
-      V3 rad = as->radiance / (dist * dist);
+      V3 rad = gpu_arith ? as->radiance * (inv * inv) : as->radiance / (dist * dist);
       if (!with_brdf) return rad;
       return cmul(rad, cos_n * brdf(L));
     }
@@ -853,15 +860,15 @@ static V3 NeeGpu(const Scene& s, V3 P, V3 N, V3 wo, const Mat* mat, V2 uv,
         float u1 = x.uniform();
         float u2 = x.uniform();
         AreaSample as = SampleAreaLightAt(s, l, tri, u1, u2);
-        rd = LightRadiance(s, l, &as, P, N, !texel, mat, uv, wo, &ray);
+        rd = LightRadiance(s, l, &as, P, N, !texel, mat, uv, wo, &ray, true);
         ap = as.pdf;
       } else {
         AreaSample as{};
-        rd = LightRadiance(s, l, &as, P, N, !texel, mat, uv, wo, &ray);
+        rd = LightRadiance(s, l, &as, P, N, !texel, mat, uv, wo, &ray, true);
         ap = as.pdf;
       }
     } else {
-      rd = LightRadiance(s, l, nullptr, P, N, !texel, mat, uv, wo, &ray);
+      rd = LightRadiance(s, l, nullptr, P, N, !texel, mat, uv, wo, &ray, true);
     }
     float w = maxc(rd);
     float u = x.uniform();

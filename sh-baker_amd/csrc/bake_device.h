@@ -180,18 +180,23 @@ __device__ __forceinline__ bool nee_sample(const SceneView& sc, f3 P, f3 N, f3 b
       float dist = sqrtf(dist_sq);
       sdir = target - P;
       tfar = dist - 0.001f;
-      Lv = Lv / dist;
+      // one division per light (the reference divides per component:
+      // light.h:82-93, 127-143, 176-188); the oracle's GPU-stream mode
+      // follows this arithmetic
+      const float inv = 1.0f / dist;
+      const float inv2 = inv * inv;
+      Lv = Lv * inv;
       cos_n = dot(N, Lv);
       if (type == kLightSpot) {
         f3 ld = {L.dir[0], L.dir[1], L.dir[2]};
         float cos_l = dot(ld, -Lv);
         float falloff = (cos_l - L.cos_outer) / (L.cos_inner - L.cos_outer);
         falloff = fminf(fmaxf(falloff, 0.0f), 1.0f);
-        if (!(cos_n < 0.f)) rad = f3{L.ic[0], L.ic[1], L.ic[2]} / (dist * dist) * falloff;
+        if (!(cos_n < 0.f)) rad = f3{L.ic[0], L.ic[1], L.ic[2]} * inv2 * falloff;
       } else if (type == kLightPoint) {
-        if (!(cos_n < 0.f)) rad = f3{L.ic[0], L.ic[1], L.ic[2]} / (dist * dist);
+        if (!(cos_n < 0.f)) rad = f3{L.ic[0], L.ic[1], L.ic[2]} * inv2;
       } else {
-        if (!(cos_n < 0.f)) rad = emit / (dist * dist);
+        if (!(cos_n < 0.f)) rad = emit * inv2;
       }
     }
     // BRDF * cosine at a path vertex: scaled albedo for the contribution,

@@ -1,6 +1,6 @@
 // Host BVH builder (see bvh_build.h).
 //
-// 1. Binary BVH by binned SAH (16 bins per axis on centroids, leaves of at
+// 1. Binary BVH by binned SAH (32 bins per axis on centroids, leaves of at
 //    most 3 triangles).
 // 2. Greedy collapse into nodes of up to `width` children: repeatedly open the
 //    largest-area internal child (Ylitie et al. 2017 use an optimal DP; the
@@ -75,7 +75,7 @@ struct Builder {
       uint32_t first, count;
     };
     std::vector<Item> work{{0, 0, n}};
-    constexpr int NB = 16;
+    constexpr int NB = 32;
     while (!work.empty()) {
       Item it = work.back();
       work.pop_back();
