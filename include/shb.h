@@ -286,6 +286,24 @@ int shb_trace_rays(shb_ctx* ctx, const shb_ray* rays, uint32_t n,
                    int32_t any_hit, shb_hit* hits, uint32_t* node_visits,
                    uint32_t* tri_tests);
 
+/* ------------------------------------------------------------ EXR output */
+/* Save layouts of the reference's SaveMode (src/saver.h:19-24). */
+enum {
+  SHB_SAVE_COMBINED = 0,         /* one file, channels L0.R .. L22.B + EnvVisibility */
+  SHB_SAVE_SPLIT = 1,            /* path_<coeff>.exr x 9, RGB (L0: RGBA, A = vis)   */
+  SHB_SAVE_LUMINANCE_PACKED = 2  /* path_packed_{0,1,2}.exr, RGBA: L0 + vis, luma of
+                                    L1m1 L10 L11 L2m2, luma of L2m1 L20 L21 L22     */
+};
+
+/* Writes an SH lightmap (sh[texel*27 + coeff*3 + channel], vis[texel] or
+ * NULL) of width x height texels as OpenEXR, HALF channels, ZIP
+ * compression. Host only (needs no context). On failure a message goes to
+ * err_out (err_len bytes, may be NULL). Replaces SaveSHLightMap
+ * (src/saver.cpp:103-422). */
+int shb_save_lightmap(const char* path, int32_t width, int32_t height,
+                      const float* sh, const float* vis, int32_t mode,
+                      char* err_out, size_t err_len);
+
 /* ------------------------------------------------ batched (no reference) */
 /* Bakes several contexts (each with its own scene and surface; all on the
  * same device) in one persistent launch per bounce over a shared ray queue,

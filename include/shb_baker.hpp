@@ -18,11 +18,13 @@
 #ifndef SHB_BAKER_HPP_
 #define SHB_BAKER_HPP_
 
+#include <algorithm>
 #include <array>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <filesystem>
 #include <limits>
 #include <memory>
 #include <optional>
@@ -508,6 +510,43 @@ inline void Dilate<float>(int width, int height, std::vector<float>& pixels,
   shb_ctx* c = detail::thread_context().get();
   int rc = shb_dilate(c, width, height, 1, pixels.data(), mask.data(), passes);
   if (rc != SHB_OK) detail::log_error("shb_dilate", c, rc);
+}
+
+// --------------------------------------------------------------- saver.h
+// Replaces SaveMode / SaveSHLightMap (src/saver.h:19-47, src/saver.cpp:
+// 103-422): OpenEXR, HALF channels, ZIP. kCombined: one file of 28 channels
+// (L0.R .. L22.B, EnvVisibility); kSplitChannels: nine files path_L0 ..
+// path_L22 (RGB; L0 also carries visibility in A); kLuminancePacked: three
+// RGBA files path_packed_0..2 (L0 RGB + visibility, then the Rec. 709 luma
+// of coefficients 1-4 and 5-8). Host code: needs no GPU.
+enum class SaveMode { kCombined, kSplitChannels, kLuminancePacked };
+
+inline bool SaveSHLightMap(const SHTexture& sh_texture,
+                           const Texture32F& environment_visibility_texture,
+                           const std::filesystem::path& path,
+                           SaveMode mode = SaveMode::kCombined) {
+  if (sh_texture.pixels.empty() || sh_texture.width <= 0 || sh_texture.height <= 0 ||
+      sh_texture.pixels.size() < (size_t)sh_texture.width * sh_texture.height) {
+    std::fprintf(stderr, "shb_baker: invalid SHTexture dimensions or empty pixels\n");
+    return false;
+  }
+  const size_t n = (size_t)sh_texture.width * sh_texture.height;
+  std::vector<float> sh = detail::pack_sh(sh_texture.pixels);
+  // missing visibility pixels are written as 0 (src/saver.cpp:166-172)
+  std::vector<float> vis(n, 0.0f);
+  const auto& v = environment_visibility_texture.pixel_data;
+  std::copy(v.begin(), v.begin() + std::min(v.size(), n), vis.begin());
+  const int32_t m = mode == SaveMode::kCombined ? SHB_SAVE_COMBINED
+                    : mode == SaveMode::kSplitChannels ? SHB_SAVE_SPLIT
+                                                       : SHB_SAVE_LUMINANCE_PACKED;
+  char err[512] = {0};
+  int rc = shb_save_lightmap(path.string().c_str(), sh_texture.width, sh_texture.height,
+                             sh.data(), vis.data(), m, err, sizeof(err));
+  if (rc != SHB_OK) {
+    std::fprintf(stderr, "shb_baker: SaveSHLightMap failed: %s\n", err);
+    return false;
+  }
+  return true;
 }
 
 // ------------------------------------------------------------ occlusion.h

@@ -28,6 +28,7 @@ LIGHT_POINT, LIGHT_DIRECTIONAL, LIGHT_SPOT, LIGHT_AREA = 0, 1, 2, 3
 GEOM_HAS_NORMALS, GEOM_HAS_TEXTURE_UVS, GEOM_HAS_LIGHTMAP_UVS, GEOM_HAS_TANGENTS = 1, 2, 4, 8
 BAKE_COUNT_VISITS = 1
 BAKE_USE_GRAPH = 2
+SAVE_COMBINED, SAVE_SPLIT, SAVE_LUMINANCE_PACKED = 0, 1, 2
 
 
 class ShbGeometry(C.Structure):
@@ -107,7 +108,7 @@ assert HIT_DTYPE.itemsize == 56
 
 EXPORTED = [
     "shb_create", "shb_destroy", "shb_last_error", "shb_version", "shb_upload_scene",
-    "shb_rasterize", "shb_set_surface", "shb_validity_mask", "shb_bake", "shb_bake_lightmap", "shb_read_lightmap",
+    "shb_rasterize", "shb_set_surface", "shb_validity_mask", "shb_bake", "shb_bake_lightmap", "shb_read_lightmap", "shb_save_lightmap",
     "shb_downsample", "shb_downsample_mask", "shb_dilate", "shb_trace_rays",
     "shb_bake_batched", "shb_autotune", "shb_set_tuning", "shb_get_tuning", "shb_set_params",
     "shb_bake_adjoint",
@@ -135,6 +136,8 @@ def _load():
     lib.shb_bake_lightmap.argtypes = [vp, C.POINTER(ShbBakeCfg), C.c_int32, _fp, _fp, u8p,
                                       C.POINTER(ShbBakeStats)]
     lib.shb_read_lightmap.argtypes = [vp, _fp, _fp, u8p]
+    lib.shb_save_lightmap.argtypes = [C.c_char_p, C.c_int32, C.c_int32, _fp, _fp, C.c_int32,
+                                      C.c_char_p, C.c_size_t]
     lib.shb_downsample.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _fp, _fp]
     lib.shb_downsample_mask.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, u8p, u8p]
     lib.shb_dilate.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, _fp, u8p, C.c_int32]
@@ -375,3 +378,15 @@ def bake_batched(bakers, samples, bounces, seeds=None, flags=0, want=True):
     if rc != SHB_OK:
         raise ShbError(rc, lib.shb_last_error(bakers[0].h).decode())
     return shs, viss, st.as_dict()
+
+
+def save_lightmap(path, sh, vis, width, height, mode=SAVE_COMBINED):
+    """Writes an SH lightmap ((width*height, 27) and (width*height,)) as
+    OpenEXR (HALF, ZIP) in one of the reference's three layouts."""
+    get_lib()
+    sh = np.ascontiguousarray(sh, np.float32).reshape(width * height, 27)
+    vis = None if vis is None else np.ascontiguousarray(vis, np.float32).reshape(width * height)
+    err = C.create_string_buffer(512)
+    rc = lib.shb_save_lightmap(str(path).encode(), width, height, _f(sh), _f(vis), mode, err, 512)
+    if rc != SHB_OK:
+        raise ShbError(rc, err.value.decode())
