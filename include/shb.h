@@ -304,6 +304,27 @@ int shb_save_lightmap(const char* path, int32_t width, int32_t height,
                       const float* sh, const float* vis, int32_t mode,
                       char* err_out, size_t err_len);
 
+/* ------------------------------------------------------- environment SH */
+/* SH9 (sh_out[coeff*3 + channel]) of an equirectangular radiance map of
+ * width x height texels with `channels` >= 3 floats per texel (the first
+ * three are RGB), +Y up, phi from +Z towards +X, rows with sin(theta) <
+ * 1e-4 skipped. Runs on the context's device; pixels == NULL reuses the map
+ * uploaded by the previous call (same size). seconds_device (may be NULL)
+ * receives the kernels' time. Replaces ProjectEnvironmentToSH for
+ * Environment::Type::Texture, i.e. ProjectLatLongMap (src/scene.cpp:31-71,
+ * 152-158); sums are FP64 in a fixed order instead of a serial float sum. */
+int shb_env_sh_latlong(shb_ctx* ctx, int32_t width, int32_t height,
+                       int32_t channels, const float* pixels, float* sh_out,
+                       double* seconds_device);
+
+/* SH9 of the clamped Preetham/Perez clear sky for a sun direction (any
+ * length; normalised first): 64 x 128 midpoint quadrature of the upper
+ * hemisphere, radiance (0.2, 0.5, 0.9) * min(F, 30 F_zenith) / F_zenith.
+ * Host only (needs no context). Replaces ProjectEnvironmentToSH for
+ * Environment::Type::Preetham, i.e. ProjectPreethamToSH
+ * (src/scene.cpp:73-150). */
+int shb_env_sh_preetham(const float sun_direction[3], float* sh_out);
+
 /* ------------------------------------------------ batched (no reference) */
 /* Bakes several contexts (each with its own scene and surface; all on the
  * same device) in one persistent launch per bounce over a shared ray queue,

@@ -114,10 +114,34 @@ struct Light {
   int material_index = -1;
 };
 
+// ------------------------------------------------------- (sh_coeffs.h)
+struct SHCoeffs {
+  std::array<Vector3f, 9> coeffs{};  // L0, L1m1, L10, L11, L2m2, L2m1, L20, L21, L22
+  SHCoeffs& operator+=(const SHCoeffs& o) {
+    for (int i = 0; i < 9; ++i)
+      for (int c = 0; c < 3; ++c) coeffs[i][c] += o.coeffs[i][c];
+    return *this;
+  }
+};
+
+// Replaces Environment (src/scene.h:90-107): an HDR LatLong texture or a
+// Preetham sky from a sun direction, with its SH9 projection.
+struct Environment {
+  enum class Type { Texture, Preetham };
+  Type type = Type::Preetham;
+  Texture32F texture;
+  float intensity_multiplier = 1.0f;
+  Vector3f sun_direction = Vector3f(0, 1, 0);
+  float sun_intensity = 1.0f;
+  float turbidity = 2.5f;
+  SHCoeffs sh_coeffs;
+};
+
 struct Scene {
   std::vector<Geometry> geometries;
   std::vector<Material> materials;
   std::vector<Light> lights;
+  std::optional<Environment> environment;
 };
 
 // ---------------------------------------------- raster (rasterizer.h)
@@ -134,16 +158,7 @@ struct SurfacePoint {
   int32_t material_id = -1;  // Invalid if < 0.
 };
 
-// ------------------------------------------------ SH (sh_coeffs.h, saver.h)
-struct SHCoeffs {
-  std::array<Vector3f, 9> coeffs{};  // L0, L1m1, L10, L11, L2m2, L2m1, L20, L21, L22
-  SHCoeffs& operator+=(const SHCoeffs& o) {
-    for (int i = 0; i < 9; ++i)
-      for (int c = 0; c < 3; ++c) coeffs[i][c] += o.coeffs[i][c];
-    return *this;
-  }
-};
-
+// ----------------------------------------------------------- (saver.h)
 struct SHTexture {
   int width = 0;
   int height = 0;
@@ -547,6 +562,41 @@ inline bool SaveSHLightMap(const SHTexture& sh_texture,
     return false;
   }
   return true;
+}
+
+// ------------------------------------------------------------- scene.h
+// Replaces ProjectEnvironmentToSH (src/scene.cpp:152-158): a LatLong texture
+// (channels >= 3) is projected on the GPU (per-thread context), the Preetham
+// sky on the host. On failure logs and returns zero coefficients.
+inline SHCoeffs ProjectEnvironmentToSH(const Environment& env) {
+  SHCoeffs out;
+  float f[27] = {0};
+  int rc;
+  if (env.type == Environment::Type::Texture) {
+    shb_ctx* c = detail::thread_context().get();
+    if (!c) return out;
+    const Texture32F& t = env.texture;
+    if ((size_t)t.width * t.height * t.channels > t.pixel_data.size()) {
+      std::fprintf(stderr, "shb_baker: environment texture smaller than its size\n");
+      return out;
+    }
+    rc = shb_env_sh_latlong(c, (int32_t)t.width, (int32_t)t.height, (int32_t)t.channels,
+                            t.pixel_data.data(), f, nullptr);
+    if (rc != SHB_OK) {
+      detail::log_error("shb_env_sh_latlong", c, rc);
+      return out;
+    }
+  } else {
+    const float s[3] = {env.sun_direction.x(), env.sun_direction.y(), env.sun_direction.z()};
+    rc = shb_env_sh_preetham(s, f);
+    if (rc != SHB_OK) {
+      std::fprintf(stderr, "shb_baker: shb_env_sh_preetham failed (%d)\n", rc);
+      return out;
+    }
+  }
+  for (int i = 0; i < 9; ++i)
+    for (int k = 0; k < 3; ++k) out.coeffs[i][k] = f[3 * i + k];
+  return out;
 }
 
 // ------------------------------------------------------------ occlusion.h

@@ -568,11 +568,10 @@ static V3 HemisphereUV(float u1, float u2) {
 
 // --------------------------------------------------------------- SH9
 // src/sh_coeffs.cpp:50-78 (constants :27-31), direction not normalised.
-static void AccumulateRadiance(V3 L, V3 d, float* sh) {
+static void ShBasis(V3 d, float Y[9]) {
   const float k0 = 0.282095f, k1 = 0.488603f, k21 = 1.092548f,
               k20 = 0.315392f, k22 = 0.546274f;
   float x = d.x, y = d.y, z = d.z;
-  float Y[9];
   Y[0] = k0;
   Y[1] = k1 * y;
   Y[2] = k1 * z;
@@ -582,6 +581,10 @@ static void AccumulateRadiance(V3 L, V3 d, float* sh) {
   Y[6] = k20 * (3.0f * z * z - 1.0f);
   Y[7] = k21 * x * z;
   Y[8] = k22 * (x * x - y * y);
+}
+static void AccumulateRadiance(V3 L, V3 d, float* sh) {
+  float Y[9];
+  ShBasis(d, Y);
   for (int i = 0; i < 9; ++i) {
     sh[3 * i + 0] += L.x * Y[i];
     sh[3 * i + 1] += L.y * Y[i];
@@ -1522,6 +1525,75 @@ int orc_bake(void* sp, const shb_surface_point* pts, int width, int height,
     stats->threads = threads;
   }
   return 0;
+}
+
+// ------------------------------------------------------- environment SH
+// src/scene.cpp:31-71 ProjectLatLongMap: the reference's serial float sum
+// into sh_ref, and the FP64 sum of the same float terms into sh64.
+void orc_env_latlong(const float* px, int w, int h, int ch, float* sh_ref, double* sh64) {
+  const float kPiF = 3.14159265359f, kTwoPiF = 6.28318530718f;  // scene.cpp:12-13
+  const float d_phi = kTwoPiF / w, d_theta = kPiF / h;
+  std::fill(sh_ref, sh_ref + 27, 0.0f);
+  std::fill(sh64, sh64 + 27, 0.0);
+  for (int y = 0; y < h; ++y) {
+    const float theta = (y + 0.5f) * d_theta;
+    const float sin_theta = std::sin(theta);
+    const float d_omega = d_phi * d_theta * sin_theta;
+    if (sin_theta < 1e-4f) continue;
+    for (int x = 0; x < w; ++x) {
+      const float phi = (x + 0.5f) * d_phi;
+      const V3 dir{std::sin(theta) * std::sin(phi), std::cos(theta),
+                   std::sin(theta) * std::cos(phi)};
+      const size_t idx = ((size_t)y * w + x) * ch;
+      const V3 L = V3{px[idx], px[idx + 1], px[idx + 2]} * d_omega;
+      AccumulateRadiance(L, dir, sh_ref);
+      float Y[9];
+      ShBasis(dir, Y);
+      for (int i = 0; i < 9; ++i) {
+        sh64[3 * i + 0] += (double)(L.x * Y[i]);
+        sh64[3 * i + 1] += (double)(L.y * Y[i]);
+        sh64[3 * i + 2] += (double)(L.z * Y[i]);
+      }
+    }
+  }
+}
+
+// src/scene.cpp:75-94 EvaluatePerez, coefficients :15-20.
+static float EvaluatePerez(V3 dir, V3 sun) {
+  const float A = -1.0f, B = -0.32f, C = 10.0f, D = -3.0f, E = 0.45f;
+  const float cos_zenith = std::max(0.001f, dir.y);
+  const float cos_sun = dot(dir, sun);
+  const float gamma = std::acos(std::clamp(cos_sun, -1.0f, 1.0f));
+  const float f_zenith = (1.0f + A * std::exp(B / cos_zenith));
+  const float f_sun = (1.0f + C * std::exp(D * gamma) + E * cos_sun * cos_sun);
+  return f_zenith * f_sun;
+}
+
+// src/scene.cpp:96-150 ProjectPreethamToSH.
+void orc_env_preetham(const float* sun_direction, float* sh) {
+  const float kPiF = 3.14159265359f, kTwoPiF = 6.28318530718f;
+  const int steps_theta = 64, steps_phi = 128;
+  const float d_theta = kPiF / steps_theta, d_phi = kTwoPiF / steps_phi;
+  V3 s{sun_direction[0], sun_direction[1], sun_direction[2]};
+  const float n2 = dot(s, s);
+  const V3 sun = n2 > 0.0f ? s / std::sqrt(n2) : s;  // Eigen normalized()
+  float zenith = EvaluatePerez(V3{0, 1, 0}, sun);
+  if (zenith <= 1e-6f) zenith = 1.0f;
+  std::fill(sh, sh + 27, 0.0f);
+  for (int t = 0; t < steps_theta; ++t) {
+    const float theta = (t + 0.5f) * d_theta;
+    const float sin_theta = std::sin(theta);
+    for (int p = 0; p < steps_phi; ++p) {
+      const float phi = (p + 0.5f) * d_phi;
+      const float y = std::cos(theta), x = sin_theta * std::sin(phi),
+                  z = sin_theta * std::cos(phi);
+      const V3 dir{x, y, z};
+      if (y < 0.0f) continue;
+      const float lum = std::min(EvaluatePerez(dir, sun), zenith * 30.0f);
+      const V3 rad = V3{0.2f, 0.5f, 0.9f} * (lum / zenith);
+      AccumulateRadiance(rad * (sin_theta * d_theta * d_phi), dir, sh);
+    }
+  }
 }
 
 }  // extern "C"

@@ -53,6 +53,9 @@ def lib():
         L.orc_site_uniforms.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32,
                                         C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, fp]
         L.orc_sh_accumulate.argtypes = [fp, fp, fp]
+        L.orc_env_latlong.argtypes = [fp, C.c_int, C.c_int, C.c_int, fp,
+                                      C.POINTER(C.c_double)]
+        L.orc_env_preetham.argtypes = [fp, fp]
         L.orc_srgb_to_linear.argtypes = [C.c_uint8]
         L.orc_srgb_to_linear.restype = C.c_float
         L.orc_sample_material.argtypes = [fp, C.c_float, C.c_float, fp]
@@ -112,6 +115,23 @@ def sh_accumulate(radiance, direction, sh=None):
     sh = np.zeros(27, np.float32) if sh is None else np.ascontiguousarray(sh, np.float32)
     lib().orc_sh_accumulate(_f(_vec(radiance, 3)), _f(_vec(direction, 3)), _f(sh))
     return sh
+
+
+def env_latlong(pixels):
+    """ProjectLatLongMap of an (h, w, ch) float map: (reference serial float
+    sum, FP64 sum of the same terms), each (27,)."""
+    px = np.ascontiguousarray(pixels, np.float32)
+    h, w, ch = px.shape
+    ref = np.zeros(27, np.float32)
+    s64 = np.zeros(27, np.float64)
+    lib().orc_env_latlong(_f(px), w, h, ch, _f(ref), s64.ctypes.data_as(C.POINTER(C.c_double)))
+    return ref, s64
+
+
+def env_preetham(sun):
+    out = np.zeros(27, np.float32)
+    lib().orc_env_preetham(_f(_vec(sun, 3)), _f(out))
+    return out
 
 
 def srgb_to_linear(x):

@@ -190,7 +190,17 @@ struct RasterArgs {
   float4* surface;            // 3 float4 per texel (out)
 };
 
+// Launch shape of the LatLong environment projection: gx x gy blocks of 256
+// columns x rows_per_block rows; gx * gy * 27 FP64 partials.
+struct EnvGrid {
+  int gx = 0, gy = 0, rows_per_block = 0;
+};
+EnvGrid env_grid(int w, int h);
+
 #ifndef SHB_HOST_ONLY
+// SH9 of an RGB(A) equirectangular map (channels >= 3); out: 27 FP64.
+hipError_t launch_env_latlong(const float* px, int w, int h, int ch, const EnvGrid& g,
+                              double* partials, double* out, hipStream_t st);
 // counts: num_tris + 1 uint32 scratch; scan: num_tris + 1 uint64 scratch.
 hipError_t launch_raster(const RasterArgs& a, uint32_t* counts, unsigned long long* scan,
                          hipStream_t st);

@@ -78,6 +78,9 @@ struct Ctx {
   DevBuf d_dsh, d_grad;
   hipGraphExec_t graph_exec = nullptr;
   std::string graph_key;
+  // environment map of the last shb_env_sh_latlong call, its partial sums
+  DevBuf d_env, d_env_part;
+  int env_w = 0, env_h = 0, env_ch = 0;
 
   SceneView view() const {
     SceneView v;

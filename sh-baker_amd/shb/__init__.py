@@ -109,6 +109,7 @@ assert HIT_DTYPE.itemsize == 56
 EXPORTED = [
     "shb_create", "shb_destroy", "shb_last_error", "shb_version", "shb_upload_scene",
     "shb_rasterize", "shb_set_surface", "shb_validity_mask", "shb_bake", "shb_bake_lightmap", "shb_read_lightmap", "shb_save_lightmap",
+    "shb_env_sh_latlong", "shb_env_sh_preetham",
     "shb_downsample", "shb_downsample_mask", "shb_dilate", "shb_trace_rays",
     "shb_bake_batched", "shb_autotune", "shb_set_tuning", "shb_get_tuning", "shb_set_params",
     "shb_bake_adjoint",
@@ -138,6 +139,9 @@ def _load():
     lib.shb_read_lightmap.argtypes = [vp, _fp, _fp, u8p]
     lib.shb_save_lightmap.argtypes = [C.c_char_p, C.c_int32, C.c_int32, _fp, _fp, C.c_int32,
                                       C.c_char_p, C.c_size_t]
+    lib.shb_env_sh_latlong.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, _fp, _fp,
+                                       C.POINTER(C.c_double)]
+    lib.shb_env_sh_preetham.argtypes = [_fp, _fp]
     lib.shb_downsample.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _fp, _fp]
     lib.shb_downsample_mask.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, u8p, u8p]
     lib.shb_dilate.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, _fp, u8p, C.c_int32]
@@ -348,6 +352,22 @@ class Baker:
         self._chk(lib.shb_bake_adjoint(self.h, C.byref(cfg), _f(g), _f(dl), _f(dm), C.byref(st)))
         return dl, dm, st.as_dict()
 
+    def env_sh_latlong(self, pixels=None, shape=None):
+        """SH9 (27,) of an (h, w, ch >= 3) equirectangular float map on the
+        device; pixels None reuses the map of the previous call (shape =
+        (h, w, ch) then). Returns (sh, kernel seconds)."""
+        if pixels is not None:
+            px = np.ascontiguousarray(pixels, np.float32)
+            h, w, ch = px.shape
+            ptr = _f(px)
+        else:
+            h, w, ch = shape
+            ptr = None
+        out = np.zeros(27, np.float32)
+        sec = C.c_double(0.0)
+        self._chk(lib.shb_env_sh_latlong(self.h, w, h, ch, ptr, _f(out), C.byref(sec)))
+        return out, sec.value
+
     def set_params(self, light_scale=None, albedo_scale=None):
         ls = None if light_scale is None else np.ascontiguousarray(light_scale, np.float32)
         al = None if albedo_scale is None else np.ascontiguousarray(albedo_scale, np.float32)
@@ -378,6 +398,17 @@ def bake_batched(bakers, samples, bounces, seeds=None, flags=0, want=True):
     if rc != SHB_OK:
         raise ShbError(rc, lib.shb_last_error(bakers[0].h).decode())
     return shs, viss, st.as_dict()
+
+
+def env_sh_preetham(sun_direction):
+    """SH9 (27,) of the clamped Preetham clear sky for a sun direction
+    (host computation)."""
+    get_lib()
+    out = np.zeros(27, np.float32)
+    rc = lib.shb_env_sh_preetham(_f(np.ascontiguousarray(sun_direction, np.float32)), _f(out))
+    if rc != SHB_OK:
+        raise ShbError(rc, "bad sun direction")
+    return out
 
 
 def save_lightmap(path, sh, vis, width, height, mode=SAVE_COMBINED):

@@ -295,6 +295,36 @@ TEST(DilationTest, AverageNeighbors) {
   EXPECT_NEAR(pixels[center].coeffs[0].z(), 0.5f, 0.001f);
 }
 
+// scene_test.cpp:96-126
+TEST(SceneTest, ProjectEnvironmentUniformWhite) {
+  Texture32F tex;
+  tex.width = 64;
+  tex.height = 32;
+  tex.channels = 3;
+  tex.pixel_data.resize(tex.width * tex.height * 3, 1.0f);
+  Environment env;
+  env.type = Environment::Type::Texture;
+  env.texture = tex;
+  SHCoeffs coeffs = ProjectEnvironmentToSH(env);
+  float expected_L0 = 0.282095f * 4.0f * M_PI;
+  EXPECT_NEAR(coeffs.coeffs[0].x(), expected_L0, 0.1f);
+  EXPECT_NEAR(coeffs.coeffs[0].y(), expected_L0, 0.1f);
+  EXPECT_NEAR(coeffs.coeffs[0].z(), expected_L0, 0.1f);
+  for (int i = 1; i < 9; ++i) EXPECT_NEAR(coeffs.coeffs[i].x(), 0.0f, 0.1f);
+}
+
+// Preetham sky for a zenith sun: brighter above, symmetric about +Y
+HOST_TEST(SceneTest, ProjectEnvironmentPreethamZenith) {
+  Environment env;
+  env.type = Environment::Type::Preetham;
+  env.sun_direction = Vector3f(0, 1, 0);
+  SHCoeffs c = ProjectEnvironmentToSH(env);
+  EXPECT_GT(c.coeffs[0].z(), 0.0f);
+  EXPECT_GT(c.coeffs[1].z(), 0.0f);
+  EXPECT_NEAR(c.coeffs[3].z(), 0.0f, 1e-4f);
+  EXPECT_NEAR(c.coeffs[0].x() / c.coeffs[0].z(), 0.2f / 0.9f, 1e-5f);
+}
+
 // saver_test.cpp: IsEXR checks the 4-byte magic; also the version field
 static bool is_exr(const std::filesystem::path& p) {
   std::ifstream f(p, std::ios::binary);
