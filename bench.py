@@ -290,6 +290,8 @@ def main():
                     help="skip the adjoint / fitting measurement")
     ap.add_argument("--no-env", action="store_true",
                     help="skip the environment-SH measurement")
+    ap.add_argument("--env-only", action="store_true",
+                    help="only the environment-SH measurement (its own JSON line)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device copy of this size (counter calibration)")
     args = ap.parse_args()
@@ -303,6 +305,10 @@ def main():
 
     import shb
     from shb import scenes
+
+    if args.env_only:
+        print(json.dumps({"environment_sh": bench_env(shb, cpu=args.cpu_seconds > 0)}), flush=True)
+        return
 
     t0 = time.time()
     if args.tris == 262144:
