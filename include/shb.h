@@ -286,6 +286,46 @@ int shb_trace_rays(shb_ctx* ctx, const shb_ray* rays, uint32_t n,
                    int32_t any_hit, shb_hit* hits, uint32_t* node_visits,
                    uint32_t* tri_tests);
 
+/* ---------------------------------------------------------- glTF loader */
+/* Environment kinds of a loaded scene (src/scene.h:90-107). */
+enum {
+  SHB_ENV_NONE = 0,      /* no skybox and no directional light           */
+  SHB_ENV_TEXTURE = 1,   /* extras.skybox: an RGB float LatLong map       */
+  SHB_ENV_PREETHAM = 2   /* Preetham sky towards the brightest sun        */
+};
+
+typedef struct shb_environment_desc {
+  int32_t type;              /* SHB_ENV_*                                   */
+  int32_t width, height;     /* texture size (SHB_ENV_TEXTURE)              */
+  int32_t channels;          /* 3 for a texture, else 0                     */
+  const float* pixels;       /* width * height * 3 floats, or NULL          */
+  float sun_direction[3];    /* towards the sun (SHB_ENV_PREETHAM)          */
+  float sun_intensity;
+  const char* path;          /* absolute path of the skybox file, or NULL   */
+} shb_environment_desc;
+
+/* A glTF scene loaded on the host (owns every array its description
+ * points to). */
+typedef struct shb_gltf shb_gltf;
+
+/* Loads a .gltf / .glb file. Mesh primitives need a material, POSITION,
+ * NORMAL and TEXCOORD_0; TEXCOORD_1 becomes the lightmap UVs (else they are
+ * zero and SHB_GEOM_HAS_LIGHTMAP_UVS is clear); emissive materials
+ * (KHR_materials_emissive_strength) give one area light per primitive;
+ * KHR_lights_punctual lights are placed by their nodes; albedo textures are
+ * RGBA8 (PNG or baseline JPEG) or a 1x1 sRGB texture of baseColorFactor.
+ * On failure *out is NULL and err_out (err_len bytes, may be NULL) holds
+ * the reason. Replaces LoadScene (src/loader.cpp:113-844). */
+int shb_load_gltf(const char* path, shb_gltf** out, char* err_out, size_t err_len);
+/* The scene as a description for shb_upload_scene (valid until free). */
+const shb_scene_desc* shb_gltf_scene(const shb_gltf* scene);
+/* The environment (src/loader.cpp:656-789): skybox texture or sun. */
+int shb_gltf_environment(const shb_gltf* scene, shb_environment_desc* out);
+/* Material name and absolute albedo file path (NULL when none). */
+const char* shb_gltf_material_name(const shb_gltf* scene, uint32_t material);
+const char* shb_gltf_albedo_path(const shb_gltf* scene, uint32_t material);
+void shb_gltf_free(shb_gltf* scene);
+
 /* ------------------------------------------------------------ EXR output */
 /* Save layouts of the reference's SaveMode (src/saver.h:19-24). */
 enum {

@@ -67,6 +67,7 @@ struct Texture {
   uint32_t height = 0;
   uint32_t channels = 0;
   std::vector<uint8_t> pixel_data;
+  std::optional<std::filesystem::path> file_path;  // absolute, when loaded from a file
 };
 
 struct Texture32F {
@@ -74,6 +75,7 @@ struct Texture32F {
   uint32_t height = 0;
   uint32_t channels = 0;
   std::vector<float> pixel_data;
+  std::optional<std::filesystem::path> file_path;
 };
 
 struct Material {
@@ -597,6 +599,89 @@ inline SHCoeffs ProjectEnvironmentToSH(const Environment& env) {
   for (int i = 0; i < 9; ++i)
     for (int k = 0; k < 3; ++k) out.coeffs[i][k] = f[3 * i + k];
   return out;
+}
+
+// ------------------------------------------------------------ loader.h
+// Replaces LoadScene (src/loader.h, src/loader.cpp:113-844) over
+// shb_load_gltf: the same Scene, with the environment's SH projected as the
+// reference does at load (a skybox texture on the GPU of the per-thread
+// context, a Preetham sky on the host). std::nullopt (and a log line) when
+// the file cannot be loaded.
+inline std::optional<Scene> LoadScene(const std::filesystem::path& gltf_file) {
+  shb_gltf* g = nullptr;
+  char err[1024] = {0};
+  if (shb_load_gltf(gltf_file.string().c_str(), &g, err, sizeof(err)) != SHB_OK) {
+    std::fprintf(stderr, "shb_baker: LoadScene(%s): %s\n", gltf_file.string().c_str(), err);
+    return std::nullopt;
+  }
+  const shb_scene_desc& d = *shb_gltf_scene(g);
+  Scene s;
+  for (uint32_t gi = 0; gi < d.num_geometries; ++gi) {
+    const shb_geometry& sg = d.geometries[gi];
+    Geometry geo;
+    for (uint32_t v = sg.vertex_offset; v < sg.vertex_offset + sg.vertex_count; ++v) {
+      geo.vertices.emplace_back(d.positions[3 * v], d.positions[3 * v + 1], d.positions[3 * v + 2]);
+      geo.normals.emplace_back(d.normals[3 * v], d.normals[3 * v + 1], d.normals[3 * v + 2]);
+      geo.texture_uvs.emplace_back(d.texture_uvs[2 * v], d.texture_uvs[2 * v + 1]);
+      if (sg.flags & SHB_GEOM_HAS_LIGHTMAP_UVS)
+        geo.lightmap_uvs.emplace_back(d.lightmap_uvs[2 * v], d.lightmap_uvs[2 * v + 1]);
+      geo.tangents.emplace_back(d.tangents[4 * v], d.tangents[4 * v + 1], d.tangents[4 * v + 2],
+                                d.tangents[4 * v + 3]);
+    }
+    geo.indices.assign(d.indices + sg.index_offset, d.indices + sg.index_offset + sg.index_count);
+    geo.material_id = sg.material_id;
+    std::memcpy(geo.transform.m.data(), sg.transform, sizeof(sg.transform));
+    s.geometries.push_back(std::move(geo));
+  }
+  for (uint32_t mi = 0; mi < d.num_materials; ++mi) {
+    const shb_material& sm = d.materials[mi];
+    Material m;
+    const char* name = shb_gltf_material_name(g, mi);
+    m.name = name ? name : "";
+    m.albedo.width = sm.albedo_width;
+    m.albedo.height = sm.albedo_height;
+    m.albedo.channels = sm.albedo_channels;
+    const size_t n = (size_t)sm.albedo_width * sm.albedo_height * sm.albedo_channels;
+    m.albedo.pixel_data.assign(d.texel_pool + sm.albedo_offset, d.texel_pool + sm.albedo_offset + n);
+    if (const char* ap = shb_gltf_albedo_path(g, mi)) m.albedo.file_path = std::filesystem::path(ap);
+    m.emission_intensity = sm.emission_intensity;
+    s.materials.push_back(std::move(m));
+  }
+  for (uint32_t li = 0; li < d.num_lights; ++li) {
+    const shb_light& sl = d.lights[li];
+    Light l;
+    l.type = static_cast<Light::Type>(sl.type);
+    l.position = Vector3f(sl.position[0], sl.position[1], sl.position[2]);
+    l.direction = Vector3f(sl.direction[0], sl.direction[1], sl.direction[2]);
+    l.color = Vector3f(sl.color[0], sl.color[1], sl.color[2]);
+    l.intensity = sl.intensity;
+    l.cos_inner_cone = sl.cos_inner_cone;
+    l.cos_outer_cone = sl.cos_outer_cone;
+    l.geometry_index = sl.geometry_index;
+    l.material_index = sl.material_index;
+    s.lights.push_back(l);
+  }
+  shb_environment_desc e{};
+  shb_gltf_environment(g, &e);
+  if (e.type != SHB_ENV_NONE) {
+    Environment env;
+    if (e.type == SHB_ENV_TEXTURE) {
+      env.type = Environment::Type::Texture;
+      env.texture.width = (uint32_t)e.width;
+      env.texture.height = (uint32_t)e.height;
+      env.texture.channels = 3;
+      env.texture.pixel_data.assign(e.pixels, e.pixels + (size_t)e.width * e.height * 3);
+      if (e.path) env.texture.file_path = std::filesystem::path(e.path);
+    } else {
+      env.type = Environment::Type::Preetham;
+      env.sun_direction = Vector3f(e.sun_direction[0], e.sun_direction[1], e.sun_direction[2]);
+      env.sun_intensity = e.sun_intensity;
+    }
+    env.sh_coeffs = ProjectEnvironmentToSH(env);
+    s.environment = std::move(env);
+  }
+  shb_gltf_free(g);
+  return s;
 }
 
 // ------------------------------------------------------------ occlusion.h

@@ -66,6 +66,15 @@ class ShbSceneDesc(C.Structure):
                 ("num_lights", C.c_uint32), ("lights", C.POINTER(ShbLight))]
 
 
+class ShbEnvironmentDesc(C.Structure):
+    _fields_ = [("type", C.c_int32), ("width", C.c_int32), ("height", C.c_int32),
+                ("channels", C.c_int32), ("pixels", _fp), ("sun_direction", C.c_float * 3),
+                ("sun_intensity", C.c_float), ("path", C.c_char_p)]
+
+
+ENV_NONE, ENV_TEXTURE, ENV_PREETHAM = 0, 1, 2
+
+
 class ShbRasterCfg(C.Structure):
     _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("supersample_scale", C.c_int32)]
 
@@ -110,6 +119,8 @@ EXPORTED = [
     "shb_create", "shb_destroy", "shb_last_error", "shb_version", "shb_upload_scene",
     "shb_rasterize", "shb_set_surface", "shb_validity_mask", "shb_bake", "shb_bake_lightmap", "shb_read_lightmap", "shb_save_lightmap",
     "shb_env_sh_latlong", "shb_env_sh_preetham",
+    "shb_load_gltf", "shb_gltf_scene", "shb_gltf_environment", "shb_gltf_material_name",
+    "shb_gltf_albedo_path", "shb_gltf_free",
     "shb_downsample", "shb_downsample_mask", "shb_dilate", "shb_trace_rays",
     "shb_bake_batched", "shb_autotune", "shb_set_tuning", "shb_get_tuning", "shb_set_params",
     "shb_bake_adjoint",
@@ -142,6 +153,15 @@ def _load():
     lib.shb_env_sh_latlong.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, _fp, _fp,
                                        C.POINTER(C.c_double)]
     lib.shb_env_sh_preetham.argtypes = [_fp, _fp]
+    lib.shb_load_gltf.argtypes = [C.c_char_p, C.POINTER(vp), C.c_char_p, C.c_size_t]
+    lib.shb_gltf_scene.restype = C.POINTER(ShbSceneDesc)
+    lib.shb_gltf_scene.argtypes = [vp]
+    lib.shb_gltf_environment.argtypes = [vp, C.POINTER(ShbEnvironmentDesc)]
+    lib.shb_gltf_material_name.restype = C.c_char_p
+    lib.shb_gltf_material_name.argtypes = [vp, C.c_uint32]
+    lib.shb_gltf_albedo_path.restype = C.c_char_p
+    lib.shb_gltf_albedo_path.argtypes = [vp, C.c_uint32]
+    lib.shb_gltf_free.argtypes = [vp]
     lib.shb_downsample.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _fp, _fp]
     lib.shb_downsample_mask.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, u8p, u8p]
     lib.shb_dilate.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, _fp, u8p, C.c_int32]
@@ -421,3 +441,72 @@ def save_lightmap(path, sh, vis, width, height, mode=SAVE_COMBINED):
     rc = lib.shb_save_lightmap(str(path).encode(), width, height, _f(sh), _f(vis), mode, err, 512)
     if rc != SHB_OK:
         raise ShbError(rc, err.value.decode())
+
+
+def load_gltf(path):
+    """Loads a .gltf / .glb with the library's loader. Returns (scene, env):
+    a shb.scenes.Scene (flat arrays, materials carry 'name' and
+    'albedo_path') and a dict describing the environment ('type' ENV_*,
+    'pixels' (h, w, 3) for a skybox texture, 'sun_direction',
+    'sun_intensity', 'path')."""
+    from . import scenes
+    get_lib()
+    h = C.c_void_p()
+    err = C.create_string_buffer(1024)
+    rc = lib.shb_load_gltf(str(path).encode(), C.byref(h), err, 1024)
+    if rc != SHB_OK:
+        raise ShbError(rc, err.value.decode(errors="replace"))
+    try:
+        d = lib.shb_gltf_scene(h).contents
+        nv, ni = d.num_vertices, d.num_indices
+
+        def arr(ptr, n, dt, shape):
+            if n == 0 or not ptr:
+                return np.zeros(shape, dt)
+            return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dt).reshape(shape)
+
+        s = scenes.Scene()
+        s.positions = arr(d.positions, nv * 3, np.float32, (nv, 3))
+        s.normals = arr(d.normals, nv * 3, np.float32, (nv, 3))
+        s.texture_uvs = arr(d.texture_uvs, nv * 2, np.float32, (nv, 2))
+        s.lightmap_uvs = arr(d.lightmap_uvs, nv * 2, np.float32, (nv, 2))
+        s.tangents = arr(d.tangents, nv * 4, np.float32, (nv, 4))
+        s.indices = arr(d.indices, ni, np.uint32, (ni,))
+        s.geometries = []
+        for i in range(d.num_geometries):
+            g = d.geometries[i]
+            s.geometries.append(dict(vertex_offset=g.vertex_offset, vertex_count=g.vertex_count,
+                                     index_offset=g.index_offset, index_count=g.index_count,
+                                     material_id=g.material_id, flags=g.flags,
+                                     transform=tuple(g.transform)))
+        s.materials = []
+        for i in range(d.num_materials):
+            m = d.materials[i]
+            name = lib.shb_gltf_material_name(h, i)
+            ap = lib.shb_gltf_albedo_path(h, i)
+            s.materials.append(dict(width=m.albedo_width, height=m.albedo_height,
+                                    channels=m.albedo_channels, offset=m.albedo_offset,
+                                    emission=m.emission_intensity,
+                                    name=name.decode() if name is not None else "",
+                                    albedo_path=ap.decode() if ap is not None else None))
+        s.texels = arr(d.texel_pool, d.texel_pool_size, np.uint8, (d.texel_pool_size,))
+        s.lights = []
+        for i in range(d.num_lights):
+            l = d.lights[i]
+            s.lights.append(dict(type=l.type, position=tuple(l.position),
+                                 direction=tuple(l.direction), color=tuple(l.color),
+                                 intensity=l.intensity, cos_inner_cone=l.cos_inner_cone,
+                                 cos_outer_cone=l.cos_outer_cone,
+                                 geometry_index=l.geometry_index,
+                                 material_index=l.material_index))
+        e = ShbEnvironmentDesc()
+        lib.shb_gltf_environment(h, C.byref(e))
+        env = {"type": e.type, "sun_direction": tuple(e.sun_direction),
+               "sun_intensity": e.sun_intensity,
+               "path": e.path.decode() if e.path else None, "pixels": None}
+        if e.type == ENV_TEXTURE and e.pixels:
+            env["pixels"] = np.ctypeslib.as_array(
+                e.pixels, shape=(e.width * e.height * 3,)).copy().reshape(e.height, e.width, 3)
+        return s, env
+    finally:
+        lib.shb_gltf_free(h)

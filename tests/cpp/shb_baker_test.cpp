@@ -295,6 +295,63 @@ TEST(DilationTest, AverageNeighbors) {
   EXPECT_NEAR(pixels[center].coeffs[0].z(), 0.5f, 0.001f);
 }
 
+// the reference's glTF fixtures, copied under tests/golden/gltf
+static std::filesystem::path gltf_dir() {
+  char buf[4096] = {0};
+  ssize_t n = readlink("/proc/self/exe", buf, sizeof(buf) - 1);
+  std::filesystem::path exe = n > 0 ? std::filesystem::path(std::string(buf, n)) : "";
+  return exe.parent_path().parent_path() / "golden" / "gltf";
+}
+
+// loader_test.cpp:9-50
+HOST_TEST(LoaderTest, LoadCube) {
+  auto input_path = gltf_dir() / "cube" / "Cube.gltf";
+  ASSERT_TRUE(std::filesystem::exists(input_path));
+  std::optional<Scene> scene = LoadScene(input_path);
+  ASSERT_TRUE(scene.has_value());
+  ASSERT_EQ(scene->geometries.size(), 1u);
+  const auto& geo = scene->geometries[0];
+  EXPECT_EQ(geo.indices.size(), 36u);
+  EXPECT_GT(geo.vertices.size(), 0u);
+  EXPECT_EQ(geo.vertices.size(), geo.normals.size());
+  EXPECT_EQ(geo.vertices.size(), geo.texture_uvs.size());
+  ASSERT_EQ(scene->materials.size(), 1u);
+  EXPECT_TRUE(scene->materials[0].name == "Cube");
+  EXPECT_GT(scene->materials[0].albedo.width, 0u);
+  EXPECT_GT(scene->materials[0].albedo.height, 0u);
+  ASSERT_TRUE(scene->materials[0].albedo.file_path.has_value());
+  EXPECT_TRUE(scene->materials[0].albedo.file_path->is_absolute());
+}
+
+// loader_test.cpp:52-55
+HOST_TEST(LoaderTest, MissingFile) {
+  std::optional<Scene> scene = LoadScene("non_existent.gltf");
+  EXPECT_FALSE(scene.has_value());
+}
+
+// loader_test.cpp:57-82
+HOST_TEST(LoaderTest, LoadBoxFallbackColor) {
+  auto input_path = gltf_dir() / "box" / "scene.gltf";
+  ASSERT_TRUE(std::filesystem::exists(input_path));
+  std::optional<Scene> scene = LoadScene(input_path);
+  ASSERT_TRUE(scene.has_value());
+  ASSERT_EQ(scene->materials.size(), 1u);
+  const auto& mat = scene->materials[0];
+  EXPECT_TRUE(mat.name == "Red");
+  EXPECT_EQ(mat.albedo.width, 1u);
+  EXPECT_EQ(mat.albedo.height, 1u);
+  EXPECT_EQ(mat.albedo.channels, 4u);
+  ASSERT_EQ(mat.albedo.pixel_data.size(), 4u);
+  EXPECT_NEAR(mat.albedo.pixel_data[0], 231, 1);
+  EXPECT_EQ(mat.albedo.pixel_data[1], 0);
+  EXPECT_EQ(mat.albedo.pixel_data[2], 0);
+  EXPECT_EQ(mat.albedo.pixel_data[3], 255);
+  // the box's sun gives a Preetham environment (loader.cpp:750-789)
+  ASSERT_TRUE(scene->environment.has_value());
+  EXPECT_TRUE(scene->environment->type == Environment::Type::Preetham);
+  EXPECT_GT(scene->environment->sh_coeffs.coeffs[0].z(), 0.0f);
+}
+
 // scene_test.cpp:96-126
 TEST(SceneTest, ProjectEnvironmentUniformWhite) {
   Texture32F tex;
