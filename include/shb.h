@@ -326,6 +326,36 @@ const char* shb_gltf_material_name(const shb_gltf* scene, uint32_t material);
 const char* shb_gltf_albedo_path(const shb_gltf* scene, uint32_t material);
 void shb_gltf_free(shb_gltf* scene);
 
+/* ---------------------------------------------------------------- atlas */
+/* Lightmap UVs for a scene without them. */
+typedef struct shb_atlas shb_atlas;
+
+/* Per-geometry relative scale: density_multiplier * sqrt(albedo width *
+ * height) * sqrt(texture-UV tile count), clamped to 5x the median; out
+ * holds num_geometries floats. Replaces atlas_internal::
+ * CalculateGeometryScales (src/atlas.cpp:22-95). */
+int shb_atlas_geometry_scales(const shb_scene_desc* scene, float density_multiplier,
+                              float* out);
+
+/* Charts the scaled world-space meshes (faces grown while normals stay
+ * within 25 degrees; orthographic projection, minimum-area rectangle),
+ * packs them into one resolution^2 atlas with `padding` texels between
+ * charts at the largest density that fits, and returns the scene with
+ * vertices split per chart and SHB_GEOM_HAS_LIGHTMAP_UVS set (materials,
+ * lights, transforms and face order kept). SHB_ERR_LIMIT when the charts
+ * cannot fit. Replaces CreateAtlasGeometries (src/atlas.cpp:97-250), whose
+ * charting and packing come from xatlas. */
+int shb_atlas_create(const shb_scene_desc* scene, uint32_t resolution, uint32_t padding,
+                     float density_multiplier, shb_atlas** out, char* err_out,
+                     size_t err_len);
+const shb_scene_desc* shb_atlas_scene(const shb_atlas* atlas);
+/* Atlas size, chart count, texels per scaled world unit, and per output
+ * vertex the geometry-local index of its source vertex and its chart
+ * (any may be NULL). */
+int shb_atlas_info(const shb_atlas* atlas, int32_t* width, int32_t* height, uint32_t* charts,
+                   float* texels_per_unit, const uint32_t** xref, const uint32_t** chart);
+void shb_atlas_free(shb_atlas* atlas);
+
 /* ------------------------------------------------------------ EXR output */
 /* Save layouts of the reference's SaveMode (src/saver.h:19-24). */
 enum {

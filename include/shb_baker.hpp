@@ -684,6 +684,98 @@ inline std::optional<Scene> LoadScene(const std::filesystem::path& gltf_file) {
   return s;
 }
 
+// ------------------------------------------------------------- atlas.h
+namespace atlas_internal {
+// Replaces atlas_internal::CalculateGeometryScales (src/atlas.h,
+// src/atlas.cpp:22-95).
+inline std::vector<float> CalculateGeometryScales(const std::vector<Geometry>& geometries,
+                                                  const std::vector<Material>& materials,
+                                                  float density_multiplier) {
+  // only texture UVs and albedo sizes matter: a description of just those
+  std::vector<float> pos, uv;
+  std::vector<shb_geometry> geos;
+  for (const Geometry& g : geometries) {
+    shb_geometry d{};
+    d.vertex_offset = (uint32_t)(uv.size() / 2);
+    d.vertex_count = (uint32_t)g.texture_uvs.size();
+    d.material_id = g.material_id;
+    d.flags = SHB_GEOM_HAS_TEXTURE_UVS;
+    for (const Vector2f& t : g.texture_uvs) {
+      uv.push_back(t.x());
+      uv.push_back(t.y());
+      pos.insert(pos.end(), 3, 0.0f);
+    }
+    geos.push_back(d);
+  }
+  std::vector<shb_material> mats;
+  for (const Material& m : materials) {
+    shb_material d{};
+    d.albedo_width = m.albedo.width;
+    d.albedo_height = m.albedo.height;
+    mats.push_back(d);
+  }
+  shb_scene_desc desc{};
+  desc.num_vertices = (uint32_t)(pos.size() / 3);
+  desc.positions = pos.data();
+  desc.texture_uvs = uv.data();
+  desc.num_geometries = (uint32_t)geos.size();
+  desc.geometries = geos.data();
+  desc.num_materials = (uint32_t)mats.size();
+  desc.materials = mats.data();
+  std::vector<float> out(geometries.size());
+  if (shb_atlas_geometry_scales(&desc, density_multiplier, out.data()) != SHB_OK) {
+    std::fprintf(stderr, "shb_baker: CalculateGeometryScales: bad materials or texture UVs\n");
+    return {};
+  }
+  return out;
+}
+}  // namespace atlas_internal
+
+struct AtlasResult {
+  std::vector<Geometry> geometries;
+  int width = 0;
+  int height = 0;
+};
+
+// Replaces CreateAtlasGeometries (src/atlas.h, src/atlas.cpp:97-250): the
+// geometries with split vertices and lightmap UVs in one
+// target_resolution^2 atlas; std::nullopt when the charts cannot fit.
+inline std::optional<AtlasResult> CreateAtlasGeometries(const Scene& scene,
+                                                        unsigned target_resolution,
+                                                        unsigned padding,
+                                                        float density_multiplier = 1.0f) {
+  if (scene.geometries.empty()) return std::nullopt;
+  detail::FlatScene f(scene);
+  shb_atlas* a = nullptr;
+  char err[1024] = {0};
+  if (shb_atlas_create(&f.desc, target_resolution, padding, density_multiplier, &a, err,
+                       sizeof(err)) != SHB_OK) {
+    std::fprintf(stderr, "shb_baker: CreateAtlasGeometries: %s\n", err);
+    return std::nullopt;
+  }
+  const shb_scene_desc& d = *shb_atlas_scene(a);
+  AtlasResult r;
+  shb_atlas_info(a, &r.width, &r.height, nullptr, nullptr, nullptr, nullptr);
+  for (uint32_t gi = 0; gi < d.num_geometries; ++gi) {
+    const shb_geometry& sg = d.geometries[gi];
+    Geometry g;
+    for (uint32_t v = sg.vertex_offset; v < sg.vertex_offset + sg.vertex_count; ++v) {
+      g.vertices.emplace_back(d.positions[3 * v], d.positions[3 * v + 1], d.positions[3 * v + 2]);
+      g.normals.emplace_back(d.normals[3 * v], d.normals[3 * v + 1], d.normals[3 * v + 2]);
+      g.texture_uvs.emplace_back(d.texture_uvs[2 * v], d.texture_uvs[2 * v + 1]);
+      g.lightmap_uvs.emplace_back(d.lightmap_uvs[2 * v], d.lightmap_uvs[2 * v + 1]);
+      g.tangents.emplace_back(d.tangents[4 * v], d.tangents[4 * v + 1], d.tangents[4 * v + 2],
+                              d.tangents[4 * v + 3]);
+    }
+    g.indices.assign(d.indices + sg.index_offset, d.indices + sg.index_offset + sg.index_count);
+    g.material_id = sg.material_id;
+    std::memcpy(g.transform.m.data(), sg.transform, sizeof(sg.transform));
+    r.geometries.push_back(std::move(g));
+  }
+  shb_atlas_free(a);
+  return r;
+}
+
 // ------------------------------------------------------------ occlusion.h
 // An uploaded scene with its BVH: the counterpart of the RTCScene that
 // BuildBVH (src/scene.cpp:209-262) returns. Owns its own GPU context.

@@ -120,7 +120,8 @@ EXPORTED = [
     "shb_rasterize", "shb_set_surface", "shb_validity_mask", "shb_bake", "shb_bake_lightmap", "shb_read_lightmap", "shb_save_lightmap",
     "shb_env_sh_latlong", "shb_env_sh_preetham",
     "shb_load_gltf", "shb_gltf_scene", "shb_gltf_environment", "shb_gltf_material_name",
-    "shb_gltf_albedo_path", "shb_gltf_free",
+    "shb_gltf_albedo_path", "shb_gltf_free", "shb_atlas_geometry_scales", "shb_atlas_create",
+    "shb_atlas_scene", "shb_atlas_info", "shb_atlas_free",
     "shb_downsample", "shb_downsample_mask", "shb_dilate", "shb_trace_rays",
     "shb_bake_batched", "shb_autotune", "shb_set_tuning", "shb_get_tuning", "shb_set_params",
     "shb_bake_adjoint",
@@ -162,6 +163,16 @@ def _load():
     lib.shb_gltf_albedo_path.restype = C.c_char_p
     lib.shb_gltf_albedo_path.argtypes = [vp, C.c_uint32]
     lib.shb_gltf_free.argtypes = [vp]
+    lib.shb_atlas_geometry_scales.argtypes = [C.POINTER(ShbSceneDesc), C.c_float, _fp]
+    lib.shb_atlas_create.argtypes = [C.POINTER(ShbSceneDesc), C.c_uint32, C.c_uint32, C.c_float,
+                                     C.POINTER(vp), C.c_char_p, C.c_size_t]
+    lib.shb_atlas_scene.restype = C.POINTER(ShbSceneDesc)
+    lib.shb_atlas_scene.argtypes = [vp]
+    lib.shb_atlas_info.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                   C.POINTER(C.c_uint32), _fp,
+                                   C.POINTER(C.POINTER(C.c_uint32)),
+                                   C.POINTER(C.POINTER(C.c_uint32))]
+    lib.shb_atlas_free.argtypes = [vp]
     lib.shb_downsample.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _fp, _fp]
     lib.shb_downsample_mask.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, u8p, u8p]
     lib.shb_dilate.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, _fp, u8p, C.c_int32]
@@ -457,14 +468,32 @@ def load_gltf(path):
     if rc != SHB_OK:
         raise ShbError(rc, err.value.decode(errors="replace"))
     try:
-        d = lib.shb_gltf_scene(h).contents
-        nv, ni = d.num_vertices, d.num_indices
+        s = _scene_from_desc(lib.shb_gltf_scene(h).contents, h)
+        e = ShbEnvironmentDesc()
+        lib.shb_gltf_environment(h, C.byref(e))
+        env = {"type": e.type, "sun_direction": tuple(e.sun_direction),
+               "sun_intensity": e.sun_intensity,
+               "path": e.path.decode() if e.path else None, "pixels": None}
+        if e.type == ENV_TEXTURE and e.pixels:
+            env["pixels"] = np.ctypeslib.as_array(
+                e.pixels, shape=(e.width * e.height * 3,)).copy().reshape(e.height, e.width, 3)
+        return s, env
+    finally:
+        lib.shb_gltf_free(h)
 
-        def arr(ptr, n, dt, shape):
-            if n == 0 or not ptr:
-                return np.zeros(shape, dt)
-            return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dt).reshape(shape)
 
+def _scene_from_desc(d, gltf_handle=None):
+    """A shb.scenes.Scene copied from a shb_scene_desc."""
+    from . import scenes
+    nv, ni = d.num_vertices, d.num_indices
+
+    def arr(ptr, n, dt, shape):
+        if n == 0 or not ptr:
+            return np.zeros(shape, dt)
+        return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dt).reshape(shape)
+
+    if True:
+        h = gltf_handle
         s = scenes.Scene()
         s.positions = arr(d.positions, nv * 3, np.float32, (nv, 3))
         s.normals = arr(d.normals, nv * 3, np.float32, (nv, 3))
@@ -482,8 +511,8 @@ def load_gltf(path):
         s.materials = []
         for i in range(d.num_materials):
             m = d.materials[i]
-            name = lib.shb_gltf_material_name(h, i)
-            ap = lib.shb_gltf_albedo_path(h, i)
+            name = lib.shb_gltf_material_name(h, i) if h else None
+            ap = lib.shb_gltf_albedo_path(h, i) if h else None
             s.materials.append(dict(width=m.albedo_width, height=m.albedo_height,
                                     channels=m.albedo_channels, offset=m.albedo_offset,
                                     emission=m.emission_intensity,
@@ -499,14 +528,51 @@ def load_gltf(path):
                                  cos_outer_cone=l.cos_outer_cone,
                                  geometry_index=l.geometry_index,
                                  material_index=l.material_index))
-        e = ShbEnvironmentDesc()
-        lib.shb_gltf_environment(h, C.byref(e))
-        env = {"type": e.type, "sun_direction": tuple(e.sun_direction),
-               "sun_intensity": e.sun_intensity,
-               "path": e.path.decode() if e.path else None, "pixels": None}
-        if e.type == ENV_TEXTURE and e.pixels:
-            env["pixels"] = np.ctypeslib.as_array(
-                e.pixels, shape=(e.width * e.height * 3,)).copy().reshape(e.height, e.width, 3)
-        return s, env
+        return s
+
+
+def geometry_scales(scene, density_multiplier=1.0):
+    """Per-geometry atlas scales (CalculateGeometryScales)."""
+    get_lib()
+    desc, keep = scene.to_desc()
+    out = np.zeros(max(len(scene.geometries), 1), np.float32)
+    rc = lib.shb_atlas_geometry_scales(C.byref(desc), density_multiplier, _f(out))
+    if rc != SHB_OK:
+        raise ShbError(rc, "bad scene for atlas scales")
+    del keep
+    return out[:len(scene.geometries)]
+
+
+def create_atlas(scene, resolution, padding, density_multiplier=1.0):
+    """Lightmap atlas for `scene`: returns (scene with lightmap UVs and
+    split vertices, info dict with width, height, charts, texels_per_unit,
+    and per output vertex xref (the geometry-local source vertex) and
+    chart)."""
+    get_lib()
+    desc, keep = scene.to_desc()
+    h = C.c_void_p()
+    err = C.create_string_buffer(1024)
+    rc = lib.shb_atlas_create(C.byref(desc), resolution, padding, density_multiplier,
+                              C.byref(h), err, 1024)
+    del keep
+    if rc != SHB_OK:
+        raise ShbError(rc, err.value.decode(errors="replace"))
+    try:
+        out = _scene_from_desc(lib.shb_atlas_scene(h).contents)
+        for m_out, m_in in zip(out.materials, scene.materials):
+            for k in ("name", "albedo_path"):
+                if k in m_in:
+                    m_out[k] = m_in[k]
+        w, hh, nch = C.c_int32(), C.c_int32(), C.c_uint32()
+        tpu = C.c_float()
+        xr = C.POINTER(C.c_uint32)()
+        co = C.POINTER(C.c_uint32)()
+        lib.shb_atlas_info(h, C.byref(w), C.byref(hh), C.byref(nch), C.byref(tpu), C.byref(xr),
+                           C.byref(co))
+        nvo = out.positions.shape[0]
+        xref = np.ctypeslib.as_array(xr, shape=(max(nvo, 1),)).copy()[:nvo]
+        chart = np.ctypeslib.as_array(co, shape=(max(nvo, 1),)).copy()[:nvo]
+        return out, {"width": w.value, "height": hh.value, "charts": nch.value,
+                     "texels_per_unit": tpu.value, "xref": xref, "chart": chart}
     finally:
-        lib.shb_gltf_free(h)
+        lib.shb_atlas_free(h)

@@ -303,6 +303,75 @@ static std::filesystem::path gltf_dir() {
   return exe.parent_path().parent_path() / "golden" / "gltf";
 }
 
+// atlas_test.cpp:9-65
+HOST_TEST(AtlasTest, SimpleQuad) {
+  Geometry input_geo;
+  input_geo.vertices = {{-1, -1, 0}, {1, -1, 0}, {-1, 1, 0}, {1, 1, 0}};
+  input_geo.normals = {{0, 0, 1}, {0, 0, 1}, {0, 0, 1}, {0, 0, 1}};
+  input_geo.tangents = {{1, 0, 0, 1}, {1, 0, 0, 1}, {1, 0, 0, 1}, {1, 0, 0, 1}};
+  input_geo.texture_uvs = {{0, 0}, {1, 0}, {0, 1}, {1, 1}};
+  input_geo.indices = {0, 1, 2, 2, 1, 3};
+  Scene scene;
+  scene.geometries = {input_geo};
+  Material mat;
+  mat.albedo.width = 256;
+  mat.albedo.height = 256;
+  scene.materials.push_back(mat);
+  scene.geometries[0].material_id = 0;
+  std::optional<AtlasResult> atlas_result = CreateAtlasGeometries(scene, 256, 2);
+  ASSERT_TRUE(atlas_result.has_value());
+  EXPECT_GE(atlas_result->width, 0);
+  EXPECT_GE(atlas_result->height, 0);
+  ASSERT_EQ(atlas_result->geometries.size(), 1u);
+  const auto& out_geo = atlas_result->geometries[0];
+  ASSERT_EQ(out_geo.lightmap_uvs.size(), out_geo.vertices.size());
+  for (const auto& uv : out_geo.lightmap_uvs) {
+    EXPECT_GE(uv.x(), 0.0f);
+    EXPECT_TRUE(uv.x() <= 1.0f);
+    EXPECT_GE(uv.y(), 0.0f);
+    EXPECT_TRUE(uv.y() <= 1.0f);
+  }
+  EXPECT_EQ(out_geo.indices.size(), input_geo.indices.size());
+  EXPECT_GE(out_geo.vertices.size(), input_geo.vertices.size());
+}
+
+// atlas_test.cpp:67-166
+HOST_TEST(AtlasTest, CalculateGeometryScales) {
+  std::vector<Geometry> g(1);
+  std::vector<Material> m(1);
+  g[0].material_id = 0;
+  g[0].texture_uvs = {{0, 0}, {1, 0}, {0, 1}};
+  g[0].vertices = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  m[0].albedo.width = 100;
+  m[0].albedo.height = 100;
+  auto s = atlas_internal::CalculateGeometryScales(g, m, 1.0f);
+  ASSERT_EQ(s.size(), 1u);
+  EXPECT_NEAR(s[0], 100.0f, 1e-4f);  // Basic
+  g[0].texture_uvs = {{0, 0}, {2, 0}, {0, 2}};
+  m[0].albedo.width = m[0].albedo.height = 10;
+  s = atlas_internal::CalculateGeometryScales(g, m, 1.0f);
+  ASSERT_EQ(s.size(), 1u);
+  EXPECT_NEAR(s[0], 20.0f, 1e-4f);  // Tiling
+  std::vector<Geometry> g3(3);
+  std::vector<Material> m3(3);
+  for (int i = 0; i < 3; ++i) {
+    g3[i].material_id = i;
+    g3[i].texture_uvs = {{0, 0}, {1, 1}};
+    g3[i].vertices = {{0, 0, 0}, {0, 0, 0}};
+    m3[i].albedo.width = m3[i].albedo.height = i == 2 ? 1000 : 10;
+  }
+  s = atlas_internal::CalculateGeometryScales(g3, m3, 1.0f);
+  ASSERT_EQ(s.size(), 3u);
+  EXPECT_NEAR(s[0], 10.0f, 1e-4f);  // Clamping
+  EXPECT_NEAR(s[1], 10.0f, 1e-4f);
+  EXPECT_NEAR(s[2], 50.0f, 1e-4f);
+  g[0].texture_uvs = {{0, 0}, {1, 1}};
+  m[0].albedo.width = m[0].albedo.height = 100;
+  s = atlas_internal::CalculateGeometryScales(g, m, 2.0f);
+  ASSERT_EQ(s.size(), 1u);
+  EXPECT_NEAR(s[0], 200.0f, 1e-4f);  // DensityMultiplier
+}
+
 // loader_test.cpp:9-50
 HOST_TEST(LoaderTest, LoadCube) {
   auto input_path = gltf_dir() / "cube" / "Cube.gltf";
