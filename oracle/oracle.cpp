@@ -827,7 +827,7 @@ static V3 TraceRef(const TraceCtx& cx, V3 origin, V3 dir, int depth, RefRng& r) 
 // The stream the HIP kernels consume: every path vertex ("site") seeds a
 // xoshiro128** state from Philox4x32-10(counter, key = seed). Counter:
 // (texel, sample | depth << 16 | branch << 20, site kind, job).
-enum { SITE_TEXEL = 0, SITE_VERTEX = 1 };
+enum { SITE_TEXEL = 0, SITE_VERTEX = 1, SITE_ALPHA = 2 };
 
 static Xoshiro128ss site_rng(uint64_t seed, uint32_t texel, uint32_t sample,
                              uint32_t depth, uint32_t branch, uint32_t kind,
@@ -907,10 +907,13 @@ struct GpuCtx {
   Counters* c;
 };
 
-// Trace with the GPU stream: identical control flow to TraceRef; the vertex
-// at (depth, branch) draws NEE then the material sample from its own site.
-// An alpha split sends the transmitted ray on with branch 2b+1 and the
-// reflected continuation with 2b; opaque hits keep b.
+// Trace with the GPU stream: the control flow of TraceRef except at alpha <
+// 1, where one branch is chosen (transmission with probability 1 - alpha,
+// from the vertex's own alpha site) at weight 1 instead of summing both with
+// weights 1 - alpha and alpha (baker.cpp:97-109): the same expectation.
+// The vertex at (depth, branch) draws NEE then the material sample from its
+// own site; a transmitted ray goes on with branch 2b+1 and the reflected
+// continuation after a choice with 2b; opaque hits keep b.
 static V3 TraceGpu(const GpuCtx& cx, V3 origin, V3 dir, int depth,
                    uint32_t branch) {
   if (depth > cx.max_depth) return V3{};
@@ -924,20 +927,22 @@ static V3 TraceGpu(const GpuCtx& cx, V3 origin, V3 dir, int depth,
   V3 color{};
   uint32_t next_branch = branch;
   if (alpha < 1.0f) {
-    V3 hp = occ.position + dir * 0.001f;
-    V3 tr = TraceGpu(cx, hp, dir, depth + 1, 2 * branch + 1);
-    color += (1.0f - alpha) * tr;
+    Xoshiro128ss ax = site_rng(cx.seed, cx.texel, cx.sample, (uint32_t)depth,
+                               branch, SITE_ALPHA, cx.job);
+    if (ax.uniform() < 1.0f - alpha) {
+      V3 hp = occ.position + dir * 0.001f;
+      return TraceGpu(cx, hp, dir, depth + 1, 2 * branch + 1);
+    }
     next_branch = 2 * branch;
   }
-  if (alpha == 0.0f) return color;
   V3 em = GetEmission(mat, occ.uv);
-  if (!is_zero(em) && depth == 0) color += alpha * em;
+  if (!is_zero(em) && depth == 0) color += em;
   V3 hit_pos = occ.position + occ.normal * 0.001f;
   Xoshiro128ss x = site_rng(cx.seed, cx.texel, cx.sample, (uint32_t)depth,
                             branch, SITE_VERTEX, cx.job);
   V3 Ld = NeeGpu(*cx.s, hit_pos, occ.normal, -dir, &mat, occ.uv, x, false,
                  nullptr, *cx.c);
-  color += alpha * Ld;
+  color += Ld;
   float r1 = x.uniform(), r2 = x.uniform();
   RefSample smp = SampleMaterialUV(occ.normal, r1, r2);
   if (smp.pdf < 1e-3f) return color;
@@ -949,7 +954,7 @@ static V3 TraceGpu(const GpuCtx& cx, V3 origin, V3 dir, int depth,
   V3 incoming = TraceGpu(cx, hit_pos, smp.dir, depth + 1, next_branch);
   float cosine = dot(occ.normal, smp.dir);
   V3 Lind = cmul(incoming, brdf) * (cosine / smp.pdf);
-  color += alpha * Lind;
+  color += Lind;
   return color;
 }
 

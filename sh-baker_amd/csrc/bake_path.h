@@ -148,6 +148,28 @@ __device__ __forceinline__ void shade_path(const BakeArgs& a, const JobDev& J, c
   float4 th4 = a.thr[p];
   f3 thr = f3_of(th4);
   uint32_t branch = as_uint(th4.w);
+  uint32_t next_branch = branch;
+  // alpha < 1 (baker.cpp:97-109): the reference splits the path into the
+  // transmitted ray, weight 1 - alpha, and the reflected one, weight alpha.
+  // Here one branch is chosen, transmission with probability 1 - alpha, at
+  // weight 1: the same expectation, one path per slot. Branch ids follow
+  // the split (transmitted 2b + 1, reflected 2b); alpha 0 always transmits.
+  const float alpha = get_alpha(sc, mat, tu, tv);
+  if (alpha < 1.0f) {
+    Rng ar = site_rng(J.seed, texel, sr.smp, (uint32_t)depth, branch, kSiteAlpha, J.stream);
+    if (ar.uniform() < 1.0f - alpha) {
+      if (ADJ) a.adj_hist[(size_t)depth * a.P + p] = kHistTransmit;
+      if (depth < (int)J.bounces) {
+        const f3 o = pos + dir * 0.001f;
+        a.thr[p] = make_float4(thr.x, thr.y, thr.z, as_float(2u * branch + 1u));
+        a.ray[2 * (size_t)p] = make_float4(o.x, o.y, o.z, kRayEps);
+        a.ray[2 * (size_t)p + 1] = make_float4(dir.x, dir.y, dir.z, INFINITY);
+        extend = true;
+      }
+      return;
+    }
+    next_branch = 2u * branch;
+  }
   f3 add = {0.f, 0.f, 0.f};
   if (ADJ) a.adj_hist[(size_t)depth * a.P + p] = (uint16_t)mat;
   if (depth == 0 && sc.mats[mat].emission > 0.0f) {
@@ -198,7 +220,7 @@ __device__ __forceinline__ void shade_path(const BakeArgs& a, const JobDev& J, c
     float cosine = dot(nrm, sd);
     if (front && cosine > 0.0f && !is_zero(brdf)) {
       f3 tn = cmul(thr, brdf) * (cosine / pdf);
-      a.thr[p] = make_float4(tn.x, tn.y, tn.z, as_float(branch));
+      a.thr[p] = make_float4(tn.x, tn.y, tn.z, as_float(next_branch));
       a.ray[2 * (size_t)p] = make_float4(hit_pos.x, hit_pos.y, hit_pos.z, kRayEps);
       a.ray[2 * (size_t)p + 1] = make_float4(sd.x, sd.y, sd.z, INFINITY);
       extend = true;
@@ -250,6 +272,7 @@ __device__ __forceinline__ void adjoint_vertex(const BakeArgs& a, const JobDev& 
   const uint32_t mbase = 3 * a.adj_nlights;
   for (int k = 0; k <= depth; ++k) {
     const uint32_t m = a.adj_hist[(size_t)k * a.P + p];
+    if (m == kHistTransmit) continue;  // a transmitted vertex has no albedo factor
     const float* as = J.scene.albedo_scale + 3 * m;
     for (int ch = 0; ch < 3; ++ch) acc->add(mbase + 3 * m + ch, G[ch] / as[ch]);
   }

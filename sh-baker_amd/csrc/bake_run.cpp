@@ -24,9 +24,6 @@ int prepare_job_async(JobIn& j, hipStream_t stream, uint32_t* count_dev) {
   if (c.hs.bad_material)
     return fail(c, SHB_ERR_INVALID_ARG,
                 "a geometry's material_id does not name a material (the bake shades every hit)");
-  if (c.hs.has_transparency)
-    return fail(c, SHB_ERR_UNSUPPORTED,
-                "materials with alpha < 1 (path splitting) are not supported by the GPU bake");
   SHB_TRY(c, hipSetDevice(c.device));
   const size_t npx = (size_t)c.width * c.ss * c.height * c.ss;
   hipStream_t st = stream ? stream : c.stream;

@@ -140,7 +140,9 @@ struct GradAcc {
   }
 #endif
 };
-constexpr uint32_t kAdjLdsParams = 8192;  // LDS accumulators up to 32 KB
+constexpr uint32_t kAdjLdsParams = 8192;
+// adj_hist entry of a vertex whose path was transmitted (alpha < 1)
+constexpr uint16_t kHistTransmit = 0xFFFF;  // LDS accumulators up to 32 KB
 
 #ifndef SHB_HOST_ONLY
 // Traversal launches: `capacity` is the largest possible queue length (the

@@ -137,7 +137,7 @@ struct SceneView {
 // Philox4x32-10 (Salmon et al., SC'11) seeds one xoshiro128** state per path
 // vertex ("site"); counter = (texel, sample | depth<<16 | branch<<20, kind,
 // job), key = 64-bit seed.
-enum : uint32_t { kSiteTexel = 0, kSiteVertex = 1 };
+enum : uint32_t { kSiteTexel = 0, kSiteVertex = 1, kSiteAlpha = 2 };
 
 SHB_HD void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
 #pragma unroll

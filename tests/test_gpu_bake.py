@@ -184,11 +184,11 @@ def test_bake_errors(baker):
     fresh.close()
 
 
-def test_transparent_materials_rejected(baker):
+def test_transparent_material_bakes(baker):
+    # alpha < 1 is baked (the stochastic choice of tests/test_gpu_alpha.py)
     s = scenes.quad_scene()
     s.texels = np.array([255, 255, 255, 128], np.uint8)
     baker.upload_scene(s)
     baker.rasterize(4, 4, 1, want=False)
-    with pytest.raises(shb.ShbError) as e:
-        baker.bake(4, 1)
-    assert e.value.code == -7
+    sh, vis, st = baker.bake(4, 1)
+    assert np.isfinite(sh).all() and st["valid_texels"] > 0

@@ -6,6 +6,7 @@ downsample, dilation), FP64 brute force (traversal), closed forms (lights,
 BakeSimpleQuad) and the statistical equivalence of the oracle's two random
 streams. Runs on the CPU."""
 import math
+import os
 
 import numpy as np
 import pytest
@@ -458,3 +459,25 @@ def test_bake_skips_invalid_texels_and_is_deterministic():
     b = o.bake(pts, 24, 24, 4, 2, threads=8)[0]
     assert np.array_equal(a, b)
     assert (a[pts["material_id"] < 0] == 0).all()
+
+
+def test_stochastic_alpha_choice_matches_the_split():
+    """GPU-stream mode picks one branch at alpha < 1 (transmission with
+    probability 1 - alpha, weight 1); the reference mode sums both with
+    weights 1 - alpha and alpha (baker.cpp:97-109). Same expectation: per
+    texel within Monte Carlo error, and no bias in the mean over texels."""
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from test_gpu_alpha import ALPHAS, cutout_room
+    s = cutout_room(48, ALPHAS)
+    o = orc.Oracle(s)
+    pts = o.rasterize(48, 48, 1)
+    v = pts["material_id"] >= 0
+    g_sh, _, g_var, _ = o.bake(pts, 48, 48, 128, 3, mode=orc.MODE_GPU, seed=31)
+    r_sh, _, r_var, _ = o.bake(pts, 48, 48, 128, 3, mode=orc.MODE_REF)
+    live = (r_var[v] + g_var[v]) > 0
+    z = (g_sh[v] - r_sh[v])[live] / np.sqrt(r_var[v] + g_var[v])[live]
+    assert 0.6 < np.abs(z).mean() < 1.0 and (np.abs(z) > 4.5).mean() < 1e-3
+    d = (g_sh[v] - r_sh[v]).mean(axis=0)
+    se = np.sqrt((r_var[v] + g_var[v]).sum(axis=0)) / v.sum()
+    assert (np.abs(d) <= 4 * se + 1e-7).all()
