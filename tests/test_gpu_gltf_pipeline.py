@@ -49,17 +49,20 @@ def test_loaded_scene_raster_and_bake_match_oracle(baker, name, res):
     assert np.isfinite(g_sh).all() and np.abs(g_sh[v]).max() > 0
 
 
-def test_loaded_scene_within_3_sigma_of_reference_stream(baker):
-    s, _ = prepared("DirectionalLight/scene.gltf", 64, 2)
+@pytest.mark.parametrize("name", ["LightsPunctualLamp/LightsPunctualLamp.gltf"])
+def test_loaded_scene_within_3_sigma_of_reference_stream(baker, name):
+    # the lamp has five point lights and a textured shade with alpha < 1
+    s, _ = prepared(name, 96, 2)
     baker.upload_scene(s)
-    pts = baker.rasterize(64, 64, 1)
+    pts = baker.rasterize(96, 96, 1)
     o = orc.Oracle(s)
     spp, B = 64, 3
     g_sh, g_vis, _ = baker.bake(spp, B, seed=99)
-    r_sh, r_vis, r_var, _ = o.bake(pts, 64, 64, spp, B, mode=orc.MODE_REF)
-    _, _, g_var, _ = o.bake(pts, 64, 64, spp, B, mode=orc.MODE_GPU, seed=99)
+    r_sh, r_vis, r_var, _ = o.bake(pts, 96, 96, spp, B, mode=orc.MODE_REF)
+    _, _, g_var, _ = o.bake(pts, 96, 96, spp, B, mode=orc.MODE_GPU, seed=99)
     v = pts["material_id"] >= 0
     live = (r_var[v] + g_var[v]) > 0
+    assert live.sum() > 100
     z = (g_sh[v] - r_sh[v])[live] / np.sqrt(r_var[v] + g_var[v])[live]
     assert 0.6 < np.abs(z).mean() < 1.0, np.abs(z).mean()
     assert (np.abs(z) > 4.5).mean() < 1e-3
