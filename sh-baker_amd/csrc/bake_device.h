@@ -136,123 +136,110 @@ __device__ __forceinline__ bool nee_sample(const SceneView& sc, f3 P, f3 N, f3 b
   float pw = 0.f, pap = 1.f, ptfar = 0.f;
   f3 prad = {0.f, 0.f, 0.f}, pdir = {0.f, 0.f, 0.f};
   const int nl = sc.num_lights;
-  // |P| bound for the conservative culling tests
+  // |P| bound for the conservative culling test below
   const float pmax = fmaxf(fabsf(P.x), fmaxf(fabsf(P.y), fabsf(P.z)));
-  for (int base = 0; base < nl; base += 64) {
-    const int cnt = min(64, nl - base);
-    // pass 1 (wave-uniform lights, scalar loads): this lane's lights whose
-    // weight can be non-zero. Skipped: a directional light with cos_n < 0, a
-    // point / spot light or single-triangle emitter lying below the tangent
-    // plane by more than the float error of the sampled cosine, an emitter
-    // without triangles. Every skipped light has weight 0 at this point.
-    uint64_t live = 0;
-    for (int j = 0; j < cnt; ++j) {
-      const LightDev L = uniform_load(sc.lights, (uint32_t)(base + j));
-      bool keep = true;
-      if (L.type == kLightDirectional) {
-        keep = !(dot(N, -f3{L.dir[0], L.dir[1], L.dir[2]}) < 0.f);
-      } else if (L.type == kLightArea) {
+  for (int l = 0; l < nl; ++l) {
+    // every lane visits the same light: a wave-uniform index lets the light
+    // record come through the scalar cache
+    const LightDev L = uniform_load(sc.lights, (uint32_t)l);
+    f3 rad = {0.f, 0.f, 0.f}, sdir;
+    float tfar, ap = 1.0f, cos_n = 0.f;
+    const int type = L.type;
+    if (type == kLightDirectional) {
+      f3 ld = {L.dir[0], L.dir[1], L.dir[2]};
+      sdir = -ld;
+      tfar = 1.0e10f;
+      cos_n = dot(N, -ld);
+      if (!(cos_n < 0.f)) rad = f3{L.ic[0], L.ic[1], L.ic[2]};
+    } else {
+      f3 target;
+      f3 emit = {0.f, 0.f, 0.f};
+      bool area = type == kLightArea;
+      if (area) {
         if (L.num_tris == 0) {
-          keep = false;
-        } else if (L.num_tris == 1) {
-          const AreaTri at = uniform_load(sc.area_tris, L.first_tri);
-          const float d0 = dot(N, f3{at.v0[0], at.v0[1], at.v0[2]} - P);
-          const float d1 = dot(N, f3{at.v1[0], at.v1[1], at.v1[2]} - P);
-          const float d2 = dot(N, f3{at.v2[0], at.v2[1], at.v2[2]} - P);
-          keep = !(fmaxf(d0, fmaxf(d1, d2)) < -1e-5f * (1.0f + pmax + at.vmax));
-        }
-      } else {
-        const f3 lp = {L.pos[0], L.pos[1], L.pos[2]};
-        const float lmax = fmaxf(fabsf(lp.x), fmaxf(fabsf(lp.y), fabsf(lp.z)));
-        keep = !(dot(N, lp - P) < -1e-5f * (1.0f + pmax + lmax));
-      }
-      if (keep) live |= 1ull << j;
-    }
-    // pass 2: this lane's surviving lights in index order (per-lane light
-    // records; the reservoir sees the same sequence of non-zero weights)
-    while (live) {
-      const int j = __builtin_ctzll(live);
-      live &= live - 1;
-      const uint32_t l = (uint32_t)(base + j);
-      const LightDev L = sc.lights[l];
-      f3 rad = {0.f, 0.f, 0.f}, sdir;
-      float tfar, ap = 1.0f, cos_n = 0.f;
-      const int type = L.type;
-      if (type == kLightDirectional) {
-        f3 ld = {L.dir[0], L.dir[1], L.dir[2]};
-        sdir = -ld;
-        tfar = 1.0e10f;
-        cos_n = dot(N, -ld);
-        if (!(cos_n < 0.f)) rad = f3{L.ic[0], L.ic[1], L.ic[2]};
-      } else {
-        f3 target;
-        f3 emit = {0.f, 0.f, 0.f};
-        if (type == kLightArea) {
-          float u1 = word_uniform(light_word(lk.k1, l));
-          float u2 = word_uniform(light_word(lk.k2, l));
+          ap = 0.f;
+          target = f3{0.f, 0.f, 0.f};
+        } else {
+          float u1 = word_uniform(light_word(lk.k1, (uint32_t)l));
+          float u2 = word_uniform(light_word(lk.k2, (uint32_t)l));
           if (u1 + u2 > 1.0f) {
             u1 = 1.0f - u1;
             u2 = 1.0f - u2;
           }
           float tu = 0.f, tv = 0.f;
-          const uint32_t tri =
-              L.num_tris == 1 ? 0u : word_index(light_word(lk.k0, l), L.num_tris);
-          area_point(sc.area_tris[L.first_tri + tri], u1, u2, target, ap, tu, tv);
+          if (L.num_tris == 1) {
+            // single-triangle emitters read a wave-uniform record; when all
+            // three vertices lie below the tangent plane by more than the
+            // rounding of the sampled point's cosine, every sample has
+            // cos_n < 0 and weight 0: the light is skipped (no other light's
+            // numbers depend on it)
+            const AreaTri at = uniform_load(sc.area_tris, L.first_tri);
+            const float d0 = dot(N, f3{at.v0[0], at.v0[1], at.v0[2]} - P);
+            const float d1 = dot(N, f3{at.v1[0], at.v1[1], at.v1[2]} - P);
+            const float d2 = dot(N, f3{at.v2[0], at.v2[1], at.v2[2]} - P);
+            const float tol = 1e-5f * (1.0f + pmax + at.vmax);
+            if (fmaxf(d0, fmaxf(d1, d2)) < -tol) continue;
+            area_point(at, u1, u2, target, ap, tu, tv);
+          } else {
+            const uint32_t tri = word_index(light_word(lk.k0, (uint32_t)l), L.num_tris);
+            area_point(sc.area_tris[L.first_tri + tri], u1, u2, target, ap, tu, tv);
+          }
           if (L.emit_const)
             emit = f3{L.emit[0], L.emit[1], L.emit[2]};
           else
             emit = get_emission(sc, L.material, tu, tv);
-        } else {
-          target = f3{L.pos[0], L.pos[1], L.pos[2]};
         }
-        f3 Lv = target - P;
-        float dist_sq = sqnorm(Lv);
-        float dist = sqrtf(dist_sq);
-        sdir = target - P;
-        tfar = dist - 0.001f;
-        // one division per light (the reference divides per component:
-        // light.h:82-93, 127-143, 176-188); the oracle's GPU-stream mode
-        // follows this arithmetic
-        const float inv = 1.0f / dist;
-        const float inv2 = inv * inv;
-        Lv = Lv * inv;
-        cos_n = dot(N, Lv);
-        if (type == kLightSpot) {
-          f3 ld = {L.dir[0], L.dir[1], L.dir[2]};
-          float cos_l = dot(ld, -Lv);
-          float falloff = (cos_l - L.cos_outer) / (L.cos_inner - L.cos_outer);
-          falloff = fminf(fmaxf(falloff, 0.0f), 1.0f);
-          if (!(cos_n < 0.f)) rad = f3{L.ic[0], L.ic[1], L.ic[2]} * inv2 * falloff;
-        } else if (type == kLightPoint) {
-          if (!(cos_n < 0.f)) rad = f3{L.ic[0], L.ic[1], L.ic[2]} * inv2;
-        } else {
-          if (!(cos_n < 0.f)) rad = emit * inv2;
-        }
+      } else {
+        target = f3{L.pos[0], L.pos[1], L.pos[2]};
       }
-      // BRDF * cosine at a path vertex: scaled albedo for the contribution,
-      // unscaled for the selection weight
-      f3 radw = rad;
-      if (VERTEX && !(cos_n < 0.f)) {
-        bool ok = cos_n > 0.0f;
-        f3 bv = ok ? brdf : f3{0.f, 0.f, 0.f};
-        f3 bw = ok ? brdf_w : f3{0.f, 0.f, 0.f};
-        radw = cmul(rad, cos_n * bw);
-        rad = cmul(rad, cos_n * bv);
+      f3 Lv = target - P;
+      float dist_sq = sqnorm(Lv);
+      float dist = sqrtf(dist_sq);
+      sdir = target - P;
+      tfar = dist - 0.001f;
+      // one division per light (the reference divides per component:
+      // light.h:82-93, 127-143, 176-188); the oracle's GPU-stream mode
+      // follows this arithmetic
+      const float inv = 1.0f / dist;
+      const float inv2 = inv * inv;
+      Lv = Lv * inv;
+      cos_n = dot(N, Lv);
+      if (type == kLightSpot) {
+        f3 ld = {L.dir[0], L.dir[1], L.dir[2]};
+        float cos_l = dot(ld, -Lv);
+        float falloff = (cos_l - L.cos_outer) / (L.cos_inner - L.cos_outer);
+        falloff = fminf(fmaxf(falloff, 0.0f), 1.0f);
+        if (!(cos_n < 0.f)) rad = f3{L.ic[0], L.ic[1], L.ic[2]} * inv2 * falloff;
+      } else if (type == kLightPoint) {
+        if (!(cos_n < 0.f)) rad = f3{L.ic[0], L.ic[1], L.ic[2]} * inv2;
+      } else {
+        if (!(cos_n < 0.f)) rad = emit * inv2;
       }
-      float w = maxc(radw);
-      if (!(w > 0.f)) continue;  // W += 0 leaves W unchanged; never picked
-      const float* ls = sc.light_scale + 3 * l;
-      rad = cmul(rad, f3{ls[0], ls[1], ls[2]});
-      float u = word_uniform(light_word(lk.k3, l));
-      W += w;
-      if (u * W < w) {
-        pick = (int)l;
-        pw = w;
-        pap = ap;
-        prad = rad;
-        pdir = sdir;
-        ptfar = tfar;
-      }
+    }
+    // BRDF * cosine at a path vertex: scaled albedo for the contribution,
+    // unscaled for the selection weight
+    f3 radw = rad;
+    if (VERTEX && !(cos_n < 0.f)) {
+      bool ok = cos_n > 0.0f;
+      f3 bv = ok ? brdf : f3{0.f, 0.f, 0.f};
+      f3 bw = ok ? brdf_w : f3{0.f, 0.f, 0.f};
+      radw = cmul(rad, cos_n * bw);
+      rad = cmul(rad, cos_n * bv);
+    }
+    float w = maxc(radw);
+    if (!(w > 0.f)) continue;  // W += 0 leaves W unchanged; never picked
+    const uint32_t l3 = 3u * (uint32_t)l;
+    rad = cmul(rad, f3{uniform_load(sc.light_scale, l3), uniform_load(sc.light_scale, l3 + 1),
+                       uniform_load(sc.light_scale, l3 + 2)});
+    float u = word_uniform(light_word(lk.k3, (uint32_t)l));
+    W += w;
+    if (u * W < w) {
+      pick = l;
+      pw = w;
+      pap = ap;
+      prad = rad;
+      pdir = sdir;
+      ptfar = tfar;
     }
   }
   if (W < 1e-6f || pick < 0) return false;
