@@ -827,8 +827,7 @@ static V3 TraceRef(const TraceCtx& cx, V3 origin, V3 dir, int depth, RefRng& r) 
 // The stream the HIP kernels consume: every path vertex ("site") seeds a
 // xoshiro128** state from Philox4x32-10(counter, key = seed). Counter:
 // (texel, sample | depth << 16 | branch << 20, site kind, job).
-enum { SITE_TEXEL = 0, SITE_VERTEX = 1, SITE_ALPHA = 2, SITE_LIGHTS_TEXEL = 3,
-       SITE_LIGHTS_VERTEX = 4 };
+enum { SITE_TEXEL = 0, SITE_VERTEX = 1, SITE_ALPHA = 2 };
 
 static Xoshiro128ss site_rng(uint64_t seed, uint32_t texel, uint32_t sample,
                              uint32_t depth, uint32_t branch, uint32_t kind,
@@ -843,37 +842,26 @@ static Xoshiro128ss site_rng(uint64_t seed, uint32_t texel, uint32_t sample,
   return x;
 }
 
-static LightWords light_words(uint64_t seed, uint32_t texel, uint32_t sample, uint32_t depth,
-                              uint32_t branch, uint32_t kind, uint32_t job) {
-  uint32_t ctr[4] = {texel, sample | (depth << 16) | (branch << 20), kind, job};
-  uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
-  LightWords w;
-  philox4x32_10(ctr, key, w.k);
-  return w;
-}
-
 // One NEE sample with the reservoir pick over lights: for each light in
-// order, its own words (area: triangle from word 0, u1, u2 from words 1, 2)
-// and u from word 3; W += w; pick light if u*W < w. A light of weight 0
-// draws nothing (no other light's words depend on it).
+// order, draw (area: triangle, u1, u2) then u; W += w; pick light if u*W < w.
+// The pick is light i with probability w_i / W, as discrete_distribution's.
 static V3 NeeGpu(const Scene& s, V3 P, V3 N, V3 wo, const Mat* mat, V2 uv,
-                 const LightWords& lw, bool texel, float* sh, Counters& c) {
+                 Xoshiro128ss& x, bool texel, float* sh, Counters& c) {
   float W = 0.f;
   int pick = -1;
   float pw = 0.f, pap = 1.f;
   V3 prad{};
   ShadowRay pray;
   for (const Light& l : s.lights) {
-    const uint32_t li = (uint32_t)(&l - s.lights.data());
     ShadowRay ray;
     float ap = 1.0f;
     V3 rd;
     if (l.type == SHB_LIGHT_AREA) {
       if (area_valid(s, l)) {
         uint32_t nt = (uint32_t)(s.geos[l.geom].idx.size() / 3);
-        uint32_t tri = nt == 1 ? 0u : lw.index(0, li, nt);
-        float u1 = lw.uniform(1, li);
-        float u2 = lw.uniform(2, li);
+        uint32_t tri = x.index(nt);
+        float u1 = x.uniform();
+        float u2 = x.uniform();
         AreaSample as = SampleAreaLightAt(s, l, tri, u1, u2);
         rd = LightRadiance(s, l, &as, P, N, !texel, mat, uv, wo, &ray, true);
         ap = as.pdf;
@@ -886,11 +874,10 @@ static V3 NeeGpu(const Scene& s, V3 P, V3 N, V3 wo, const Mat* mat, V2 uv,
       rd = LightRadiance(s, l, nullptr, P, N, !texel, mat, uv, wo, &ray, true);
     }
     float w = maxc(rd);
-    if (!(w > 0.f)) continue;
-    float u = lw.uniform(3, li);
+    float u = x.uniform();
     W += w;
-    if (u * W < w) {
-      pick = (int)li;
+    if (w > 0.f && u * W < w) {
+      pick = (int)(&l - s.lights.data());
       pw = w;
       pap = ap;
       prad = rd;
@@ -953,9 +940,7 @@ static V3 TraceGpu(const GpuCtx& cx, V3 origin, V3 dir, int depth,
   V3 hit_pos = occ.position + occ.normal * 0.001f;
   Xoshiro128ss x = site_rng(cx.seed, cx.texel, cx.sample, (uint32_t)depth,
                             branch, SITE_VERTEX, cx.job);
-  const LightWords lw = light_words(cx.seed, cx.texel, cx.sample, (uint32_t)depth, branch,
-                                    SITE_LIGHTS_VERTEX, cx.job);
-  V3 Ld = NeeGpu(*cx.s, hit_pos, occ.normal, -dir, &mat, occ.uv, lw, false,
+  V3 Ld = NeeGpu(*cx.s, hit_pos, occ.normal, -dir, &mat, occ.uv, x, false,
                  nullptr, *cx.c);
   color += Ld;
   float r1 = x.uniform(), r2 = x.uniform();
@@ -1502,9 +1487,7 @@ int orc_bake(void* sp, const shb_surface_point* pts, int width, int height,
             TraceCtx cx{&s, bounces, num_light_samples, &vis, &c};
             Li = TraceRef(cx, origin, dw, 0, rr) * inv_pdf_uniform;
           } else {
-            const LightWords lw = light_words(seed, (uint32_t)idx, (uint32_t)smp, 0, 1,
-                                              SITE_LIGHTS_TEXEL, job);
-            NeeGpu(s, P, N, V3{}, nullptr, V2{}, lw, true, nee1.data(), c);
+            NeeGpu(s, P, N, V3{}, nullptr, V2{}, x, true, nee1.data(), c);
             GpuCtx cx{&s, bounces, seed, (uint32_t)idx, (uint32_t)smp, job, &vis, &c};
             Li = TraceGpu(cx, origin, dw, 0, 1) * inv_pdf_uniform;
           }

@@ -124,28 +124,6 @@ struct Xoshiro128ss {
   }
 };
 
-// Words of the NEE light loop in the GPU stream: four Philox words per site
-// (a site kind of their own), and for light l the word mix32(key_k + l *
-// 0x9E3779B9) with mix32 the two-round xorshift-multiply hash (Wellons'
-// lowbias32 form: shifts 16, 15, 15; multipliers 0x21f0aaad, 0x735a2d97).
-inline uint32_t mix32(uint32_t x) {
-  x ^= x >> 16;
-  x *= 0x21f0aaadu;
-  x ^= x >> 15;
-  x *= 0x735a2d97u;
-  x ^= x >> 15;
-  return x;
-}
-
-struct LightWords {
-  uint32_t k[4];
-  uint32_t word(int i, uint32_t l) const { return mix32(k[i] + l * 0x9E3779B9u); }
-  float uniform(int i, uint32_t l) const { return (float)(word(i, l) >> 8) * 0x1p-24f; }
-  uint32_t index(int i, uint32_t l, uint32_t n) const {
-    return (uint32_t)(((uint64_t)word(i, l) * (uint64_t)n) >> 32);
-  }
-};
-
 }  // namespace orc
 
 #endif  // SHB_ORACLE_MATH_H_

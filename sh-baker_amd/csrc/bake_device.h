@@ -119,25 +119,19 @@ __device__ __forceinline__ void area_point(const AreaTri& at, float u1, float u2
 // light.cpp:16-78), weight = maxCoeff, selection by a streaming reservoir:
 // for each light draw u, W += w, keep light i when u*W < w -- the same
 // w_i / sum(w) selection law as std::discrete_distribution. Returns false
-// when nothing is selected, sum(w) < 1e-6 or joint pdf < 1e-3. Light l
-// draws its own words (triangle index, barycentrics, reservoir uniform)
-// from `lk`; lights of weight zero draw nothing.
+// when nothing is selected, sum(w) < 1e-6 or joint pdf < 1e-3.
 // Selection weights use the radiance without the light / albedo scales of
 // shb_set_params, so the selection law does not depend on the fitted
 // parameters (with every scale 1 this is the reference's law exactly); the
 // returned radiance carries the scales.
 template <bool VERTEX>
 __device__ __forceinline__ bool nee_sample(const SceneView& sc, f3 P, f3 N, f3 brdf, f3 brdf_w,
-                                           bool front, const LightKeys& lk, NeeResult& out) {
-  // at a path vertex seen from behind the BRDF is zero for every light
-  if (VERTEX && !front) return false;
+                                           bool front, Rng& rng, NeeResult& out) {
   float W = 0.f;
   int pick = -1;
   float pw = 0.f, pap = 1.f, ptfar = 0.f;
   f3 prad = {0.f, 0.f, 0.f}, pdir = {0.f, 0.f, 0.f};
   const int nl = sc.num_lights;
-  // |P| bound for the conservative culling test below
-  const float pmax = fmaxf(fabsf(P.x), fmaxf(fabsf(P.y), fabsf(P.z)));
   for (int l = 0; l < nl; ++l) {
     // every lane visits the same light: a wave-uniform index lets the light
     // record come through the scalar cache
@@ -160,30 +154,19 @@ __device__ __forceinline__ bool nee_sample(const SceneView& sc, f3 P, f3 N, f3 b
           ap = 0.f;
           target = f3{0.f, 0.f, 0.f};
         } else {
-          float u1 = word_uniform(light_word(lk.k1, (uint32_t)l));
-          float u2 = word_uniform(light_word(lk.k2, (uint32_t)l));
+          uint32_t tri = rng.index(L.num_tris);
+          float u1 = rng.uniform();
+          float u2 = rng.uniform();
           if (u1 + u2 > 1.0f) {
             u1 = 1.0f - u1;
             u2 = 1.0f - u2;
           }
+          // single-triangle emitters (tri is 0) read a wave-uniform record
           float tu = 0.f, tv = 0.f;
-          if (L.num_tris == 1) {
-            // single-triangle emitters read a wave-uniform record; when all
-            // three vertices lie below the tangent plane by more than the
-            // rounding of the sampled point's cosine, every sample has
-            // cos_n < 0 and weight 0: the light is skipped (no other light's
-            // numbers depend on it)
-            const AreaTri at = uniform_load(sc.area_tris, L.first_tri);
-            const float d0 = dot(N, f3{at.v0[0], at.v0[1], at.v0[2]} - P);
-            const float d1 = dot(N, f3{at.v1[0], at.v1[1], at.v1[2]} - P);
-            const float d2 = dot(N, f3{at.v2[0], at.v2[1], at.v2[2]} - P);
-            const float tol = 1e-5f * (1.0f + pmax + at.vmax);
-            if (fmaxf(d0, fmaxf(d1, d2)) < -tol) continue;
-            area_point(at, u1, u2, target, ap, tu, tv);
-          } else {
-            const uint32_t tri = word_index(light_word(lk.k0, (uint32_t)l), L.num_tris);
+          if (L.num_tris == 1)
+            area_point(uniform_load(sc.area_tris, L.first_tri), u1, u2, target, ap, tu, tv);
+          else
             area_point(sc.area_tris[L.first_tri + tri], u1, u2, target, ap, tu, tv);
-          }
           if (L.emit_const)
             emit = f3{L.emit[0], L.emit[1], L.emit[2]};
           else
@@ -220,20 +203,19 @@ __device__ __forceinline__ bool nee_sample(const SceneView& sc, f3 P, f3 N, f3 b
     // unscaled for the selection weight
     f3 radw = rad;
     if (VERTEX && !(cos_n < 0.f)) {
-      bool ok = cos_n > 0.0f;
+      bool ok = front && cos_n > 0.0f;
       f3 bv = ok ? brdf : f3{0.f, 0.f, 0.f};
       f3 bw = ok ? brdf_w : f3{0.f, 0.f, 0.f};
       radw = cmul(rad, cos_n * bw);
       rad = cmul(rad, cos_n * bv);
     }
-    float w = maxc(radw);
-    if (!(w > 0.f)) continue;  // W += 0 leaves W unchanged; never picked
     const uint32_t l3 = 3u * (uint32_t)l;
     rad = cmul(rad, f3{uniform_load(sc.light_scale, l3), uniform_load(sc.light_scale, l3 + 1),
                        uniform_load(sc.light_scale, l3 + 2)});
-    float u = word_uniform(light_word(lk.k3, (uint32_t)l));
+    float w = maxc(radw);
+    float u = rng.uniform();
     W += w;
-    if (u * W < w) {
+    if (w > 0.f && u * W < w) {
       pick = l;
       pw = w;
       pap = ap;

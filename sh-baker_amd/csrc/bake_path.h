@@ -96,8 +96,7 @@ __device__ __forceinline__ bool generate_path(const BakeArgs& a, const JobDev& J
   a.li[p] = make_float4(0.f, 0.f, 0.f, 0.f);
   NeeResult nr;
   f3 zero = {0.f, 0.f, 0.f};
-  const LightKeys lk = light_keys(J.seed, texel, sr.smp, 0, 1, kSiteLightsTexel, J.stream);
-  if (nee_sample<false>(J.scene, P, N, zero, zero, false, lk, nr)) {
+  if (nee_sample<false>(J.scene, P, N, zero, zero, false, rng, nr)) {
     a.nee0[p] = make_float4(nr.L.x, nr.L.y, nr.L.z, 0.f);
     a.nee0_dir[p] = make_float4(nr.dir.x, nr.dir.y, nr.dir.z, as_float((uint32_t)nr.light));
     size_t so = (size_t)a.P + p;
@@ -195,10 +194,8 @@ __device__ __forceinline__ void shade_path(const BakeArgs& a, const JobDev& J, c
   f3 brdf_w;
   f3 brdf = lambert(sc, mat, tu, tv, &brdf_w);
   Rng rng = site_rng(J.seed, texel, sr.smp, (uint32_t)depth, branch, kSiteVertex, J.stream);
-  const LightKeys lk =
-      light_keys(J.seed, texel, sr.smp, (uint32_t)depth, branch, kSiteLightsVertex, J.stream);
   NeeResult nr;
-  if (nee_sample<true>(sc, hit_pos, nrm, brdf, brdf_w, front, lk, nr)) {
+  if (nee_sample<true>(sc, hit_pos, nrm, brdf, brdf_w, front, rng, nr)) {
     f3 c = cmul(thr, nr.L);
     a.scontrib[p] = make_float4(c.x, c.y, c.z, as_float((uint32_t)nr.light));
     a.sray[2 * (size_t)p] = make_float4(hit_pos.x, hit_pos.y, hit_pos.z, kRayEps);

@@ -252,9 +252,6 @@ int build_host_scene(const shb_scene_desc* d, HostScene* out, std::string* err) 
           float area = std::sqrt(h3sq(h3cross(h3sub(v[0], v[1]), h3sub(v[0], v[2])))) / 2.f;
           at.pdf = std::max(1e-6f, 1.f / (float)nt * 1.f / area);
           at.has_uv = hasUV ? 1.f : 0.f;
-          at.vmax = 0.f;
-          for (int k = 0; k < 3; ++k)
-            at.vmax = std::max({at.vmax, std::fabs(v[k].x), std::fabs(v[k].y), std::fabs(v[k].z)});
           hs.area_tris.push_back(at);
         }
         const MaterialDev& md = hs.mats[mi];

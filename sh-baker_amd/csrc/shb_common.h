@@ -94,7 +94,7 @@ struct alignas(16) AreaTri {
   float v1[3];
   float has_uv;
   float v2[3];
-  float vmax;          // largest |coordinate| of the three vertices
+  float pad;
   float uv[6];
   float pad2[2];
 };
@@ -136,17 +136,8 @@ struct SceneView {
 // ------------------------------------------------------------ RNG
 // Philox4x32-10 (Salmon et al., SC'11) seeds one xoshiro128** state per path
 // vertex ("site"); counter = (texel, sample | depth<<16 | branch<<20, kind,
-// job), key = 64-bit seed. The NEE light loop draws from its own site kind:
-// four Philox words (LightKeys), hashed with each light's index, so a light
-// whose weight is provably zero can be skipped without shifting the numbers
-// of any other light.
-enum : uint32_t {
-  kSiteTexel = 0,
-  kSiteVertex = 1,
-  kSiteAlpha = 2,
-  kSiteLightsTexel = 3,
-  kSiteLightsVertex = 4
-};
+// job), key = 64-bit seed.
+enum : uint32_t { kSiteTexel = 0, kSiteVertex = 1, kSiteAlpha = 2 };
 
 SHB_HD void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
 #pragma unroll
@@ -192,35 +183,6 @@ SHB_HD Rng site_rng(uint64_t seed, uint32_t texel, uint32_t sample,
   Rng r{c[0], c[1], c[2], c[3]};
   if ((c[0] | c[1] | c[2] | c[3]) == 0u) r.s0 = 1u;
   return r;
-}
-
-struct LightKeys {
-  uint32_t k0, k1, k2, k3;
-};
-
-SHB_HD LightKeys light_keys(uint64_t seed, uint32_t texel, uint32_t sample, uint32_t depth,
-                            uint32_t branch, uint32_t kind, uint32_t job) {
-  uint32_t c[4] = {texel, sample | (depth << 16) | (branch << 20), kind, job};
-  philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
-  return LightKeys{c[0], c[1], c[2], c[3]};
-}
-
-// 32-bit integer hash (xorshift-multiply, two rounds; Wellons' "lowbias32"
-// form with constants of low measured bias)
-SHB_HD uint32_t mix32(uint32_t x) {
-  x ^= x >> 16;
-  x *= 0x21f0aaadu;
-  x ^= x >> 15;
-  x *= 0x735a2d97u;
-  x ^= x >> 15;
-  return x;
-}
-
-// word k of light l: mix32(key_k + l * golden); uniform = top 24 bits / 2^24
-SHB_HD uint32_t light_word(uint32_t key, uint32_t l) { return mix32(key + l * 0x9E3779B9u); }
-SHB_HD float word_uniform(uint32_t w) { return (float)(w >> 8) * 0x1p-24f; }
-SHB_HD uint32_t word_index(uint32_t w, uint32_t n) {
-  return (uint32_t)(((uint64_t)w * (uint64_t)n) >> 32);
 }
 
 // A value the caller knows to be equal on every active lane, made visibly
