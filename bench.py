@@ -199,6 +199,42 @@ def bench_env(shb, w=8192, h=4096, reps=10, cpu=True):
     return res
 
 
+def bench_stress(shb, scenes, spp=32):
+    """The large stress configuration (4.2M triangles, 256 lights, 4096^2
+    lightmap, supersample 2, 5 bounces, 4 dilation passes) at `spp` samples
+    per texel instead of 512: the same scene, surface and path depth, 512 /
+    spp times fewer independent samples. Reports throughput and the bake
+    time scaled to 512 spp."""
+    t0 = time.time()
+    s = scenes.stress_scene(4096)
+    gen_s = time.time() - t0
+    log(f"stress: {s.num_triangles} tris, {len(s.lights)} lights ({gen_s:.1f}s)")
+    b = shb.Baker(0)
+    t0 = time.perf_counter()
+    b.upload_scene(s)
+    up_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    b.rasterize(4096, 4096, 2, want=False)
+    ra_s = time.perf_counter() - t0
+    valid = int(b.validity_mask().sum())
+    log(f"stress: upload + BVH {up_s:.2f}s, raster 8192^2 {ra_s:.2f}s, {valid} valid texels")
+    t0 = time.perf_counter()
+    _, _, _, st = b.bake_lightmap(spp, 5, 4, seed=12345, to_host=False)
+    bake_s = time.perf_counter() - t0
+    b.close()
+    rays = st["rays_extension"] + st["rays_shadow"]
+    res = {"config": "stress: 4.2M-triangle room, 256 lights, 4096^2 ss2, 5 bounces, dilation 4",
+           "tris": s.num_triangles, "lights": len(s.lights), "valid_texels": valid,
+           "spp_run": spp, "spp_config": 512, "rays": rays,
+           "mrays_s": round(rays / bake_s / 1e6, 1), "bake_seconds": round(bake_s, 3),
+           "bake_seconds_scaled_to_512spp": round(bake_s * 512 / spp, 1),
+           "upload_bvh_seconds": round(up_s, 2), "raster_seconds": round(ra_s, 3),
+           "scene_generation_seconds": round(gen_s, 1),
+           "note": "one context (no 64-tile batched form: tiles of one surface are not jobs)"}
+    log(f"stress: {json.dumps(res)}")
+    return res
+
+
 def bench_adjoint(shb, scenes, res=512, spp=64, bounces=3, iters=50, reps=3):
     """Differentiation form: the 262k-triangle room at res^2 (ss 1), spp,
     bounces; adjoint vs forward time, gradient vs central finite differences
@@ -292,6 +328,9 @@ def main():
                     help="skip the environment-SH measurement")
     ap.add_argument("--env-only", action="store_true",
                     help="only the environment-SH measurement (its own JSON line)")
+    ap.add_argument("--stress", action="store_true",
+                    help="only the 4.2M-triangle / 256-light stress bake (its own JSON line)")
+    ap.add_argument("--stress-spp", type=int, default=32)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device copy of this size (counter calibration)")
     args = ap.parse_args()
@@ -308,6 +347,9 @@ def main():
 
     if args.env_only:
         print(json.dumps({"environment_sh": bench_env(shb, cpu=args.cpu_seconds > 0)}), flush=True)
+        return
+    if args.stress:
+        print(json.dumps({"stress": bench_stress(shb, scenes, args.stress_spp)}), flush=True)
         return
 
     t0 = time.time()

@@ -15,7 +15,8 @@ import math
 import numpy as np
 
 from . import (GEOM_HAS_LIGHTMAP_UVS, GEOM_HAS_NORMALS, GEOM_HAS_TANGENTS,
-               GEOM_HAS_TEXTURE_UVS, LIGHT_AREA, LIGHT_DIRECTIONAL, LIGHT_POINT, ShbGeometry,
+               GEOM_HAS_TEXTURE_UVS, LIGHT_AREA, LIGHT_DIRECTIONAL, LIGHT_POINT, LIGHT_SPOT,
+               ShbGeometry,
                ShbLight, ShbMaterial, ShbSceneDesc)
 
 IDENTITY = (1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0)
@@ -397,7 +398,7 @@ SYNTH_SEED = 0x5EED0002
 
 def synthetic_room_scene(width=1024, height=1024, seed=SYNTH_SEED, n_pillars=400, n_boxes=2000,
                          n_emissive=64, n_point=4, n_materials=128, segments=32, rings=8,
-                         target_tris=262144, padding=2) -> Scene:
+                         target_tris=262144, padding=2, n_spot=0) -> Scene:
     """Sponza-scale synthetic room: closed room 40 x 15 x 20 m
     (x in [-20,20], y in [0,15], z in [-10,10]) with `n_pillars` 32-segment
     cylinders (radius U[0.2,0.6], full height, `rings` rings, 30-triangle fan
@@ -405,7 +406,8 @@ def synthetic_room_scene(width=1024, height=1024, seed=SYNTH_SEED, n_pillars=400
     subdivided so the total reaches about `target_tris`; `n_materials`
     albedos U[0.2,0.8]^3 (sRGB8), `n_point` point lights (log-U[10,200]),
     one directional light and `n_emissive` emissive triangles (white,
-    emission U[1,20]) near the ceiling, each its own area light."""
+    emission U[1,20]) near the ceiling, each its own area light; `n_spot`
+    downward spot lights (log-U[20,400], cones 20-35 / 35-50 degrees)."""
     rng = np.random.default_rng(seed)
     b = SceneBuilder()
     mats = []
@@ -449,6 +451,15 @@ def synthetic_room_scene(width=1024, height=1024, seed=SYNTH_SEED, n_pillars=400
                     position=(rng.uniform(-18, 18), rng.uniform(1, 14), rng.uniform(-9, 9)),
                     intensity=float(math.exp(rng.uniform(math.log(10), math.log(200)))),
                     color=(1, 1, 1))
+    for _ in range(n_spot):
+        inner = math.radians(rng.uniform(20, 35))
+        outer = inner + math.radians(rng.uniform(5, 15))
+        sd = np.array([rng.uniform(-0.3, 0.3), -1.0, rng.uniform(-0.3, 0.3)])
+        b.add_light(type=LIGHT_SPOT,
+                    position=(rng.uniform(-18, 18), rng.uniform(10, 14), rng.uniform(-9, 9)),
+                    direction=tuple(sd / np.linalg.norm(sd)),
+                    intensity=float(math.exp(rng.uniform(math.log(20), math.log(400)))),
+                    color=(1, 1, 1), cos_inner_cone=math.cos(inner), cos_outer_cone=math.cos(outer))
     d = np.array([rng.uniform(-0.5, 0.5), -1.0, rng.uniform(-0.5, 0.5)])
     b.add_light(type=LIGHT_DIRECTIONAL, direction=tuple(d / np.linalg.norm(d)), intensity=3.0,
                 color=(1, 1, 1))
@@ -464,6 +475,16 @@ def scaled_synthetic_scene(target_tris, width, seed, **kw) -> Scene:
     n_e = max(2, min(64, int(round(64 * max(f, 0.1)))))
     return synthetic_room_scene(width, width, seed, n_pillars=n_p, n_boxes=n_b, n_emissive=n_e,
                                 target_tris=target_tris, **kw)
+
+
+def stress_scene(width=4096, seed=0x5EED0005) -> Scene:
+    """The large stress configuration: the synthetic room with 400 pillars of
+    64 segments x 48 rings, 16x the boxes (32000), walls filling up to about
+    4M triangles, and 256 lights (184 emissive triangles, 56 point, 15 spot,
+    1 directional)."""
+    return synthetic_room_scene(width, width, seed, n_pillars=400, n_boxes=32000, n_emissive=184,
+                                n_point=56, n_spot=15, segments=64, rings=48,
+                                target_tris=4194304)
 
 
 # -------------------------------------------------- reference test scenes
