@@ -199,7 +199,7 @@ def bench_env(shb, w=8192, h=4096, reps=10, cpu=True):
     return res
 
 
-def bench_stress(shb, scenes, spp=32):
+def bench_stress(shb, scenes, spp=512):
     """The large stress configuration (4.2M triangles, 256 lights, 4096^2
     lightmap, supersample 2, 5 bounces, 4 dilation passes) at `spp` samples
     per texel instead of 512: the same scene, surface and path depth, 512 /
@@ -330,7 +330,7 @@ def main():
                     help="only the environment-SH measurement (its own JSON line)")
     ap.add_argument("--stress", action="store_true",
                     help="only the 4.2M-triangle / 256-light stress bake (its own JSON line)")
-    ap.add_argument("--stress-spp", type=int, default=32)
+    ap.add_argument("--stress-spp", type=int, default=512)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device copy of this size (counter calibration)")
     args = ap.parse_args()
