@@ -328,6 +328,8 @@ def main():
                     help="skip the environment-SH measurement")
     ap.add_argument("--env-only", action="store_true",
                     help="only the environment-SH measurement (its own JSON line)")
+    ap.add_argument("--no-stress", action="store_true",
+                    help="skip the 4.2M-triangle / 256-light stress bake")
     ap.add_argument("--stress", action="store_true",
                     help="only the 4.2M-triangle / 256-light stress bake (its own JSON line)")
     ap.add_argument("--stress-spp", type=int, default=512)
@@ -439,6 +441,7 @@ def main():
 
     adjoint = None if args.no_adjoint else bench_adjoint(shb, scenes)
     env = None if args.no_env else bench_env(shb, cpu=args.cpu_seconds > 0)
+    stress = None if args.no_stress else bench_stress(shb, scenes, args.stress_spp)
 
     batched = small = None
     if not args.no_batched:
@@ -504,6 +507,7 @@ def main():
         "batched_16_jobs": batched,
         "batched_48_small_jobs": small,
         "environment_sh": env,
+        "stress_4m_tris_256_lights": stress,
         "cpu_baseline": cpu,
     }
     if cpu:
