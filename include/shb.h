@@ -167,6 +167,12 @@ typedef struct shb_bake_stats {
   double seconds_trace_shadow;
   uint32_t launches_extension;  /* traversal kernel launches of each kind   */
   uint32_t launches_shadow;
+  /* alpha < 1 splits (baker.cpp:97-109) whose transmitted ray took a spare
+   * path slot; bake passes run (a pass that runs out of spare slots is
+   * redone with more; the result does not depend on the count)            */
+  uint64_t branches;
+  uint32_t passes;
+  uint32_t reserved;
 } shb_bake_stats;
 
 /* Launch parameters of the wavefront kernels (chosen by shb_autotune). */

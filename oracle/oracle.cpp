@@ -610,7 +610,7 @@ static AreaSample SampleAreaLightAt(const Scene& s, const Light& l,
     u2 = 1.0f - u2;
   }
   float w = 1.0f - u1 - u2;
-  V3 v0 = geo.v[i0], v1 = geo.v[i1], v2 = geo.v[i2];  // object space (Q4)
+  V3 v0 = geo.v[i0], v1 = geo.v[i1], v2 = geo.v[i2];  // object space (light.cpp:45-49)
   V3 p = w * v0 + u1 * v1 + u2 * v2;
   V3 n{0, 1, 0};
   if (!geo.n.empty()) n = normalized(w * geo.n[i0] + u1 * geo.n[i1] + u2 * geo.n[i2]);
@@ -826,13 +826,16 @@ static V3 TraceRef(const TraceCtx& cx, V3 origin, V3 dir, int depth, RefRng& r) 
 // ------------------------------------------ GPU stream (Philox + xoshiro)
 // The stream the HIP kernels consume: every path vertex ("site") seeds a
 // xoshiro128** state from Philox4x32-10(counter, key = seed). Counter:
-// (texel, sample | depth << 16 | branch << 20, site kind, job).
-enum { SITE_TEXEL = 0, SITE_VERTEX = 1, SITE_ALPHA = 2 };
+// (texel, sample | depth << 16 | branch << 20, kind | (branch >> 12) << 8,
+// job); the branch id is the vertex's node in the tree of alpha splits
+// (root 1, transmitted 2b + 1, reflected 2b).
+enum { SITE_TEXEL = 0, SITE_VERTEX = 1 };
 
 static Xoshiro128ss site_rng(uint64_t seed, uint32_t texel, uint32_t sample,
                              uint32_t depth, uint32_t branch, uint32_t kind,
                              uint32_t job) {
-  uint32_t ctr[4] = {texel, sample | (depth << 16) | (branch << 20), kind, job};
+  uint32_t ctr[4] = {texel, sample | (depth << 16) | (branch << 20),
+                     kind | ((branch >> 12) << 8), job};
   uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
   uint32_t o[4];
   philox4x32_10(ctr, key, o);
@@ -907,13 +910,11 @@ struct GpuCtx {
   Counters* c;
 };
 
-// Trace with the GPU stream: the control flow of TraceRef except at alpha <
-// 1, where one branch is chosen (transmission with probability 1 - alpha,
-// from the vertex's own alpha site) at weight 1 instead of summing both with
-// weights 1 - alpha and alpha (baker.cpp:97-109): the same expectation.
-// The vertex at (depth, branch) draws NEE then the material sample from its
-// own site; a transmitted ray goes on with branch 2b+1 and the reflected
-// continuation after a choice with 2b; opaque hits keep b.
+// Trace with the GPU stream: the control flow of TraceRef (baker.cpp:70-149)
+// with counter-based numbers. The vertex at (depth, branch) draws NEE then
+// the material sample from its own site. At alpha < 1 the path splits as in
+// the reference: the transmitted ray, weight 1 - alpha, goes on with branch
+// 2b + 1, the reflected continuation with 2b; opaque hits keep b.
 static V3 TraceGpu(const GpuCtx& cx, V3 origin, V3 dir, int depth,
                    uint32_t branch) {
   if (depth > cx.max_depth) return V3{};
@@ -927,34 +928,32 @@ static V3 TraceGpu(const GpuCtx& cx, V3 origin, V3 dir, int depth,
   V3 color{};
   uint32_t next_branch = branch;
   if (alpha < 1.0f) {
-    Xoshiro128ss ax = site_rng(cx.seed, cx.texel, cx.sample, (uint32_t)depth,
-                               branch, SITE_ALPHA, cx.job);
-    if (ax.uniform() < 1.0f - alpha) {
-      V3 hp = occ.position + dir * 0.001f;
-      return TraceGpu(cx, hp, dir, depth + 1, 2 * branch + 1);
-    }
+    V3 hp = occ.position + dir * 0.001f;
+    V3 tr = TraceGpu(cx, hp, dir, depth + 1, 2 * branch + 1);
+    color += (1.0f - alpha) * tr;
     next_branch = 2 * branch;
   }
+  if (alpha == 0.0f) return color;
   V3 em = GetEmission(mat, occ.uv);
-  if (!is_zero(em) && depth == 0) color += em;
+  if (!is_zero(em) && depth == 0) color += alpha * em;
   V3 hit_pos = occ.position + occ.normal * 0.001f;
   Xoshiro128ss x = site_rng(cx.seed, cx.texel, cx.sample, (uint32_t)depth,
                             branch, SITE_VERTEX, cx.job);
   V3 Ld = NeeGpu(*cx.s, hit_pos, occ.normal, -dir, &mat, occ.uv, x, false,
                  nullptr, *cx.c);
-  color += Ld;
+  color += alpha * Ld;
   float r1 = x.uniform(), r2 = x.uniform();
   RefSample smp = SampleMaterialUV(occ.normal, r1, r2);
   if (smp.pdf < 1e-3f) return color;
-  // The counter-based stream lets a zero-BRDF continuation (back-facing
-  // object-space normal, Q9) end the path without tracing it; the value is
-  // the same and no later draw depends on it.
+  // The counter-based stream lets a zero-BRDF continuation (a back-facing
+  // object-space normal) end the path without tracing it; the value is the
+  // same and no later draw depends on it.
   V3 brdf = EvalMaterial(mat, occ.uv, occ.normal, smp.dir, -dir);
   if (is_zero(brdf) || depth + 1 > cx.max_depth) return color;
   V3 incoming = TraceGpu(cx, hit_pos, smp.dir, depth + 1, next_branch);
   float cosine = dot(occ.normal, smp.dir);
   V3 Lind = cmul(incoming, brdf) * (cosine / smp.pdf);
-  color += Lind;
+  color += alpha * Lind;
   return color;
 }
 

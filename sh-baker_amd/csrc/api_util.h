@@ -56,6 +56,8 @@ inline TraceArgs trace_args(Ctx& c) {
   a.nseg = 1;
   a.root0 = c.bvh.nodes.empty() ? kNoRoot : 0u;
   a.P = 0xffffffffu;
+  a.Pc = 0xffffffffu;
+  a.link = nullptr;
   return a;
 }
 
@@ -75,8 +77,9 @@ inline int check_overflow(Ctx& c, size_t cap) {
 
 // Counter layout per batch (uint32): extension-queue counts by depth, shadow
 // counts by depth, queue heads of the persistent extension and shadow
-// traversals by depth.
-constexpr int kCntExt = 0, kCntSh = 20, kCntHeadExt = 40, kCntHeadSh = 56, kCntStride = 72;
+// traversals by depth, branch slots asked for by alpha < 1 splits.
+constexpr int kCntExt = 0, kCntSh = 20, kCntHeadExt = 40, kCntHeadSh = 56, kCntSpawn = 72,
+              kCntStride = 76;
 
 // SHB_DEBUG_STAGES=1: synchronise after every bake stage and log its time to
 // stderr (diagnostics; off by default).
@@ -111,6 +114,8 @@ struct BakeRun {
   uint64_t node_visits_sh = 0, tri_tests_sh = 0;
   double dev_s = 0, trace_s = 0, trace_sh_s = 0;
   uint32_t launches_ext = 0, launches_sh = 0;
+  uint64_t branches = 0;
+  uint32_t passes = 0;
   bool graph_captured = false;
 };
 
@@ -129,6 +134,9 @@ inline void fill_stats(shb_bake_stats* s, const BakeRun& r, uint32_t spp, double
   s->tri_tests_shadow = r.tri_tests_sh;
   s->seconds_trace_shadow = r.trace_sh_s;
   s->launches_extension = r.launches_ext;
+  s->branches = r.branches;
+  s->passes = r.passes;
+  s->reserved = 0;
   s->launches_shadow = r.launches_sh;
 }
 

@@ -7,6 +7,8 @@
 //   k_trace<any>      over the shadow queue, epilogue adds the NEE term
 // k_generate makes the primary rays and the texel NEE shadow rays; k_project
 // folds each texel's samples into 27 SH coefficients with compensated sums.
+// Transmitted rays of alpha < 1 splits run in spare slots of the batch;
+// k_merge_branches adds their radiance to their parents' before k_project.
 // Mirrors BakeSHLightMap / Trace / EvaluateLightSamples /
 // AccumulateIncomingLightSamples (reference src/baker.cpp:70-304,
 // src/light.cpp:82-293).
@@ -54,17 +56,17 @@ struct OccludedEpilogue {
   }
 };
 
-// Shadow rays of the bake: slot < P is a vertex NEE ray (adds its
-// contribution to the path's radiance), slot >= P a texel NEE ray (marks the
-// texel term visible).
+// Shadow rays of the bake: slot < Pc is a vertex NEE ray (adds its
+// contribution to the path's radiance), slot >= Pc a texel NEE ray (marks
+// the texel term visible).
 struct BakeShadowEpilogue {
   float4* li;
   const float4* contrib;
   float4* nee0;
-  uint32_t P;
+  uint32_t Pc;
   __device__ void operator()(uint32_t slot, const TravState& s) const {
     if (s.best != 0x7fffffff) return;
-    if (slot < P) {
+    if (slot < Pc) {
       float4 c = contrib[slot];
       float4 l = li[slot];
       l.x += c.x;
@@ -72,7 +74,7 @@ struct BakeShadowEpilogue {
       l.z += c.z;
       li[slot] = l;
     } else {
-      nee0[slot - P].w = 1.0f;
+      nee0[slot - Pc].w = 1.0f;
     }
   }
 };
@@ -83,13 +85,13 @@ struct BakeShadowEpilogue {
 struct BakeShadowAdjEpilogue {
   uint32_t* vflag;
   float4* nee0;
-  uint32_t P;
+  uint32_t Pc;
   __device__ void operator()(uint32_t slot, const TravState& s) const {
     if (s.best != 0x7fffffff) return;
-    if (slot < P)
+    if (slot < Pc)
       vflag[slot] = 1u;
     else
-      nee0[slot - P].w = 1.0f;
+      nee0[slot - Pc].w = 1.0f;
   }
 };
 
@@ -114,7 +116,11 @@ __device__ __forceinline__ void trace_begin(const TraceArgs& a, uint32_t slot, T
   trav_init(s, r, d.w);
   uint32_t root = a.root0;
   if (a.nseg > 1) {
-    const uint32_t p = slot >= a.P ? slot - a.P : slot;  // texel shadow rays sit at P + p
+    uint32_t p = slot;
+    if (p >= a.Pc)
+      p -= a.Pc;  // texel shadow rays sit at Pc + p
+    else if (p >= a.P)
+      p = a.link[p - a.P].x;  // a branch slot: its root
     root = a.map->root[find_segment(a.map->slot_base, a.nseg, p)];
   }
   s.gbase = root;
@@ -323,7 +329,7 @@ __global__ void k_generate(BakeArgs a) {
       with_job(a, sr.job, [&](const JobDev& J) { shadow = generate_path(a, J, sr, p); });
     }
     uint32_t at = wave_append(a.shadow_count, shadow);
-    if (shadow) a.shadow_queue[at] = a.P + p;
+    if (shadow) a.shadow_queue[at] = a.Pc + p;
   }
 }
 
@@ -341,24 +347,29 @@ __global__ void k_shade(BakeArgs a, int depth, const uint32_t* in_queue,
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
     uint32_t i = base + threadIdx.x;
     bool shadow = false, extend = false;
-    uint32_t p = 0;
+    uint32_t p = 0, child = kNoSlot;
     if (i < n) {
       p = in_queue ? in_queue[i] : i;
-      const SlotRef sr = slot_ref(a, p);
+      const SlotRef sr = slot_ref(a, root_slot(a, p));
       with_job(a, sr.job, [&](const JobDev& J) {
-        shade_path<ADJ>(a, J, sr, depth, p, shadow, extend, &acc);
+        shade_path<ADJ>(a, J, sr, depth, p, shadow, extend, child, &acc);
       });
     }
     uint32_t at = wave_append(a.shadow_count, shadow);
     if (shadow) a.shadow_queue[at] = p;
     uint32_t at2 = wave_append(out_count, extend);
     if (extend) out_queue[at2] = p;
+    if (a.S) {  // batches with spare slots: transmitted rays of splits
+      const bool spawned = child != kNoSlot;
+      uint32_t at3 = wave_append(out_count, spawned);
+      if (spawned) out_queue[at3] = child;
+    }
   }
   if (ADJ) acc.flush();
 }
 
 // Adjoint of the visible vertex NEE terms of one depth: the shadow queue's
-// entries below P (texel NEE rays sit at P + slot).
+// entries below Pc (texel NEE rays sit at Pc + slot).
 __global__ void k_adjoint_vertex(BakeArgs a, int depth, const uint32_t* sh_count) {
   extern __shared__ float grad_lds[];
   GradAcc acc;
@@ -366,9 +377,9 @@ __global__ void k_adjoint_vertex(BakeArgs a, int depth, const uint32_t* sh_count
   const uint32_t n = *sh_count;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const uint32_t slot = a.shadow_queue[i];
-    if (slot >= a.P || a.adj_vflag[slot] == 0u) continue;
+    if (slot >= a.Pc || a.adj_vflag[slot] == 0u) continue;
     a.adj_vflag[slot] = 0u;
-    const SlotRef sr = slot_ref(a, slot);
+    const SlotRef sr = slot_ref(a, root_slot(a, slot));
     adjoint_vertex(a, a.jobs[sr.job], depth, slot, &acc);
   }
   acc.flush();
@@ -405,6 +416,25 @@ __global__ void k_project(BakeArgs a) {
       with_job(a, job, [&](const JobDev& J) { project_texel(a, J, rank, slot0, stride); });
   }
   if (ADJ) acc.flush();
+}
+
+// Branch slots spawned at `depth` add their radiance to their parents'. A
+// path shades one vertex per depth, so no parent gains two branches in one
+// launch, and a branch spawned at `depth` spawns its own only deeper, so it
+// is complete (merged into by the earlier, deeper launches) when read here:
+// the sums do not depend on the order of the slots.
+__global__ void k_merge_branches(BakeArgs a, int depth) {
+  const uint32_t n = min(*a.spawn_count, a.S);
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint4 L = a.link[i];
+    if ((int)L.z != depth) continue;
+    const float4 c = a.li[a.P + i];
+    float4 l = a.li[L.y];
+    l.x += c.x;
+    l.y += c.y;
+    l.z += c.z;
+    a.li[L.y] = l;
+  }
 }
 
 __global__ void k_relocate_nodes(const CWNode* src, CWNode* dst, uint32_t n, uint32_t node_off,
@@ -468,14 +498,14 @@ hipError_t launch_trace_occluded(const TraceArgs& a, uint32_t* occ, bool count,
 
 hipError_t launch_trace_bake_shadow(const TraceArgs& a, const BakeArgs& b, bool count,
                                     uint32_t capacity, int block, hipStream_t st) {
-  BakeShadowEpilogue e{b.li, b.scontrib, b.nee0, b.P};
+  BakeShadowEpilogue e{b.li, b.scontrib, b.nee0, b.Pc};
   return count ? launch_trace_t<true, true>(a, e, capacity, block, st)
                : launch_trace_t<true, false>(a, e, capacity, block, st);
 }
 
 hipError_t launch_trace_bake_shadow_adj(const TraceArgs& a, const BakeArgs& b, bool count,
                                         uint32_t capacity, int block, hipStream_t st) {
-  BakeShadowAdjEpilogue e{b.adj_vflag, b.nee0, b.P};
+  BakeShadowAdjEpilogue e{b.adj_vflag, b.nee0, b.Pc};
   return count ? launch_trace_t<true, true>(a, e, capacity, block, st)
                : launch_trace_t<true, false>(a, e, capacity, block, st);
 }
@@ -531,6 +561,12 @@ hipError_t launch_project(const BakeArgs& b, hipStream_t st) {
     hipLaunchKernelGGL(k_project<true>, dim3((T + 127) / 128), dim3(128), adj_lds(b), st, b);
   else
     hipLaunchKernelGGL(k_project<false>, dim3((T + 127) / 128), dim3(128), 0, st, b);
+  return hipGetLastError();
+}
+
+hipError_t launch_merge_branches(const BakeArgs& b, int depth, int grid, hipStream_t st) {
+  if (b.S == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_merge_branches, dim3(grid), dim3(256), 0, st, b, depth);
   return hipGetLastError();
 }
 

@@ -39,6 +39,12 @@ __device__ __forceinline__ SlotRef slot_ref(const BakeArgs& a, uint32_t p) {
   return r;
 }
 
+// The sample (root) slot of path slot p: itself, or for a branch slot the
+// root recorded when it was spawned.
+__device__ __forceinline__ uint32_t root_slot(const BakeArgs& a, uint32_t p) {
+  return p < a.P ? p : a.link[p - a.P].x;
+}
+
 // Calls f(job record) with the record read wave-uniformly when all active
 // lanes share `job` (the common case: segments are large and contiguous).
 template <class F>
@@ -59,7 +65,7 @@ __device__ __forceinline__ void with_job(const BakeArgs& a, uint32_t job, F&& f)
 
 // Primary ray and texel NEE of path slot p (baker.cpp:245-276,
 // light.cpp:191-293). Returns true when a texel shadow ray was made (stored
-// at shadow slot P + p).
+// at shadow slot Pc + p).
 __device__ __forceinline__ bool generate_path(const BakeArgs& a, const JobDev& J,
                                               const SlotRef& sr, uint32_t p) {
   uint32_t texel = J.texel_list[sr.rank];
@@ -99,7 +105,7 @@ __device__ __forceinline__ bool generate_path(const BakeArgs& a, const JobDev& J
   if (nee_sample<false>(J.scene, P, N, zero, zero, false, rng, nr)) {
     a.nee0[p] = make_float4(nr.L.x, nr.L.y, nr.L.z, 0.f);
     a.nee0_dir[p] = make_float4(nr.dir.x, nr.dir.y, nr.dir.z, as_float((uint32_t)nr.light));
-    size_t so = (size_t)a.P + p;
+    size_t so = (size_t)a.Pc + p;
     a.sray[2 * so] = make_float4(P.x, P.y, P.z, kRayEps);
     a.sray[2 * so + 1] = make_float4(nr.dir.x, nr.dir.y, nr.dir.z, nr.tfar);
     return true;
@@ -108,16 +114,36 @@ __device__ __forceinline__ bool generate_path(const BakeArgs& a, const JobDev& J
   return false;
 }
 
+// Appends a wave's spawning lanes to the batch's branch slots (one atomic
+// per wave); kNoSlot when the spare slots have run out.
+__device__ __forceinline__ uint32_t claim_branch_slot(const BakeArgs& a) {
+#if defined(SHB_HOST_ONLY)
+  const uint32_t k = (*a.spawn_count)++;
+#else
+  const uint64_t m = __ballot(true);
+  const int leader = __ffsll((unsigned long long)m) - 1;
+  const uint32_t lane = __lane_id();
+  uint32_t base = 0;
+  if ((int)lane == leader) base = atomicAdd(a.spawn_count, (uint32_t)__popcll(m));
+  base = __shfl(base, leader);
+  const uint32_t k = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+#endif
+  return k < a.S ? k : kNoSlot;
+}
+
 // Shading of path slot p's closest hit at depth `depth` (baker.cpp:94-146):
-// sky, emission at depth 0, vertex NEE (shadow slot p), cosine-weighted
-// continuation. Sets shadow / extend when a ray was made.
+// sky, alpha split, emission at depth 0, vertex NEE (shadow slot p),
+// cosine-weighted continuation. Sets shadow / extend when a ray was made and
+// `child` to the branch slot of a transmitted ray (kNoSlot: none). `sr` is
+// the root slot's texel and sample.
 // ADJ: records the vertex's material for the replay adjoint and adds the
 // gradient of the depth-0 emission term to `acc`.
 template <bool ADJ>
 __device__ __forceinline__ void shade_path(const BakeArgs& a, const JobDev& J, const SlotRef& sr,
                                            int depth, uint32_t p, bool& shadow, bool& extend,
-                                           GradAcc* acc) {
+                                           uint32_t& child, GradAcc* acc) {
   shadow = extend = false;
+  child = kNoSlot;
   const SceneView& sc = J.scene;
   float4 h = a.hit[p];
   int id = (int)as_uint(h.y);
@@ -149,36 +175,59 @@ __device__ __forceinline__ void shade_path(const BakeArgs& a, const JobDev& J, c
   f3 thr = f3_of(th4);
   uint32_t branch = as_uint(th4.w);
   uint32_t next_branch = branch;
-  // alpha < 1 (baker.cpp:97-109): the reference splits the path into the
-  // transmitted ray, weight 1 - alpha, and the reflected one, weight alpha.
-  // Here one branch is chosen, transmission with probability 1 - alpha, at
-  // weight 1: the same expectation, one path per slot. Branch ids follow
-  // the split (transmitted 2b + 1, reflected 2b); alpha 0 always transmits.
+  // alpha < 1 (baker.cpp:97-109): the path splits into the transmitted ray
+  // from pos + 0.001 dir, weight 1 - alpha, and the reflected part
+  // (emission, NEE, continuation), weight alpha; alpha 0 keeps only the
+  // transmitted ray. Branch ids: transmitted 2b + 1, reflected continuation
+  // 2b. The transmitted ray goes on in this slot when alpha is 0, else in a
+  // spare slot of the batch; no ray is made past the last bounce
+  // (baker.cpp:73).
   const float alpha = get_alpha(sc, mat, tu, tv);
   if (alpha < 1.0f) {
-    Rng ar = site_rng(J.seed, texel, sr.smp, (uint32_t)depth, branch, kSiteAlpha, J.stream);
-    if (ar.uniform() < 1.0f - alpha) {
-      if (ADJ) a.adj_hist[(size_t)depth * a.P + p] = kHistTransmit;
+    const f3 o = pos + dir * 0.001f;
+    const uint32_t tb = 2u * branch + 1u;
+    if (alpha == 0.0f) {
+      if (ADJ) a.adj_hist[(size_t)depth * a.Pc + p] = kHistTransmit;
       if (depth < (int)J.bounces) {
-        const f3 o = pos + dir * 0.001f;
-        a.thr[p] = make_float4(thr.x, thr.y, thr.z, as_float(2u * branch + 1u));
+        a.thr[p] = make_float4(thr.x, thr.y, thr.z, as_float(tb));
         a.ray[2 * (size_t)p] = make_float4(o.x, o.y, o.z, kRayEps);
         a.ray[2 * (size_t)p + 1] = make_float4(dir.x, dir.y, dir.z, INFINITY);
         extend = true;
       }
       return;
     }
+    if (depth < (int)J.bounces) {
+      const uint32_t k = claim_branch_slot(a);
+      if (k != kNoSlot) {
+        const uint32_t c = a.P + k;
+        const f3 tt = thr * (1.0f - alpha);
+        a.link[k] = make_uint4(root_slot(a, p), p, (uint32_t)depth, 0u);
+        a.thr[c] = make_float4(tt.x, tt.y, tt.z, as_float(tb));
+        a.li[c] = make_float4(0.f, 0.f, 0.f, 0.f);
+        a.ray[2 * (size_t)c] = make_float4(o.x, o.y, o.z, kRayEps);
+        a.ray[2 * (size_t)c + 1] = make_float4(dir.x, dir.y, dir.z, INFINITY);
+        if (ADJ) {
+          // the branch carries the parent's vertices before this one
+          a.adj_gvec[c] = a.adj_gvec[p];
+          for (int k2 = 0; k2 < depth; ++k2)
+            a.adj_hist[(size_t)k2 * a.Pc + c] = a.adj_hist[(size_t)k2 * a.Pc + p];
+          a.adj_hist[(size_t)depth * a.Pc + c] = kHistTransmit;
+        }
+        child = c;
+      }
+    }
+    thr = thr * alpha;
     next_branch = 2u * branch;
   }
   f3 add = {0.f, 0.f, 0.f};
-  if (ADJ) a.adj_hist[(size_t)depth * a.P + p] = (uint16_t)mat;
+  if (ADJ) a.adj_hist[(size_t)depth * a.Pc + p] = (uint16_t)mat;
   if (depth == 0 && sc.mats[mat].emission > 0.0f) {
     f3 em = get_emission(sc, mat, tu, tv);
     if (info.y >= 0) {
       const float* ls = sc.light_scale + 3 * info.y;
       if (ADJ) {
-        // d/d scale of the emitter: unscaled emission times the projection
-        // weight (the throughput is 1 at depth 0)
+        // d/d scale of the emitter: unscaled emission times throughput (1 at
+        // depth 0, alpha after a split) times the projection weight
         const float4 g = a.adj_gvec[p];
         acc->add(3u * info.y + 0, em.x * thr.x * g.x);
         acc->add(3u * info.y + 1, em.y * thr.y * g.y);
@@ -271,7 +320,7 @@ __device__ __forceinline__ void adjoint_vertex(const BakeArgs& a, const JobDev& 
   for (int ch = 0; ch < 3; ++ch) acc->add(3 * l + ch, G[ch] / s[ch]);
   const uint32_t mbase = 3 * a.adj_nlights;
   for (int k = 0; k <= depth; ++k) {
-    const uint32_t m = a.adj_hist[(size_t)k * a.P + p];
+    const uint32_t m = a.adj_hist[(size_t)k * a.Pc + p];
     if (m == kHistTransmit) continue;  // a transmitted vertex has no albedo factor
     const float* as = J.scene.albedo_scale + 3 * m;
     for (int ch = 0; ch < 3; ++ch) acc->add(mbase + 3 * m + ch, G[ch] / as[ch]);

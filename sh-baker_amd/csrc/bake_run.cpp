@@ -110,11 +110,17 @@ struct Rec {
   }
 };
 
-}  // namespace
+// Spare path slots of one batch at most.
+constexpr uint64_t kMaxSpareSlots = 1ull << 30;
 
-int run_bake(Ctx& x, std::vector<JobIn>& jobs, bool graph, BakeRun* run, const AdjIn* adj) {
+// One pass of the bake. With transparent materials in some job every batch
+// gets S spare path slots for the transmitted rays of alpha < 1 splits:
+// at least `spare_min`, the context's hint and an eighth of the batch;
+// *S_out and *demand (the most branch slots any batch asked for) tell the
+// caller whether some were dropped.
+int run_pass(Ctx& x, std::vector<JobIn>& jobs, bool graph, BakeRun* run, const AdjIn* adj,
+             uint64_t spare_min, uint32_t* S_out, uint32_t* demand) {
   SHB_TRY(x, hipSetDevice(x.device));
-  if (adj && jobs.size() != 1) return fail(x, SHB_ERR_INVALID_ARG, "the adjoint runs one job");
   hipStream_t st = x.stream;
   const uint32_t nj = (uint32_t)jobs.size();
   const bool count = (jobs[0].cfg.flags & SHB_BAKE_COUNT_VISITS) != 0;
@@ -200,20 +206,38 @@ int run_bake(Ctx& x, std::vector<JobIn>& jobs, bool graph, BakeRun* run, const A
     }
     Pmax = std::max(Pmax, m.slot_base[m.nseg]);
   }
+  bool split = false;
+  for (const JobIn& j : jobs) split = split || j.c->hs.has_transparency;
+  uint32_t S = 0;
+  if (split) {
+    const uint64_t want = std::max<uint64_t>(
+        {spare_min, (uint64_t)x.spare_slots, ((uint64_t)Pmax + 7) / 8, (uint64_t)256});
+    if (want > kMaxSpareSlots || (uint64_t)Pmax * 2 + want > 0xffffffffull)
+      return fail(x, SHB_ERR_LIMIT,
+                  "alpha < 1 splits need more than " + std::to_string(kMaxSpareSlots) +
+                      " spare path slots per batch: lower tuning.batch_paths");
+    S = (uint32_t)want;
+  }
+  *S_out = S;
+  const uint32_t Pcmax = Pmax + S;
   SHB_TRY(x, x.d_maps.reserve(maps.size() * sizeof(BatchMap)));
   SHB_TRY(x, hipMemcpyAsync(x.d_maps.ptr, maps.data(), maps.size() * sizeof(BatchMap),
                             hipMemcpyHostToDevice, st));
 
   // path state: 13 float4 per slot (+1 for the adjoint's projection
-  // weights); queues 4 uint32 per slot
+  // weights), Pc = P + S slots; queues: two extension queues of Pc, the
+  // shadow queue of Pc + P (vertex and texel rays); branch links
   const size_t f4_per_slot = adj ? 14 : 13;
-  SHB_TRY(x, x.d_paths.reserve(std::max<size_t>((size_t)Pmax * f4_per_slot * sizeof(float4), 16)));
+  SHB_TRY(x,
+          x.d_paths.reserve(std::max<size_t>((size_t)Pcmax * f4_per_slot * sizeof(float4), 16)));
   uint32_t maxB_all = 0;
   for (uint32_t v : maxB) maxB_all = std::max(maxB_all, v);
   // adjoint: vertex materials (u16 per depth and slot), visible flags
-  const size_t hist_bytes = ((size_t)(maxB_all + 1) * Pmax * 2 + 15) / 16 * 16;
-  if (adj) SHB_TRY(x, x.d_adj.reserve(hist_bytes + (size_t)Pmax * 4 + 16));
-  SHB_TRY(x, x.d_queues.reserve(std::max<size_t>((size_t)Pmax * 4 * sizeof(uint32_t), 16)));
+  const size_t hist_bytes = ((size_t)(maxB_all + 1) * Pcmax * 2 + 15) / 16 * 16;
+  if (adj) SHB_TRY(x, x.d_adj.reserve(hist_bytes + (size_t)Pcmax * 4 + 16));
+  SHB_TRY(x, x.d_queues.reserve(
+                 std::max<size_t>(((size_t)Pcmax * 3 + Pmax) * sizeof(uint32_t), 16)));
+  if (S) SHB_TRY(x, x.d_link.reserve((size_t)S * sizeof(uint4)));
   SHB_TRY(x, x.d_counters.reserve(std::max<size_t>((size_t)nb * kCntStride * 4, 64)));
   SHB_TRY(x, x.d_work.reserve(kWorkBytes));
   float4* base = x.d_paths.as<float4>();
@@ -221,6 +245,7 @@ int run_bake(Ctx& x, std::vector<JobIn>& jobs, bool graph, BakeRun* run, const A
   unsigned long long* totals = reinterpret_cast<unsigned long long*>(x.d_work.as<uint32_t>() + 4);
   unsigned long long* totals_sh = totals + 128;
   const int sgrid = std::max(1, std::min<int>((int)((Pmax + 255) / 256), x.num_cus * 16));
+  const int mgrid = std::max(1, std::min<int>((int)((S + 255) / 256), x.num_cus * 16));
 
   TraceArgs tbase{};
   tbase.scene.nodes = tnodes;
@@ -259,11 +284,12 @@ int run_bake(Ctx& x, std::vector<JobIn>& jobs, bool graph, BakeRun* run, const A
     SHB_TRY(x, fill(x.d_counters.as<uint32_t>(), (size_t)nb * kCntStride));
     SHB_TRY(x, fill(x.d_work.as<uint32_t>(), kWorkBytes / 4));
     if (adj)
-      SHB_TRY(x, fill(reinterpret_cast<uint32_t*>(x.d_adj.as<uint8_t>() + hist_bytes), Pmax));
+      SHB_TRY(x, fill(reinterpret_cast<uint32_t*>(x.d_adj.as<uint8_t>() + hist_bytes), Pcmax));
     for (uint32_t bi = 0; bi < nb; ++bi) {
       const BatchMap& m = maps[bi];
       uint32_t* cnt = x.d_counters.as<uint32_t>() + (size_t)bi * kCntStride;
       const uint32_t P = m.slot_base[m.nseg];
+      const uint32_t Pc = P + S;
       BakeArgs b{};
       b.jobs = x.d_jobs.as<JobDev>();
       b.map = x.d_maps.as<BatchMap>() + bi;
@@ -273,24 +299,28 @@ int run_bake(Ctx& x, std::vector<JobIn>& jobs, bool graph, BakeRun* run, const A
       b.T0 = m.tex_base[1] - m.tex_base[0];
       b.T = m.tex_base[m.nseg];
       b.P = P;
+      b.S = S;
+      b.Pc = Pc;
+      b.link = S ? x.d_link.as<uint4>() : nullptr;
+      b.spawn_count = cnt + kCntSpawn;
       b.ray = base;
-      b.hit = base + 2 * (size_t)P;
-      b.thr = base + 3 * (size_t)P;
-      b.li = base + 4 * (size_t)P;
-      b.nee0 = base + 5 * (size_t)P;
-      b.nee0_dir = base + 6 * (size_t)P;
-      b.dir0 = base + 7 * (size_t)P;
-      b.sray = base + 8 * (size_t)P;
-      b.scontrib = base + 12 * (size_t)P;
-      uint32_t* qext[2] = {q, q + P};
-      b.shadow_queue = q + 2 * (size_t)P;
+      b.hit = base + 2 * (size_t)Pc;
+      b.thr = base + 3 * (size_t)Pc;
+      b.li = base + 4 * (size_t)Pc;
+      b.nee0 = base + 5 * (size_t)Pc;
+      b.nee0_dir = base + 6 * (size_t)Pc;
+      b.dir0 = base + 7 * (size_t)Pc;
+      b.sray = base + 8 * (size_t)Pc;
+      b.scontrib = base + 12 * (size_t)Pc;
+      uint32_t* qext[2] = {q, q + Pc};
+      b.shadow_queue = q + 2 * (size_t)Pc;
       b.shadow_count = cnt + kCntSh;
       if (adj) {
         b.adj_dsh = adj->dsh;
         b.adj_grad = adj->grad;
         b.adj_nlights = adj->nlights;
         b.adj_nparam = adj->nparam;
-        b.adj_gvec = base + 13 * (size_t)P;
+        b.adj_gvec = base + 13 * (size_t)Pc;
         b.adj_hist = x.d_adj.as<uint16_t>();
         b.adj_vflag = reinterpret_cast<uint32_t*>(x.d_adj.as<uint8_t>() + hist_bytes);
       }
@@ -299,6 +329,8 @@ int run_bake(Ctx& x, std::vector<JobIn>& jobs, bool graph, BakeRun* run, const A
       tb.nseg = m.nseg;
       tb.root0 = m.root[0];
       tb.P = P;
+      tb.Pc = Pc;
+      tb.link = b.link;
       if (r.dry) r.add(2, b);
       else SHB_TRY(x, launch_generate(b, sgrid, st));
       if (!r.dry) SHB_TRY(x, stage_done(st, "generate", bi, -1));
@@ -314,7 +346,7 @@ int run_bake(Ctx& x, std::vector<JobIn>& jobs, bool graph, BakeRun* run, const A
           r.add(3, ta);
         } else {
           trace_ev(0);
-          SHB_TRY(x, launch_trace_hits(ta, b.hit, count, P, x.tuning.block_size, st));
+          SHB_TRY(x, launch_trace_hits(ta, b.hit, count, Pc, x.tuning.block_size, st));
           trace_ev(0);
           ++run->launches_ext;
           SHB_TRY(x, stage_done(st, "extend", bi, (int)d));
@@ -339,15 +371,27 @@ int run_bake(Ctx& x, std::vector<JobIn>& jobs, bool graph, BakeRun* run, const A
         } else {
           trace_ev(1);
           if (adj)
-            SHB_TRY(x, launch_trace_bake_shadow_adj(sa, b, count, 2 * P, x.tuning.block_size, st));
+            SHB_TRY(x,
+                    launch_trace_bake_shadow_adj(sa, b, count, Pc + P, x.tuning.block_size, st));
           else
-            SHB_TRY(x, launch_trace_bake_shadow(sa, b, count, 2 * P, x.tuning.block_size, st));
+            SHB_TRY(x, launch_trace_bake_shadow(sa, b, count, Pc + P, x.tuning.block_size, st));
           trace_ev(1);
           ++run->launches_sh;
           SHB_TRY(x, stage_done(st, "shadow", bi, (int)d));
           if (adj) SHB_TRY(x, launch_adjoint_vertex(b, (int)d, cnt + kCntSh + d, sgrid, st));
         }
       }
+      // branch radiance into the parents, deepest spawn depth first (splits
+      // happen above the last bounce only); the adjoint needs no radiance
+      if (S && !adj)
+        for (int d = (int)maxB[bi] - 1; d >= 0; --d) {
+          if (r.dry) {
+            r.add(7, b);
+            r.add(7, d);
+          } else {
+            SHB_TRY(x, launch_merge_branches(b, d, mgrid, st));
+          }
+        }
       if (r.dry) r.add(6, b);
       else {
         SHB_TRY(x, launch_project(b, st));
@@ -425,8 +469,11 @@ int run_bake(Ctx& x, std::vector<JobIn>& jobs, bool graph, BakeRun* run, const A
   std::vector<uint32_t> hc((size_t)nb * kCntStride);
   if (nb)
     SHB_TRY(x, hipMemcpy(hc.data(), x.d_counters.ptr, hc.size() * 4, hipMemcpyDeviceToHost));
+  *demand = 0;
   for (uint32_t bi = 0; bi < nb; ++bi) {
     const uint32_t* k = hc.data() + (size_t)bi * kCntStride;
+    *demand = std::max(*demand, k[kCntSpawn]);
+    run->branches += std::min(k[kCntSpawn], S);
     run->rays_ext += maps[bi].slot_base[maps[bi].nseg];
     for (uint32_t d = 1; d <= maxB[bi]; ++d) run->rays_ext += k[kCntExt + d];
     for (uint32_t d = 0; d <= maxB[bi]; ++d) run->rays_sh += k[kCntSh + d];
@@ -444,6 +491,32 @@ int run_bake(Ctx& x, std::vector<JobIn>& jobs, bool graph, BakeRun* run, const A
     }
   }
   return SHB_OK;
+}
+
+}  // namespace
+
+int run_bake(Ctx& x, std::vector<JobIn>& jobs, bool graph, BakeRun* run, const AdjIn* adj) {
+  if (adj && jobs.size() != 1) return fail(x, SHB_ERR_INVALID_ARG, "the adjoint runs one job");
+  uint64_t want = 0;
+  for (uint32_t pass = 1;; ++pass) {
+    BakeRun r;
+    uint32_t S = 0, demand = 0;
+    const int rc = run_pass(x, jobs, graph, &r, adj, want, &S, &demand);
+    if (rc) return rc;
+    r.passes = pass;
+    *run = r;
+    if (demand <= S) return SHB_OK;
+    if (S == 0)  // splits need a material with alpha < 255, which sets S
+      return fail(x, SHB_ERR_DEVICE, "alpha split in a scene without transparent texels");
+    // Some transmitted rays found no spare slot (and their own splits were
+    // never asked for): bake again with room for more than the demand seen.
+    // Every path depends only on its counters, so the pass that fits gives
+    // the result any larger pool would.
+    want = std::max<uint64_t>(2ull * S, (uint64_t)demand + demand / 4);
+    x.spare_slots = (uint32_t)std::min<uint64_t>(want, kMaxSpareSlots);
+    if (adj) SHB_TRY(x, launch_fill_u32(reinterpret_cast<uint32_t*>(adj->grad), 0u,
+                                        2 * (size_t)adj->nparam, x.stream));
+  }
 }
 
 }  // namespace shb

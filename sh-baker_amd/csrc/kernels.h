@@ -70,11 +70,15 @@ struct TraceArgs {
   uint32_t* ray_counts;       // optional: node visits, triangle tests per slot
   unsigned long long* totals; // optional: 64 pairs of summed (nodes, tris)
   // root node per ray: root0 when nseg <= 1, else map->root[] of the
-  // segment of (slot mod P)
+  // segment of the ray's root path slot (slot < P: itself; P <= slot < Pc:
+  // a branch slot, root link[slot - P].x; slot >= Pc: a texel shadow ray of
+  // slot - Pc)
   const BatchMap* map;
   uint32_t nseg;
   uint32_t root0;
   uint32_t P;
+  uint32_t Pc;
+  const uint4* link;
 };
 
 struct BakeArgs {
@@ -85,7 +89,15 @@ struct BakeArgs {
   uint32_t texel_begin0;
   uint32_t T0;
   uint32_t T;                 // texels in the batch
-  uint32_t P;                 // path slots in the batch
+  uint32_t P;                 // path slots of the batch's samples (roots)
+  // Branch slots: the transmitted parts of alpha < 1 splits take spare slots
+  // P + k, k < S (Pc = P + S slots in all per-slot arrays); link[k] =
+  // (root slot, parent slot, spawn depth, 0); *spawn_count counts the
+  // branches asked for, so a count above S means some were dropped.
+  uint32_t S;
+  uint32_t Pc;
+  uint4* link;
+  uint32_t* spawn_count;
   float4* ray;                // 2 per slot: extension ray
   float4* hit;                // 1 per slot: (t, id, u, v)
   float4* thr;                // throughput, branch id in w
@@ -93,14 +105,14 @@ struct BakeArgs {
   float4* nee0;               // texel NEE radiance; w = visible
   float4* nee0_dir;           // texel NEE direction (unnormalised)
   float4* dir0;               // primary direction
-  float4* sray;               // 2 per shadow slot; 2P slots (vertex, texel)
+  float4* sray;               // 2 per shadow slot; Pc + P slots (vertex, texel)
   float4* scontrib;           // vertex NEE contribution per slot
   uint32_t* shadow_queue;
   uint32_t* shadow_count;
   // path-replay adjoint (single job): dL/dSH per supersampled texel
   // (27 floats), gradient sums (3 per light, then 3 per material), the
   // per-slot projection weights, the materials of each path's vertices
-  // (hist[depth * P + slot]) and the visible flags of vertex NEE rays
+  // (hist[depth * Pc + slot]) and the visible flags of vertex NEE rays
   const float* adj_dsh;
   double* adj_grad;
   uint32_t adj_nlights;
@@ -140,9 +152,10 @@ struct GradAcc {
   }
 #endif
 };
-constexpr uint32_t kAdjLdsParams = 8192;
+constexpr uint32_t kAdjLdsParams = 8192;  // LDS accumulators up to 32 KB
 // adj_hist entry of a vertex whose path was transmitted (alpha < 1)
-constexpr uint16_t kHistTransmit = 0xFFFF;  // LDS accumulators up to 32 KB
+constexpr uint16_t kHistTransmit = 0xFFFF;
+constexpr uint32_t kNoSlot = 0xffffffffu;
 
 #ifndef SHB_HOST_ONLY
 // Traversal launches: `capacity` is the largest possible queue length (the
@@ -161,6 +174,9 @@ hipError_t launch_shade(const BakeArgs& b, int depth, const uint32_t* in_q,
                         const uint32_t* in_n, uint32_t* out_q, uint32_t* out_n, int grid,
                         hipStream_t st);
 hipError_t launch_project(const BakeArgs& b, hipStream_t st);
+// Adds the radiance of every branch slot spawned at `depth` to its parent's
+// (one launch per depth, deepest first).
+hipError_t launch_merge_branches(const BakeArgs& b, int depth, int grid, hipStream_t st);
 // Adjoint of the visible vertex NEE terms of depth `depth` (shadow queue).
 hipError_t launch_adjoint_vertex(const BakeArgs& b, int depth, const uint32_t* sh_count, int grid,
                                  hipStream_t st);

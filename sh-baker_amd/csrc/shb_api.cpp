@@ -98,7 +98,7 @@ void shb_destroy(shb_ctx* h) {
                     &c.d_mask_a, &c.d_mask_b,   &c.d_rays,    &c.d_hits,     &c.d_hit_out,
                     &c.d_raycounts, &c.d_adj,   &c.d_jobs,    &c.d_maps,     &c.d_cnodes,
                     &c.d_ctris,  &c.d_vcount,  &c.d_dsh,     &c.d_grad,
-                    &c.d_env,    &c.d_env_part};
+                    &c.d_env,    &c.d_env_part, &c.d_link};
   if (c.graph_exec) (void)hipGraphExecDestroy(c.graph_exec);
   c.graph_exec = nullptr;
   for (DevBuf* b : bufs) b->release();

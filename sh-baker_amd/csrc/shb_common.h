@@ -135,9 +135,11 @@ struct SceneView {
 
 // ------------------------------------------------------------ RNG
 // Philox4x32-10 (Salmon et al., SC'11) seeds one xoshiro128** state per path
-// vertex ("site"); counter = (texel, sample | depth<<16 | branch<<20, kind,
-// job), key = 64-bit seed.
-enum : uint32_t { kSiteTexel = 0, kSiteVertex = 1, kSiteAlpha = 2 };
+// vertex ("site"); counter = (texel, sample | depth<<16 | branch<<20,
+// kind | (branch>>12)<<8, job), key = 64-bit seed. The branch id names the
+// vertex's place in the tree of alpha splits (root 1; transmitted 2b + 1,
+// reflected 2b), below 2^16 for 15 bounces.
+enum : uint32_t { kSiteTexel = 0, kSiteVertex = 1 };
 
 SHB_HD void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
 #pragma unroll
@@ -178,7 +180,8 @@ struct Rng {
 SHB_HD Rng site_rng(uint64_t seed, uint32_t texel, uint32_t sample,
                     uint32_t depth, uint32_t branch, uint32_t kind,
                     uint32_t job) {
-  uint32_t c[4] = {texel, sample | (depth << 16) | (branch << 20), kind, job};
+  uint32_t c[4] = {texel, sample | (depth << 16) | (branch << 20), kind | ((branch >> 12) << 8),
+                   job};
   philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
   Rng r{c[0], c[1], c[2], c[3]};
   if ((c[0] | c[1] | c[2] | c[3]) == 0u) r.s0 = 1u;

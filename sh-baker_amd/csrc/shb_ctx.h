@@ -70,6 +70,10 @@ struct Ctx {
   DevBuf d_img_a, d_img_b, d_mask_a, d_mask_b;
   DevBuf d_rays, d_hits, d_hit_out, d_raycounts;
   DevBuf d_adj;               // adjoint accumulators
+  // branch links of alpha < 1 splits; spare slots per batch that the last
+  // bake needed (the next bake starts with at least as many)
+  DevBuf d_link;
+  uint32_t spare_slots = 0;
   // batched / graph bakes run on this context: job table, batch maps,
   // concatenated BVHs of several jobs, the captured launch sequence
   DevBuf d_jobs, d_maps, d_cnodes, d_ctris, d_vcount;

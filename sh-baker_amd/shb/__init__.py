@@ -97,7 +97,8 @@ class ShbBakeStats(C.Structure):
                 ("seconds_trace", C.c_double),
                 ("node_visits_shadow", C.c_uint64), ("tri_tests_shadow", C.c_uint64),
                 ("seconds_trace_shadow", C.c_double),
-                ("launches_extension", C.c_uint32), ("launches_shadow", C.c_uint32)]
+                ("launches_extension", C.c_uint32), ("launches_shadow", C.c_uint32),
+                ("branches", C.c_uint64), ("passes", C.c_uint32), ("reserved", C.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -1,12 +1,13 @@
 """Materials with alpha < 1 (baker.cpp:97-109). The reference splits a path
 at such a hit into the transmitted ray (weight 1 - alpha) and the reflected
-one (weight alpha); the GPU picks one branch, transmission with probability
-1 - alpha, at weight 1 (same expectation, one path per slot; alpha 0 always
-transmits). Checks: the GPU matches the oracle's GPU-stream mode (which
-makes the same choice from the same counter) sample for sample; it agrees
-within Monte Carlo error with the oracle's reference mode (the deterministic
-split); the path-replay adjoint matches central differences; alpha 0
-surfaces are invisible to the bake."""
+part (weight alpha); the GPU does the same, the transmitted ray going on in
+a spare path slot of the batch whose radiance is added to its parent's after
+the last bounce. Checks: the GPU matches the oracle's GPU-stream mode sample
+for sample; it agrees within Monte Carlo error with the oracle's reference
+mode; the bake is a polynomial in alpha and obeys the closed-form weight of
+a twice-transmitted ray; running out of spare slots re-runs the bake with
+the same result; batches that mix jobs and the path-replay adjoint (central
+differences) handle branches."""
 import numpy as np
 import pytest
 
@@ -17,13 +18,14 @@ from shb import scenes
 pytestmark = pytest.mark.gpu
 
 
-def cutout_room(res, alphas):
+def cutout_room(res, alphas, rgb=200):
     """The box room whose central cube carries a 4x4 RGBA texture with the
-    given alpha values (texture UVs from the cube's x, y)."""
+    given alpha values and grey level `rgb` (texture UVs from the cube's
+    x, y)."""
     s = scenes.box_room_scene(res, res)
     a = np.asarray(alphas, np.uint8).reshape(4, 4)
     tex = np.zeros((4, 4, 4), np.uint8)
-    tex[..., :3] = 200
+    tex[..., :3] = rgb
     tex[..., 3] = a
     off = s.texels.size
     s.texels = np.concatenate([s.texels, tex.ravel()])
@@ -134,3 +136,114 @@ def test_adjoint_with_transparency(baker, room):
     fd = (lin(lp, ones_m) - lin(lm, ones_m)) / (2 * h)
     assert abs(dl[0, 1] - fd) <= 1e-2 * abs(fd)
     baker.set_params(ones_l, ones_m)
+
+
+def alpha_weight_residual(f0, f1, fa, a):
+    """Largest deviation from f(a) = f(1) + (1 - a)^2 (f(0) - f(1)), and the
+    spread |f(0) - f(1)|, both relative to the largest |f(0)|. A black cube
+    reflects nothing, so at 2 bounces a path that meets it contributes only
+    through the ray transmitted at both of its faces, weight (1 - a)^2, with
+    the same draws as at alpha 0 (branch ids 3 then 7); every other path is
+    as with the opaque cube. A dropped 1 - a, or a in its place, fails this."""
+    top = max(np.abs(f0).max(), 1e-30)
+    pred = f1 + (1.0 - a) ** 2 * (f0 - f1)
+    return np.abs(fa - pred).max() / top, np.abs(f0 - f1).max() / top
+
+
+def bake_at(baker, s, res, spp, B, seed):
+    """GPU bake of scene s at res x res: (valid texels' SH as float64, stats)."""
+    baker.upload_scene(s)
+    pts = baker.rasterize(res, res, 1)
+    sh, vis, st = baker.bake(spp, B, seed=seed)
+    return sh[pts["material_id"] >= 0].astype(np.float64), st
+
+
+def test_polynomial_in_alpha(baker):
+    # one bounce: a quadratic in a uniform alpha (the traced rays and draws do
+    # not depend on alpha, each vertex weighs alpha or 1 - alpha); third
+    # difference zero up to rounding
+    f0, f1, f2, f3 = [bake_at(baker, cutout_room(32, [a8] * 16), 32, 8, 1, 3)[0]
+                      for a8 in (51, 102, 153, 204)]
+    scale = max(np.abs(f0).max(), np.abs(f3).max())
+    assert np.abs(f0 - 3 * f1 + 3 * f2 - f3).max() <= 2e-5 * scale
+    assert np.abs(f3 - f0).max() > 0.05 * scale
+
+
+def test_transmission_weight_closed_form(baker):
+    f = {a8: bake_at(baker, cutout_room(32, [a8] * 16, rgb=0), 32, 8, 2, 9)[0]
+         for a8 in (0, 255, 77, 179)}
+    for a8 in (77, 179):
+        res, spread = alpha_weight_residual(f[0], f[255], f[a8], a8 / 255.0)
+        assert res < 1e-5, (a8, res)
+        assert spread > 0.01
+
+
+def see_through_room(res):
+    """The box room with every surface at alpha 128/255."""
+    b = scenes.SceneBuilder()
+    g = scenes.linear_to_srgb8(0.7)
+    m = b.add_material((g, g, g, 128))
+    b.add_box((0, 1.5, 0), (2, 1.5, 2.5), material=m, inward=True)
+    b.add_box((0, 1.5, 0), (0.25, 0.25, 0.25), material=m)
+    b.add_light(type=scenes.LIGHT_POINT, position=(0, 2.5, 0), intensity=50.0, color=(1, 1, 1))
+    return b.build(res, res, 2)
+
+
+def test_spare_slots_run_out_and_the_bake_is_redone():
+    # every hit splits: far more branches than the first pass's spare slots
+    # (an eighth of the paths); the second pass has room, later bakes on the
+    # context start with it; any batch size gives the same bits
+    s = see_through_room(48)
+    spp, B, seed = 16, 4, 21
+    b = shb.Baker(0)
+    try:
+        b.upload_scene(s)
+        pts = b.rasterize(48, 48, 1)
+        sh1, vis1, st1 = b.bake(spp, B, seed=seed)
+        P = st1["paths"]
+        first = max((P + 7) // 8, 256)
+        assert st1["branches"] > first, (st1["branches"], first)
+        assert st1["passes"] >= 2
+        sh2, vis2, st2 = b.bake(spp, B, seed=seed)
+        assert st2["passes"] == 1 and st2["branches"] == st1["branches"]
+        assert np.array_equal(sh1.view(np.uint32), sh2.view(np.uint32))
+        assert np.array_equal(vis1.view(np.uint32), vis2.view(np.uint32))
+        b.set_tuning(batch_paths=5000)
+        sh3, vis3, st3 = b.bake(spp, B, seed=seed)
+        assert np.array_equal(sh1.view(np.uint32), sh3.view(np.uint32))
+        assert st3["branches"] == st1["branches"]
+        # and the same paths as the oracle
+        o_sh, _, _, o_st = orc.Oracle(s).bake(pts, 48, 48, spp, B, mode=orc.MODE_GPU, seed=seed)
+        v = pts["material_id"] >= 0
+        scale = np.abs(o_sh[v]).max(axis=1, keepdims=True) + 1e-6
+        assert ((np.abs(sh1[v] - o_sh[v]) / scale).max(axis=1) < 1e-3).mean() > 0.97
+        n_ext = o_st["rays_extension"]
+        assert abs(st1["rays_extension"] - n_ext) <= 1e-3 * n_ext
+    finally:
+        b.close()
+
+
+def test_batches_mixing_jobs_with_branches():
+    # an alpha job and an opaque one in batches that hold both: each comes
+    # out bit-identical to its plain bake
+    specs = [(cutout_room(40, ALPHAS), 40, 16, 3, 5), (scenes.box_room_scene(32, 32), 32, 8, 2, 6)]
+    bakers = []
+    try:
+        for s, res, *_ in specs:
+            b = shb.Baker(0)
+            b.upload_scene(s)
+            b.rasterize(res, res, 1, want=False)
+            bakers.append(b)
+        ref = [b.bake(spp, B, seed=seed)[:2] for b, (_, _, spp, B, seed) in zip(bakers, specs)]
+        for bp in (None, 4000):
+            if bp:
+                bakers[0].set_tuning(batch_paths=bp)
+            sh, vis, st = shb.bake_batched(bakers, [x[2] for x in specs], [x[3] for x in specs],
+                                           [x[4] for x in specs])
+            for k in range(2):
+                assert np.array_equal(sh[k].view(np.uint32), ref[k][0].view(np.uint32)), (bp, k)
+                assert np.array_equal(vis[k].view(np.uint32), ref[k][1].view(np.uint32)), (bp, k)
+            assert st["branches"] > 0
+    finally:
+        for b in bakers:
+            b.close()

@@ -185,7 +185,7 @@ def test_bake_errors(baker):
 
 
 def test_transparent_material_bakes(baker):
-    # alpha < 1 is baked (the stochastic choice of tests/test_gpu_alpha.py)
+    # alpha < 1 is baked (the path split of tests/test_gpu_alpha.py)
     s = scenes.quad_scene()
     s.texels = np.array([255, 255, 255, 128], np.uint8)
     baker.upload_scene(s)

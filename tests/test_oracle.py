@@ -461,15 +461,23 @@ def test_bake_skips_invalid_texels_and_is_deterministic():
     assert (a[pts["material_id"] < 0] == 0).all()
 
 
-def test_stochastic_alpha_choice_matches_the_split():
-    """GPU-stream mode picks one branch at alpha < 1 (transmission with
-    probability 1 - alpha, weight 1); the reference mode sums both with
-    weights 1 - alpha and alpha (baker.cpp:97-109). Same expectation: per
-    texel within Monte Carlo error, and no bias in the mean over texels."""
+def _alpha_tests():
     import sys
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
-    from test_gpu_alpha import ALPHAS, cutout_room
-    s = cutout_room(48, ALPHAS)
+    import test_gpu_alpha
+    return test_gpu_alpha
+
+
+def _cutout(res, alphas, rgb=200):
+    return _alpha_tests().cutout_room(res, alphas, rgb)
+
+
+def test_gpu_stream_split_matches_the_reference_split():
+    """Both modes split a path at alpha < 1 into the transmitted ray (weight
+    1 - alpha) and the reflected part (weight alpha), baker.cpp:97-109, with
+    different random streams: per texel within Monte Carlo error, and no bias
+    in the mean over texels."""
+    s = _cutout(48, _alpha_tests().ALPHAS)
     o = orc.Oracle(s)
     pts = o.rasterize(48, 48, 1)
     v = pts["material_id"] >= 0
@@ -481,3 +489,43 @@ def test_stochastic_alpha_choice_matches_the_split():
     d = (g_sh[v] - r_sh[v]).mean(axis=0)
     se = np.sqrt((r_var[v] + g_var[v]).sum(axis=0)) / v.sum()
     assert (np.abs(d) <= 4 * se + 1e-7).all()
+
+
+@pytest.mark.parametrize("mode", ["gpu", "ref"])
+def test_split_is_a_polynomial_in_alpha(mode):
+    """With the deterministic split the traced rays and every random draw are
+    the same for any 0 < alpha < 1, and each path vertex contributes a factor
+    alpha or 1 - alpha: at one bounce the bake is a quadratic in a uniform
+    alpha, so the third difference over four equally spaced alphas is zero
+    up to rounding. Choosing one branch (piecewise constant in alpha) fails
+    this."""
+    m = orc.MODE_GPU if mode == "gpu" else orc.MODE_REF
+    outs = []
+    for a8 in (51, 102, 153, 204):
+        s = _cutout(32, [a8] * 16)
+        o = orc.Oracle(s)
+        pts = o.rasterize(32, 32, 1)
+        outs.append(o.bake(pts, 32, 32, 8, 1, mode=m, seed=3)[0].astype(np.float64))
+    v = pts["material_id"] >= 0
+    f0, f1, f2, f3 = (x[v] for x in outs)
+    d3 = f0 - 3 * f1 + 3 * f2 - f3
+    d1 = np.abs(f3 - f0)
+    scale = max(np.abs(f0).max(), np.abs(f3).max())
+    assert np.abs(d3).max() <= 2e-5 * scale, np.abs(d3).max() / scale
+    # alpha matters: the bakes differ well beyond that tolerance
+    assert d1.max() > 0.05 * scale
+
+
+def test_transmission_weight_closed_form():
+    """A black cube of uniform alpha at 2 bounces: every texel obeys
+    f(a) = f(1) + (1 - a)^2 (f(0) - f(1)) (alpha_weight_residual says why)."""
+    f = {}
+    for a8 in (0, 255, 77, 179):
+        s = _cutout(32, [a8] * 16, rgb=0)
+        o = orc.Oracle(s)
+        pts = o.rasterize(32, 32, 1)
+        f[a8] = o.bake(pts, 32, 32, 8, 2, mode=orc.MODE_GPU, seed=9)[0].astype(np.float64)
+    for a8 in (77, 179):
+        res, spread = _alpha_tests().alpha_weight_residual(f[0], f[255], f[a8], a8 / 255.0)
+        assert res < 1e-5, res
+        assert spread > 0.01  # the light transmitted through the cube matters
