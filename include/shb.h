@@ -8,8 +8,8 @@
  *
  * Threading: one context per host thread. A context is not safe to use from
  * several threads at once. All GPU work of a context runs on its own stream.
- * Results are deterministic for a given (scene, surface, config, seed) when no
- * material has alpha < 1 (alpha splits add into shared sample accumulators).
+ * Results are deterministic bit for bit for a given (scene, surface, config,
+ * seed), whatever the batch size, launch shapes, BVH width or job packing.
  *
  * Each entry point names the reference interface it replaces
  * (DaviesX/sh-baker, paths relative to the repository root). Host buffers are
