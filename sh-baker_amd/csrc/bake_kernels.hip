@@ -334,8 +334,9 @@ __global__ void k_generate(BakeArgs a) {
 }
 
 // Hit shading at depth d (ADJ: the replay form of the adjoint, with the
-// block's gradient sums in dynamic LDS).
-template <bool ADJ>
+// block's gradient sums in dynamic LDS; SPLIT: batches with spare slots for
+// alpha splits).
+template <bool ADJ, bool SPLIT>
 __global__ void k_shade(BakeArgs a, int depth, const uint32_t* in_queue,
                         const uint32_t* in_count, uint32_t* out_queue,
                         uint32_t* out_count) {
@@ -350,16 +351,16 @@ __global__ void k_shade(BakeArgs a, int depth, const uint32_t* in_queue,
     uint32_t p = 0, child = kNoSlot;
     if (i < n) {
       p = in_queue ? in_queue[i] : i;
-      const SlotRef sr = slot_ref(a, root_slot(a, p));
+      const SlotRef sr = slot_ref(a, SPLIT ? root_slot(a, p) : p);
       with_job(a, sr.job, [&](const JobDev& J) {
-        shade_path<ADJ>(a, J, sr, depth, p, shadow, extend, child, &acc);
+        shade_path<ADJ, SPLIT>(a, J, sr, depth, p, shadow, extend, child, &acc);
       });
     }
     uint32_t at = wave_append(a.shadow_count, shadow);
     if (shadow) a.shadow_queue[at] = p;
     uint32_t at2 = wave_append(out_count, extend);
     if (extend) out_queue[at2] = p;
-    if (a.S) {  // batches with spare slots: transmitted rays of splits
+    if (SPLIT) {  // transmitted rays of splits
       const bool spawned = child != kNoSlot;
       uint32_t at3 = wave_append(out_count, spawned);
       if (spawned) out_queue[at3] = child;
@@ -531,12 +532,22 @@ static size_t adj_lds(const BakeArgs& b) {
 hipError_t launch_shade(const BakeArgs& b, int depth, const uint32_t* in_q,
                         const uint32_t* in_n, uint32_t* out_q, uint32_t* out_n, int grid,
                         hipStream_t st) {
-  if (b.adj_grad)
-    hipLaunchKernelGGL(k_shade<true>, dim3(grid), dim3(256), adj_lds(b), st, b, depth, in_q, in_n,
-                       out_q, out_n);
-  else
-    hipLaunchKernelGGL(k_shade<false>, dim3(grid), dim3(256), 0, st, b, depth, in_q, in_n, out_q,
-                       out_n);
+  const size_t lds = b.adj_grad ? adj_lds(b) : 0;
+  if (b.adj_grad) {
+    if (b.S)
+      hipLaunchKernelGGL((k_shade<true, true>), dim3(grid), dim3(256), lds, st, b, depth, in_q,
+                         in_n, out_q, out_n);
+    else
+      hipLaunchKernelGGL((k_shade<true, false>), dim3(grid), dim3(256), lds, st, b, depth, in_q,
+                         in_n, out_q, out_n);
+  } else {
+    if (b.S)
+      hipLaunchKernelGGL((k_shade<false, true>), dim3(grid), dim3(256), 0, st, b, depth, in_q,
+                         in_n, out_q, out_n);
+    else
+      hipLaunchKernelGGL((k_shade<false, false>), dim3(grid), dim3(256), 0, st, b, depth, in_q,
+                         in_n, out_q, out_n);
+  }
   return hipGetLastError();
 }
 

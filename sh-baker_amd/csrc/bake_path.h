@@ -138,7 +138,10 @@ __device__ __forceinline__ uint32_t claim_branch_slot(const BakeArgs& a) {
 // the root slot's texel and sample.
 // ADJ: records the vertex's material for the replay adjoint and adds the
 // gradient of the depth-0 emission term to `acc`.
-template <bool ADJ>
+// SPLIT: the batch has spare slots (S > 0). Without them no material has a
+// texel with alpha < 255 (the host sets S from the texel pool), every alpha
+// is 1 and the split code is compiled out.
+template <bool ADJ, bool SPLIT>
 __device__ __forceinline__ void shade_path(const BakeArgs& a, const JobDev& J, const SlotRef& sr,
                                            int depth, uint32_t p, bool& shadow, bool& extend,
                                            uint32_t& child, GradAcc* acc) {
@@ -182,8 +185,8 @@ __device__ __forceinline__ void shade_path(const BakeArgs& a, const JobDev& J, c
   // 2b. The transmitted ray goes on in this slot when alpha is 0, else in a
   // spare slot of the batch; no ray is made past the last bounce
   // (baker.cpp:73).
-  const float alpha = get_alpha(sc, mat, tu, tv);
-  if (alpha < 1.0f) {
+  const float alpha = SPLIT ? get_alpha(sc, mat, tu, tv) : 1.0f;
+  if (SPLIT && alpha < 1.0f) {
     const f3 o = pos + dir * 0.001f;
     const uint32_t tb = 2u * branch + 1u;
     if (alpha == 0.0f) {
