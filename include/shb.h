@@ -292,6 +292,16 @@ int shb_trace_rays(shb_ctx* ctx, const shb_ray* rays, uint32_t n,
                    int32_t any_hit, shb_hit* hits, uint32_t* node_visits,
                    uint32_t* tri_tests);
 
+/* The flattened BVH the traversal kernels run on (no reference counterpart;
+ * Embree's tree is internal to the reference): 80-byte compressed 8-wide
+ * nodes (root first) and 48-byte triangles in leaf order (3 x float4,
+ * v0.w = global triangle id as a bit pattern). With NULL buffers only the
+ * counts are written; otherwise *num_nodes / *num_tris give the buffers'
+ * capacities on entry and the counts on return. For host recounts of the
+ * per-ray node visits and triangle tests behind the traversal roofline. */
+int shb_bvh_export(shb_ctx* ctx, void* nodes, uint32_t* num_nodes, void* tris,
+                   uint32_t* num_tris);
+
 /* ---------------------------------------------------------- glTF loader */
 /* Environment kinds of a loaded scene (src/scene.h:90-107). */
 enum {

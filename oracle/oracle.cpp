@@ -17,7 +17,10 @@
 // Traversal: closest hit in [tnear, tfar] by the watertight FP32 test of Woop,
 // Benthin, Wald (JCGT 2013) with ties resolved to the lowest triangle id, over
 // a conservative binned-SAH BVH2 (or brute force); plus an FP64 brute-force
-// Moller-Trumbore for checking. Embree itself is unavailable here.
+// Moller-Trumbore for checking. Embree itself is unavailable here. A second
+// traversal (cw::trace) restates this build's compressed 8-wide BVH walk over
+// the bytes the library exports, to recount per-ray node visits and triangle
+// tests (pinned by a hand-built two-node tree with hand-counted visits).
 //
 // Parity pins (tests/test_oracle.py): SH basis KAT (sh_coeffs_test.cpp:25-37)
 // and FP64 closed form; occlusion KATs (occlusion_test.cpp:37-59,83-111,
@@ -1024,6 +1027,152 @@ using namespace orc;
 // ======================================================================
 // C API for tests (ctypes). Prefix orc_.
 // ======================================================================
+// -------------------------------------- compressed 8-wide BVH (recount)
+// Restates the traversal of this build's own compressed 8-wide BVH (no
+// reference counterpart: Embree's tree is internal to the reference) to
+// recount per-ray node visits and triangle tests, the figures behind the
+// traversal roofline's algorithmic bytes, from the bytes shb_bvh_export
+// returns. Node (80 B): origin p (3 f32), per-axis power-of-two exponents
+// (3 i8), internal-child mask (u8), child base (u32), triangle base (u32),
+// meta[8] (0 empty; 0x80 | rank: internal child child_base + rank; else a
+// leaf of meta >> 5 triangles at tri_base + (meta & 31)), then the 8-bit
+// child planes lo x, lo y, lo z, hi x, hi y, hi z (8 bytes each); child box
+// plane = p + q * 2^e. Groups of children are visited nearest octant first:
+// bit k of a group's hit mask stands for slot k ^ sstar (sstar = the slot
+// opposite the ray's direction octant); the slab test widens each plane
+// distance by 16 ulps of (|b| + 256 |a|) so rounding never culls a box
+// holding a hit. Triangles: 3 x float4, v0.w = the id's bits.
+namespace cw {
+
+static inline float f32_at(const uint8_t* b, int off) {
+  float f;
+  std::memcpy(&f, b + off, 4);
+  return f;
+}
+static inline uint32_t u32_at(const uint8_t* b, int off) {
+  uint32_t u;
+  std::memcpy(&u, b + off, 4);
+  return u;
+}
+static inline float pow2f(int e) {
+  uint32_t u = (uint32_t)(e + 127) << 23;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+static inline float inv_or_big(float d) {
+  if (std::fabs(d) < 1e-20f) return (d < 0.f || (d == 0.f && std::signbit(d))) ? -1e20f : 1e20f;
+  return 1.0f / d;
+}
+
+struct Group {
+  uint32_t base, hits, imask;
+};
+
+static void trace(const uint8_t* nodes, uint32_t nn, const float* tris, const shb_ray& ray,
+                  bool any_hit, shb_hit& h, uint32_t& nv, uint32_t& tt) {
+  std::memset(&h, 0, sizeof(h));
+  h.prim_id = -1;
+  h.geom_id = -1;
+  h.material_id = -1;
+  nv = tt = 0;
+  const V3 o{ray.origin[0], ray.origin[1], ray.origin[2]};
+  const V3 d{ray.direction[0], ray.direction[1], ray.direction[2]};
+  const float sum = o.x + o.y + o.z + d.x + d.y + d.z;
+  const float dd = d.x * d.x + d.y * d.y + d.z * d.z;
+  if (nn == 0 || !((sum - sum) == 0.0f && dd > 0.0f && (dd - dd) == 0.0f)) return;
+  const RayPre rp = ray_pre(o, d);
+  const float inv[3] = {inv_or_big(d.x), inv_or_big(d.y), inv_or_big(d.z)};
+  const bool par[3] = {std::fabs(d.x) < 1e-20f, std::fabs(d.y) < 1e-20f, std::fabs(d.z) < 1e-20f};
+  const bool neg[3] = {d.x < 0.f, d.y < 0.f, d.z < 0.f};
+  const float org[3] = {o.x, o.y, o.z};
+  const uint32_t sstar = ((uint32_t)neg[0] | ((uint32_t)neg[1] << 1) | ((uint32_t)neg[2] << 2)) ^ 7u;
+  float tbest = ray.tfar, bu = 0.f, bv = 0.f;
+  int best = INT_MAX;
+  std::vector<Group> stack;
+  Group g{0u, 1u << sstar, 1u};  // the root: slot 0 of a virtual parent
+  while (true) {
+    if (g.hits == 0u) {
+      if (stack.empty()) break;
+      g = stack.back();
+      stack.pop_back();
+    }
+    const uint32_t k = (uint32_t)__builtin_ctz(g.hits);
+    g.hits &= ~(1u << k);
+    const uint32_t slot = k ^ sstar;
+    const uint32_t ni = g.base + (uint32_t)__builtin_popcount(g.imask & ((1u << slot) - 1u));
+    if (g.hits) stack.push_back(g);
+    const uint8_t* b = nodes + 80 * (size_t)ni;
+    ++nv;
+    float a_[3], bn[3], bf[3];
+    for (int ax = 0; ax < 3; ++ax) {
+      const float sc = pow2f((int)(int8_t)b[12 + ax]);
+      const float a = sc * inv[ax];
+      const float bb = (f32_at(b, 4 * ax) - org[ax]) * inv[ax];
+      const float e = par[ax] ? 0.f : (std::fabs(bb) + 256.0f * std::fabs(a)) * 0x1p-20f;
+      a_[ax] = a;
+      bn[ax] = bb - e;
+      bf[ax] = bb + e;
+    }
+    const uint8_t imask = b[15];
+    const uint8_t* meta = b + 24;
+    const uint8_t* lo[3] = {b + 32, b + 40, b + 48};
+    const uint8_t* hi[3] = {b + 56, b + 64, b + 72};
+    uint32_t hits = 0, tmask = 0;
+    const float tcap = tbest;
+    for (int c = 0; c < 8; ++c) {
+      float tn[3], tf[3];
+      for (int ax = 0; ax < 3; ++ax) {
+        const uint8_t qn = neg[ax] ? hi[ax][c] : lo[ax][c];
+        const uint8_t qf = neg[ax] ? lo[ax][c] : hi[ax][c];
+        tn[ax] = std::fma((float)qn, a_[ax], bn[ax]);
+        tf[ax] = std::fma((float)qf, a_[ax], bf[ax]);
+      }
+      const float tmin = std::fmax(std::fmax(tn[0], tn[1]), std::fmax(tn[2], ray.tnear));
+      const float tmax = std::fmin(std::fmin(std::fmin(tf[0], tf[1]), tf[2]), tcap);
+      const uint32_t m = meta[c];
+      if (m != 0u && tmin <= tmax) {
+        if (m & 0x80u)
+          hits |= 1u << ((uint32_t)c ^ sstar);
+        else
+          tmask |= ((1u << (m >> 5)) - 1u) << (m & 31u);
+      }
+    }
+    g = Group{u32_at(b, 16), hits, imask};
+    const uint32_t tri_base = u32_at(b, 20);
+    bool found = false;
+    while (tmask) {
+      const uint32_t j = (uint32_t)__builtin_ctz(tmask);
+      tmask &= tmask - 1u;
+      const float* tp = tris + 12 * (size_t)(tri_base + j);
+      ++tt;
+      float t, u, v;
+      if (tri_isect(rp, V3{tp[0], tp[1], tp[2]}, V3{tp[4], tp[5], tp[6]},
+                    V3{tp[8], tp[9], tp[10]}, t, u, v)) {
+        uint32_t idb;
+        std::memcpy(&idb, tp + 3, 4);
+        const int id = (int)idb;
+        if (t >= ray.tnear && (t < tbest || (t == tbest && id < best))) {
+          tbest = t;
+          best = id;
+          bu = u;
+          bv = v;
+          found = true;
+        }
+      }
+    }
+    if (any_hit && found) break;
+  }
+  if (best != INT_MAX) {
+    h.prim_id = best;
+    h.t = tbest;
+    h.u = bu;
+    h.v = bv;
+  }
+}
+
+}  // namespace cw
+
 extern "C" {
 
 void orc_philox4x32_10(const uint32_t* ctr, const uint32_t* key, uint32_t* out) {
@@ -1136,6 +1285,17 @@ void orc_scene_destroy(void* s) { delete (Scene*)s; }
 
 // FindOcclusion over a ray list. mode 0: FP32 watertight (BVH if built),
 // 1: FP64 brute force, 2: FP32 any-hit. Optional per-ray counters.
+// Traversal of an exported compressed 8-wide BVH (see cw::trace) over a ray
+// list: closest hit (any_hit 0) or any hit; per-ray node visits and
+// triangle tests.
+void orc_cwbvh_trace(const uint8_t* nodes, uint32_t num_nodes, const float* tris,
+                     const shb_ray* rays, uint32_t n, int any_hit, shb_hit* hits,
+                     uint32_t* node_visits, uint32_t* tri_tests) {
+  for (uint32_t i = 0; i < n; ++i)
+    cw::trace(nodes, num_nodes, tris, rays[i], any_hit != 0, hits[i], node_visits[i],
+              tri_tests[i]);
+}
+
 void orc_trace_rays(void* sp, const shb_ray* rays, uint32_t n, int mode,
                     shb_hit* hits, uint32_t* node_visits, uint32_t* tri_tests) {
   const Scene& s = *(Scene*)sp;

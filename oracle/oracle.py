@@ -48,6 +48,8 @@ def lib():
         L.orc_scene_create.argtypes = [vp, C.c_int]
         L.orc_scene_destroy.argtypes = [vp]
         L.orc_trace_rays.argtypes = [vp, vp, C.c_uint32, C.c_int, vp, u32p, u32p]
+        L.orc_cwbvh_trace.argtypes = [vp, C.c_uint32, vp, vp, C.c_uint32, C.c_int, vp,
+                                      u32p, u32p]
         L.orc_philox4x32_10.argtypes = [u32p, u32p, u32p]
         L.orc_xoshiro128ss.argtypes = [u32p, C.c_int, u32p]
         L.orc_site_uniforms.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32,
@@ -247,3 +249,22 @@ class Oracle:
                        _f(var), C.byref(st))
         stats = {k: getattr(st, k) for k, _ in st._fields_}
         return sh, vis, var, stats
+
+
+def cwbvh_trace(nodes, tris, rays, any_hit=False):
+    """Traversal of an exported compressed 8-wide BVH (nodes (n, 80) uint8,
+    tris (m, 12) float32) over a ray list: (hits, node visits, triangle
+    tests) per ray, recounted on the host."""
+    from shb import HIT_DTYPE, RAY_DTYPE
+    nodes = np.ascontiguousarray(nodes, np.uint8)
+    tris = np.ascontiguousarray(tris, np.float32)
+    rays = np.ascontiguousarray(rays, RAY_DTYPE)
+    hits = np.zeros(rays.size, HIT_DTYPE)
+    nv = np.zeros(rays.size, np.uint32)
+    tt = np.zeros(rays.size, np.uint32)
+    p = C.POINTER(C.c_uint32)
+    lib().orc_cwbvh_trace(nodes.ctypes.data_as(C.c_void_p), nodes.shape[0],
+                          tris.ctypes.data_as(C.c_void_p), rays.ctypes.data_as(C.c_void_p),
+                          rays.size, 1 if any_hit else 0, hits.ctypes.data_as(C.c_void_p),
+                          nv.ctypes.data_as(p), tt.ctypes.data_as(p))
+    return hits, nv, tt

@@ -437,6 +437,24 @@ int shb_dilate(shb_ctx* h, int32_t width, int32_t height, int32_t channels, floa
   return SHB_OK;
 }
 
+int shb_bvh_export(shb_ctx* h, void* nodes, uint32_t* num_nodes, void* tris, uint32_t* num_tris) {
+  if (!h || !num_nodes || !num_tris) return SHB_ERR_INVALID_ARG;
+  Ctx& c = h->c;
+  c.err.clear();
+  if (!c.has_scene) return fail(c, SHB_ERR_NO_SCENE, "no scene uploaded");
+  const uint32_t nn = (uint32_t)c.bvh.nodes.size();
+  const uint32_t nt = (uint32_t)(c.bvh.tris.size() / 3);
+  if ((nodes && *num_nodes < nn) || (tris && *num_tris < nt))
+    return fail(c, SHB_ERR_INVALID_ARG,
+                "buffers hold fewer than " + std::to_string(nn) + " nodes / " +
+                    std::to_string(nt) + " triangles");
+  if (nodes && nn) std::memcpy(nodes, c.bvh.nodes.data(), (size_t)nn * sizeof(CWNode));
+  if (tris && nt) std::memcpy(tris, c.bvh.tris.data(), (size_t)nt * 3 * sizeof(float4));
+  *num_nodes = nn;
+  *num_tris = nt;
+  return SHB_OK;
+}
+
 int shb_trace_rays(shb_ctx* h, const shb_ray* rays, uint32_t n, int32_t any_hit, shb_hit* hits,
                    uint32_t* node_visits, uint32_t* tri_tests) {
   if (!h) return SHB_ERR_INVALID_ARG;

@@ -123,7 +123,7 @@ EXPORTED = [
     "shb_load_gltf", "shb_gltf_scene", "shb_gltf_environment", "shb_gltf_material_name",
     "shb_gltf_albedo_path", "shb_gltf_free", "shb_atlas_geometry_scales", "shb_atlas_create",
     "shb_atlas_scene", "shb_atlas_info", "shb_atlas_free",
-    "shb_downsample", "shb_downsample_mask", "shb_dilate", "shb_trace_rays",
+    "shb_downsample", "shb_downsample_mask", "shb_dilate", "shb_trace_rays", "shb_bvh_export",
     "shb_bake_batched", "shb_autotune", "shb_set_tuning", "shb_get_tuning", "shb_set_params",
     "shb_bake_adjoint",
 ]
@@ -179,6 +179,7 @@ def _load():
     lib.shb_dilate.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, _fp, u8p, C.c_int32]
     lib.shb_trace_rays.argtypes = [vp, vp, C.c_uint32, C.c_int32, vp,
                                    C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+    lib.shb_bvh_export.argtypes = [vp, vp, C.POINTER(C.c_uint32), vp, C.POINTER(C.c_uint32)]
     lib.shb_bake_batched.argtypes = [C.POINTER(vp), C.c_int32, C.POINTER(ShbBakeCfg),
                                      C.POINTER(_fp), C.POINTER(_fp), C.POINTER(ShbBakeStats)]
     lib.shb_autotune.argtypes = [vp, C.POINTER(ShbBakeCfg), C.POINTER(ShbTuning)]
@@ -357,6 +358,18 @@ class Baker:
             nv.ctypes.data_as(p32) if counts else None,
             tt.ctypes.data_as(p32) if counts else None))
         return (hits, nv, tt) if counts else hits
+
+    def bvh_export(self):
+        """The flattened BVH of the uploaded scene as the kernels see it:
+        (nodes: (n, 80) uint8, tris: (m, 12) float32 = 3 float4 per
+        triangle in leaf order, v0.w holding the triangle id's bits)."""
+        nn, nt = C.c_uint32(0), C.c_uint32(0)
+        self._chk(lib.shb_bvh_export(self.h, None, C.byref(nn), None, C.byref(nt)))
+        nodes = np.zeros((nn.value, 80), np.uint8)
+        tris = np.zeros((nt.value, 12), np.float32)
+        self._chk(lib.shb_bvh_export(self.h, nodes.ctypes.data_as(C.c_void_p), C.byref(nn),
+                                     tris.ctypes.data_as(C.c_void_p), C.byref(nt)))
+        return nodes, tris
 
     # -- tuning / params
     def tuning(self):

@@ -146,6 +146,43 @@ def test_bvh_widths_agree(baker):
     baker.set_tuning(bvh_width=8)
 
 
+@pytest.mark.parametrize("which,width", [("c1", 8), ("synth20k", 8), ("synth20k", 4),
+                                         ("c2", 8)])
+def test_visit_counts_equal_host_recount(baker, which, width):
+    # the per-ray node visits and triangle tests behind the roofline's
+    # algorithmic bytes equal a host traversal of the same exported BVH on
+    # the same rays (closest hit and any hit); hits agree bit for bit
+    if which == "c1":
+        s = scenes.box_room_scene()
+    elif which == "synth20k":
+        s = scenes.scaled_synthetic_scene(20000, 256, 7)
+    else:
+        s = scenes.synthetic_room_scene(64, 64)  # the 262k-triangle room
+    baker.upload_scene(s)
+    old = baker.tuning()
+    baker.set_tuning(bvh_width=width)
+    try:
+        nodes, tris = baker.bvh_export()
+        assert nodes.shape[0] >= 1 and tris.shape[0] == sum(
+            g["index_count"] // 3 for g in s.geometries)
+        rays = seeded_rays(s, 20000, 12)
+        hg, nv, tt = baker.trace_rays(rays, counts=True)
+        ho, nvo, tto = orc.cwbvh_trace(nodes, tris, rays)
+        np.testing.assert_array_equal(hg["prim_id"], ho["prim_id"])
+        hit = ho["prim_id"] >= 0
+        assert np.array_equal(hg["t"][hit].view(np.uint32), ho["t"][hit].view(np.uint32))
+        np.testing.assert_array_equal(nv, nvo)
+        np.testing.assert_array_equal(tt, tto)
+        assert nv.mean() > 2 and tt.mean() > 1
+        ha, nva, tta = baker.trace_rays(rays, any_hit=True, counts=True)
+        hao, nvao, ttao = orc.cwbvh_trace(nodes, tris, rays, any_hit=True)
+        np.testing.assert_array_equal(ha["prim_id"] >= 0, hao["prim_id"] >= 0)
+        np.testing.assert_array_equal(nva, nvao)
+        np.testing.assert_array_equal(tta, ttao)
+    finally:
+        baker.set_tuning(**old)
+
+
 def test_visit_counts(baker):
     s = scenes.scaled_synthetic_scene(20000, 256, 7)
     baker.upload_scene(s)

@@ -529,3 +529,69 @@ def test_transmission_weight_closed_form():
         res, spread = _alpha_tests().alpha_weight_residual(f[0], f[255], f[a8], a8 / 255.0)
         assert res < 1e-5, res
         assert spread > 0.01  # the light transmitted through the cube matters
+
+
+def _cw_node(p, e, imask, child_base, tri_base, meta, lo, hi):
+    """80 bytes of a compressed 8-wide node: origin p, exponents e, internal
+    mask, bases, meta[8], planes lo/hi[axis][slot] (child plane = p + q 2^e)."""
+    b = np.zeros(80, np.uint8)
+    b[0:12] = np.frombuffer(np.asarray(p, np.float32).tobytes(), np.uint8)
+    b[12:15] = np.asarray(e, np.int8).view(np.uint8)
+    b[15] = imask
+    b[16:20] = np.frombuffer(np.uint32(child_base).tobytes(), np.uint8)
+    b[20:24] = np.frombuffer(np.uint32(tri_base).tobytes(), np.uint8)
+    b[24:32] = meta
+    for ax in range(3):
+        b[32 + 8 * ax:40 + 8 * ax] = lo[ax]
+        b[56 + 8 * ax:64 + 8 * ax] = hi[ax]
+    return b
+
+
+def test_cwbvh_recount_on_a_hand_built_tree():
+    """The host recount of the compressed 8-wide traversal on a two-node tree
+    written byte by byte: root with two one-triangle leaves over [0,1]^2 (z
+    near 0.03 and 2.03) and an internal child over [3,4]^3 whose node holds a
+    triangle at z = 3.5; planes on a 1/16 grid. Rays straight down: closest
+    hits, node visits and triangle tests follow from the layout alone."""
+    z = np.zeros(8, np.uint8)
+
+    def planes(boxes):  # boxes per slot: ((x0,y0,z0), (x1,y1,z1)) in 1/16 units
+        lo = [z.copy(), z.copy(), z.copy()]
+        hi = [z.copy(), z.copy(), z.copy()]
+        for s, (a, b) in boxes.items():
+            for ax in range(3):
+                lo[ax][s], hi[ax][s] = a[ax], b[ax]
+        return lo, hi
+
+    lo0, hi0 = planes({0: ((0, 0, 0), (16, 16, 1)), 1: ((0, 0, 32), (16, 16, 33)),
+                       2: ((48, 48, 48), (64, 64, 64))})
+    meta0 = z.copy()
+    meta0[0], meta0[1], meta0[2] = 0x20, 0x21, 0x80
+    root = _cw_node((0, 0, 0), (-4, -4, -4), 0x04, 1, 0, meta0, lo0, hi0)
+    lo1, hi1 = planes({0: ((48, 48, 55), (64, 64, 57))})
+    meta1 = z.copy()
+    meta1[0] = 0x20
+    node1 = _cw_node((0, 0, 0), (-4, -4, -4), 0x00, 0, 2, meta1, lo1, hi1)
+    nodes = np.stack([root, node1])
+    tri = [[(0.1, 0.1, 0.03), (0.9, 0.1, 0.03), (0.1, 0.9, 0.03)],
+           [(0.1, 0.1, 2.03), (0.9, 0.1, 2.03), (0.1, 0.9, 2.03)],
+           [(3.1, 3.1, 3.5), (3.9, 3.1, 3.5), (3.1, 3.9, 3.5)]]
+    tris = np.zeros((3, 12), np.float32)
+    for i, t in enumerate(tri):
+        for k in range(3):
+            tris[i, 4 * k:4 * k + 3] = t[k]
+        tris[i, 3] = np.array([i], np.uint32).view(np.float32)[0]
+    rays = np.zeros(4, RAY_DTYPE)
+    for i, (x, y) in enumerate([(0.3, 0.3), (3.3, 3.3), (5.0, 5.0), (0.3, 0.3)]):
+        rays[i]["origin"] = (x, y, 10.0)
+        rays[i]["direction"] = (0.0, 0.0, -1.0)
+        rays[i]["tnear"] = 0.0
+        rays[i]["tfar"] = 1e30
+    rays[3]["tfar"] = 9.0  # ends above the z = 0.03 triangle
+    h, nv, tt = orc.cwbvh_trace(nodes, tris, rays)
+    assert list(h["prim_id"]) == [1, 2, -1, 1]
+    assert np.allclose(h["t"][[0, 1, 3]], [10 - 2.03, 6.5, 10 - 2.03], rtol=1e-6)
+    assert list(nv) == [1, 2, 1, 1] and list(tt) == [2, 1, 0, 1]
+    ha, nva, tta = orc.cwbvh_trace(nodes, tris, rays, any_hit=True)
+    assert list(ha["prim_id"] >= 0) == [True, True, False, True]
+    assert list(nva) == [1, 2, 1, 1] and list(tta) == [2, 1, 0, 1]
