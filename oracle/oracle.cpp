@@ -602,8 +602,12 @@ struct AreaSample {
 };
 
 // src/light.cpp:16-78 SampleAreaLight given (triangle, u1, u2).
+// gpu_arith: the point as the HIP kernels compute it, v0 + u1 e1 + u2 e2
+// with e1 = v1 - v0, e2 = v2 - v0 and two fused multiply-adds per axis
+// (GPU-stream mode); otherwise the reference's blend.
 static AreaSample SampleAreaLightAt(const Scene& s, const Light& l,
-                                    size_t tri_idx, float u1, float u2) {
+                                    size_t tri_idx, float u1, float u2,
+                                    bool gpu_arith = false) {
   const Geo& geo = s.geos[l.geom];
   size_t num_triangles = geo.idx.size() / 3;
   uint32_t i0 = geo.idx[tri_idx * 3], i1 = geo.idx[tri_idx * 3 + 1],
@@ -615,6 +619,12 @@ static AreaSample SampleAreaLightAt(const Scene& s, const Light& l,
   float w = 1.0f - u1 - u2;
   V3 v0 = geo.v[i0], v1 = geo.v[i1], v2 = geo.v[i2];  // object space (light.cpp:45-49)
   V3 p = w * v0 + u1 * v1 + u2 * v2;
+  if (gpu_arith) {
+    const V3 e1 = v1 - v0, e2 = v2 - v0;
+    p = V3{std::fma(u2, e2.x, std::fma(u1, e1.x, v0.x)),
+           std::fma(u2, e2.y, std::fma(u1, e1.y, v0.y)),
+           std::fma(u2, e2.z, std::fma(u1, e1.z, v0.z))};
+  }
   V3 n{0, 1, 0};
   if (!geo.n.empty()) n = normalized(w * geo.n[i0] + u1 * geo.n[i1] + u2 * geo.n[i2]);
   V2 uv{};
@@ -633,9 +643,13 @@ struct ShadowRay {
 // Unshadowed incident radiance of one light (light.h:34-203) and its pending
 // shadow ray. with_brdf selects the *Radiance<Brdf> form (cos * BRDF applied)
 // used at path vertices; otherwise the *IncomingRadiance form used at texels.
-// gpu_arith: one division per light, x * (1/d) and (1/d)^2 as the HIP
-// kernels compute them (GPU-stream mode); otherwise the reference's
-// per-component divisions.
+// gpu_arith: one division per light, x * (1/d) and (1/d)^2, and the squared
+// distance and N.L as x*x + (y*y + z*z) with the inner terms fused, as the
+// HIP kernels compute them (GPU-stream mode); otherwise the reference's
+// per-component divisions and plain sums.
+static inline float sqnorm_fma(V3 a) { return std::fma(a.x, a.x, std::fma(a.y, a.y, a.z * a.z)); }
+static inline float dot_fma(V3 a, V3 b) { return std::fma(a.x, b.x, std::fma(a.y, b.y, a.z * b.z)); }
+
 static V3 LightRadiance(const Scene& s, const Light& l, const AreaSample* as,
                         V3 P, V3 N, bool with_brdf, const Mat* mat, V2 uv,
                         V3 wo, ShadowRay* ray, bool gpu_arith = false) {
@@ -655,7 +669,7 @@ static V3 LightRadiance(const Scene& s, const Light& l, const AreaSample* as,
     case SHB_LIGHT_POINT:  // light.h:72-109
     case SHB_LIGHT_SPOT: { // light.h:117-158
       V3 L = l.pos - P;
-      float dist_sq = sqnorm(L);
+      float dist_sq = gpu_arith ? sqnorm_fma(L) : sqnorm(L);
       float dist = std::sqrt(dist_sq);
       ray->o = P;
       ray->d = l.pos - P;
@@ -664,7 +678,7 @@ static V3 LightRadiance(const Scene& s, const Light& l, const AreaSample* as,
       const float inv = 1.0f / dist;
       const float inv2 = inv * inv;
       L = gpu_arith ? L * inv : L / dist;
-      float cos_n = dot(N, L);
+      float cos_n = gpu_arith ? dot_fma(N, L) : dot(N, L);
       V3 rad;
       if (l.type == SHB_LIGHT_POINT) {
         if (cos_n < 0.f) return V3{};
@@ -682,7 +696,7 @@ static V3 LightRadiance(const Scene& s, const Light& l, const AreaSample* as,
     }
     case SHB_LIGHT_AREA: {  // light.h:166-203
       V3 L = as->point - P;
-      float dist_sq = sqnorm(L);
+      float dist_sq = gpu_arith ? sqnorm_fma(L) : sqnorm(L);
       float dist = std::sqrt(dist_sq);
       ray->o = P;
       ray->d = as->point - P;
@@ -690,7 +704,7 @@ static V3 LightRadiance(const Scene& s, const Light& l, const AreaSample* as,
       ray->tfar = dist - 0.001f;
       const float inv = 1.0f / dist;
       L = gpu_arith ? L * inv : L / dist;
-      float cos_n = dot(N, L);
+      float cos_n = gpu_arith ? dot_fma(N, L) : dot(N, L);
       if (cos_n < 0.f) return V3{};
       V3 rad = gpu_arith ? as->radiance * (inv * inv) : as->radiance / (dist * dist);
       if (!with_brdf) return rad;
@@ -865,10 +879,10 @@ static V3 NeeGpu(const Scene& s, V3 P, V3 N, V3 wo, const Mat* mat, V2 uv,
     if (l.type == SHB_LIGHT_AREA) {
       if (area_valid(s, l)) {
         uint32_t nt = (uint32_t)(s.geos[l.geom].idx.size() / 3);
-        uint32_t tri = x.index(nt);
+        uint32_t tri = nt > 1 ? x.index(nt) : 0u;  // one triangle: no draw
         float u1 = x.uniform();
         float u2 = x.uniform();
-        AreaSample as = SampleAreaLightAt(s, l, tri, u1, u2);
+        AreaSample as = SampleAreaLightAt(s, l, tri, u1, u2, true);
         rd = LightRadiance(s, l, &as, P, N, !texel, mat, uv, wo, &ray, true);
         ap = as.pdf;
       } else {

@@ -98,19 +98,29 @@ struct NeeResult {
 };
 
 // Point on an area-light triangle at folded barycentrics (u1, u2)
-// (light.cpp:34-68): position, area pdf and emission texture coordinates.
+// (light.cpp:34-68): position v0 + u1 e1 + u2 e2 (fused; the reference
+// blends w v0 + u1 v1 + u2 v2, the same point up to rounding), area pdf
+// and emission texture coordinates.
 __device__ __forceinline__ void area_point(const AreaTri& at, float u1, float u2, f3& target,
                                            float& ap, float& tu, float& tv) {
-  float w = 1.0f - u1 - u2;
-  f3 v0 = {at.v0[0], at.v0[1], at.v0[2]};
-  f3 v1 = {at.v1[0], at.v1[1], at.v1[2]};
-  f3 v2 = {at.v2[0], at.v2[1], at.v2[2]};
-  target = w * v0 + u1 * v1 + u2 * v2;
+  target = {fmaf(u2, at.e2[0], fmaf(u1, at.e1[0], at.v0[0])),
+            fmaf(u2, at.e2[1], fmaf(u1, at.e1[1], at.v0[1])),
+            fmaf(u2, at.e2[2], fmaf(u1, at.e1[2], at.v0[2]))};
   ap = at.pdf;
   if (at.has_uv != 0.f) {
+    float w = 1.0f - u1 - u2;
     tu = w * at.uv[0] + u1 * at.uv[2] + u2 * at.uv[4];
     tv = w * at.uv[1] + u1 * at.uv[3] + u2 * at.uv[5];
   }
+}
+
+// Fused forms of the light loop: x.x^2 + (x.y^2 + x.z^2) and a.b as
+// x*x + (y*y + z*z) with the inner terms fused.
+__device__ __forceinline__ float sqnorm_fma(f3 a) {
+  return fmaf(a.x, a.x, fmaf(a.y, a.y, a.z * a.z));
+}
+__device__ __forceinline__ float dot_fma(f3 a, f3 b) {
+  return fmaf(a.x, b.x, fmaf(a.y, b.y, a.z * b.z));
 }
 
 // One NEE sample over all lights (light.cpp:82-189 at a path vertex when
@@ -123,8 +133,10 @@ __device__ __forceinline__ void area_point(const AreaTri& at, float u1, float u2
 // Selection weights use the radiance without the light / albedo scales of
 // shb_set_params, so the selection law does not depend on the fitted
 // parameters (with every scale 1 this is the reference's law exactly); the
-// returned radiance carries the scales.
-template <bool VERTEX>
+// returned radiance carries the scales. UNIT: every scale is 1, so the
+// scaled and unscaled values are equal bit for bit (x * 1 == x, and brdf ==
+// brdf_w) and the scale products are skipped.
+template <bool VERTEX, bool UNIT>
 __device__ __forceinline__ bool nee_sample(const SceneView& sc, f3 P, f3 N, f3 brdf, f3 brdf_w,
                                            bool front, Rng& rng, NeeResult& out) {
   float W = 0.f;
@@ -154,7 +166,8 @@ __device__ __forceinline__ bool nee_sample(const SceneView& sc, f3 P, f3 N, f3 b
           ap = 0.f;
           target = f3{0.f, 0.f, 0.f};
         } else {
-          uint32_t tri = rng.index(L.num_tris);
+          // single-triangle emitters draw no triangle index
+          uint32_t tri = L.num_tris > 1 ? rng.index(L.num_tris) : 0u;
           float u1 = rng.uniform();
           float u2 = rng.uniform();
           if (u1 + u2 > 1.0f) {
@@ -176,17 +189,17 @@ __device__ __forceinline__ bool nee_sample(const SceneView& sc, f3 P, f3 N, f3 b
         target = f3{L.pos[0], L.pos[1], L.pos[2]};
       }
       f3 Lv = target - P;
-      float dist_sq = sqnorm(Lv);
+      float dist_sq = sqnorm_fma(Lv);
       float dist = sqrtf(dist_sq);
       sdir = target - P;
       tfar = dist - 0.001f;
       // one division per light (the reference divides per component:
-      // light.h:82-93, 127-143, 176-188); the oracle's GPU-stream mode
-      // follows this arithmetic
+      // light.h:82-93, 127-143, 176-188) and fused sums; the oracle's
+      // GPU-stream mode follows this arithmetic
       const float inv = 1.0f / dist;
       const float inv2 = inv * inv;
       Lv = Lv * inv;
-      cos_n = dot(N, Lv);
+      cos_n = dot_fma(N, Lv);
       if (type == kLightSpot) {
         f3 ld = {L.dir[0], L.dir[1], L.dir[2]};
         float cos_l = dot(ld, -Lv);
@@ -205,13 +218,20 @@ __device__ __forceinline__ bool nee_sample(const SceneView& sc, f3 P, f3 N, f3 b
     if (VERTEX && !(cos_n < 0.f)) {
       bool ok = front && cos_n > 0.0f;
       f3 bv = ok ? brdf : f3{0.f, 0.f, 0.f};
-      f3 bw = ok ? brdf_w : f3{0.f, 0.f, 0.f};
-      radw = cmul(rad, cos_n * bw);
-      rad = cmul(rad, cos_n * bv);
+      if (UNIT) {
+        rad = cmul(rad, cos_n * bv);
+        radw = rad;
+      } else {
+        f3 bw = ok ? brdf_w : f3{0.f, 0.f, 0.f};
+        radw = cmul(rad, cos_n * bw);
+        rad = cmul(rad, cos_n * bv);
+      }
     }
-    const uint32_t l3 = 3u * (uint32_t)l;
-    rad = cmul(rad, f3{uniform_load(sc.light_scale, l3), uniform_load(sc.light_scale, l3 + 1),
-                       uniform_load(sc.light_scale, l3 + 2)});
+    if (!UNIT) {
+      const uint32_t l3 = 3u * (uint32_t)l;
+      rad = cmul(rad, f3{uniform_load(sc.light_scale, l3), uniform_load(sc.light_scale, l3 + 1),
+                         uniform_load(sc.light_scale, l3 + 2)});
+    }
     float w = maxc(radw);
     float u = rng.uniform();
     W += w;

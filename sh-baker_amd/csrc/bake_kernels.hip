@@ -183,6 +183,114 @@ __global__ void __launch_bounds__(256) k_trace(TraceArgs a, Epi epi) {
   trace_finish<COUNT>(a, epi, slot, i, s, overflow, capped, steps);
 }
 
+// Triangle tests of one traversal step spread over the wave. After every
+// lane has decoded its node, the (owner lane, triangle) pairs of the whole
+// wave go into a list in LDS (prefix sum of the per-lane counts by ballots)
+// and each lane tests one pair per round, with the owner's ray fetched by
+// cross-lane shuffles: about ceil(K / 64) rounds for K pairs instead of the
+// largest per-lane count. Each owner's closest hit is the minimum of the
+// 64-bit keys (t, id), the closest-hit rule itself, taken with LDS atomics;
+// the winning lane then writes its t, u, v. The result equals the lane's own
+// sequential loop (trav_tris): same accepted set, same minimum.
+constexpr uint32_t kWavePairs = 256;  // K above this: each lane loops alone
+
+struct WaveTriLds {
+  float4 hit[64];                // owner's winning (t, u, v)
+  unsigned long long key[64];    // owner's best key
+  uint32_t pair[kWavePairs];     // owner lane << 26 | leaf-order triangle
+};
+
+__device__ __forceinline__ void wave_sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Every lane of the wave calls this (active: the lane decoded a node this
+// step). Returns ANY && a hit was taken, as trav_tris does.
+template <bool ANY, bool COUNT>
+__device__ __forceinline__ bool wave_tris(const SceneView& sc, const TravRay& r, TravState& s,
+                                          bool active, uint32_t tri_base, uint32_t tmask,
+                                          WaveTriLds* L) {
+  const uint32_t lane = lane_id();
+  const uint32_t k = active ? (uint32_t)__builtin_popcount(tmask) : 0u;
+  if (COUNT) s.tris += k;
+  const uint64_t below = (1ull << lane) - 1ull;
+  uint32_t off = 0, K = 0;
+#pragma unroll
+  for (int b = 0; b < 6; ++b) {  // k <= 32
+    const uint64_t m = __ballot(((k >> b) & 1u) != 0u);
+    off += (uint32_t)__popcll(m & below) << b;
+    K += (uint32_t)__popcll(m) << b;
+  }
+  if (K == 0u) return false;
+  if (K > kWavePairs) {
+    bool f = false;
+    if (active) f = trav_tris<ANY, false>(sc, r, s, tri_base, tmask);
+    return f;
+  }
+  const unsigned long long init = hit_key(s.tbest, s.best);
+  uint32_t tm = k ? tmask : 0u, q = off;
+  while (tm) {
+    const uint32_t j = __builtin_ctz(tm);
+    tm &= tm - 1u;
+    L->pair[q++] = (lane << 26) | (tri_base + j);
+  }
+  if (k) L->key[lane] = init;
+  wave_sync_lds();
+  const int kxyz = r.kx | (r.ky << 2) | (r.kz << 4);
+  for (uint32_t base = 0; base < K; base += 64u) {
+    const uint32_t qi = base + lane;
+    const bool have = qi < K;
+    const uint32_t pr = have ? L->pair[qi] : 0u;
+    const int o = (int)(pr >> 26);
+    TravRay ro;
+    ro.ox = __shfl(r.ox, o);
+    ro.oy = __shfl(r.oy, o);
+    ro.oz = __shfl(r.oz, o);
+    ro.Sx = __shfl(r.Sx, o);
+    ro.Sy = __shfl(r.Sy, o);
+    ro.Sz = __shfl(r.Sz, o);
+    const int kk = __shfl(kxyz, o);
+    ro.kx = kk & 3;
+    ro.ky = (kk >> 2) & 3;
+    ro.kz = (kk >> 4) & 3;
+    const float tnear_o = __shfl(r.tnear, o);
+    const float tbest_o = __shfl(s.tbest, o);
+    const int best_o = __shfl(s.best, o);
+    unsigned long long key = ~0ull;
+    float t = 0.f, u = 0.f, v = 0.f;
+    if (have) {
+      const float4* tp = sc.tris + 3 * (size_t)(pr & 0x3ffffffu);
+      const float4 a = tp[0], b = tp[1], c = tp[2];
+      if (tri_test(ro, a, b, c, t, u, v)) {
+        const int id = (int)as_uint(a.w);
+        if (better_hit(t, id, tnear_o, tbest_o, best_o)) {
+          key = hit_key(t, id);
+          atomicMin(&L->key[o], key);
+        }
+      }
+    }
+    wave_sync_lds();
+    if (key != ~0ull && L->key[o] == key) L->hit[o] = make_float4(t, u, v, 0.f);
+    wave_sync_lds();
+  }
+  bool found = false;
+  if (k) {
+    const unsigned long long fin = L->key[lane];
+    if (fin != init) {
+      const float4 h = L->hit[lane];
+      s.tbest = h.x;
+      s.best = (int)(uint32_t)fin;
+      s.bu = h.y;
+      s.bv = h.z;
+      found = true;
+    }
+  }
+  wave_sync_lds();  // the next step's list reuses the area
+  return ANY && found;
+}
+
 // Persistent form: every wave keeps up to 64 rays in flight. When at least
 // a.refill lanes are idle they take the next entries of the wave's pool, a
 // range of kFetch queue positions claimed with one atomic on a.head, so lanes
@@ -194,7 +302,11 @@ constexpr uint32_t kFetch = 128;  // >= 64: one claim serves every idle lane
 
 template <bool ANY, bool COUNT, class Epi>
 __global__ void __launch_bounds__(256) k_trace_persistent(TraceArgs a, Epi epi) {
-  extern __shared__ uint2 stack_mem[];  // a.stack_entries * blockDim.x
+  // a.stack_entries * blockDim.x stack entries, then (with a.wave_tris) one
+  // WaveTriLds per wave of the block
+  extern __shared__ __align__(16) uint2 stack_mem[];
+  WaveTriLds* tri_lds = reinterpret_cast<WaveTriLds*>(stack_mem + a.stack_entries * blockDim.x) +
+                        threadIdx.x / 64u;
   const uint32_t count = a.count_ptr ? *a.count_ptr : a.count_const;
   const uint32_t lane = lane_id();
   const uint64_t lanes_below = (1ull << lane) - 1ull;
@@ -253,7 +365,25 @@ __global__ void __launch_bounds__(256) k_trace_persistent(TraceArgs a, Epi epi) 
       if (drained && pool_next >= pool_end) break;
       continue;  // all idle, more work claimable: the next pass refills
     }
-    if (has) {
+    if (a.wave_tris) {
+      // node half per lane, then the wave's triangle tests together
+      uint32_t tb = 0u, tmk = 0u;
+      bool done = false;
+      if (has) done = trav_node<COUNT>(a.scene, r, s, stack, overflow, tb, tmk);
+      const bool found = wave_tris<ANY, COUNT>(a.scene, r, s, has && !done, tb, tmk, tri_lds);
+      if (has) {
+        if (found) done = true;
+        bool capped = false;
+        if (!done && ++steps > a.max_steps) {
+          capped = true;
+          done = true;
+        }
+        if (done) {
+          trace_finish<COUNT>(a, epi, slot, qi, s, overflow, capped, steps);
+          has = false;
+        }
+      }
+    } else if (has) {
       bool done = trav_step<ANY, COUNT>(a.scene, r, s, stack, overflow);
       bool capped = false;
       if (!done && ++steps > a.max_steps) {
@@ -318,7 +448,9 @@ __global__ void k_hit_details(SceneView sc, const float4* rays, const float4* hi
 }
 
 // ------------------------------------------------------------ bake stages
-// Primary ray + texel NEE for every path slot of the batch.
+// Primary ray + texel NEE for every path slot of the batch (UNIT: every
+// light and albedo scale is 1).
+template <bool UNIT>
 __global__ void k_generate(BakeArgs a) {
   const uint32_t n = a.P;
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
@@ -326,7 +458,7 @@ __global__ void k_generate(BakeArgs a) {
     bool shadow = false;
     if (p < n) {
       const SlotRef sr = slot_ref(a, p);
-      with_job(a, sr.job, [&](const JobDev& J) { shadow = generate_path(a, J, sr, p); });
+      with_job(a, sr.job, [&](const JobDev& J) { shadow = generate_path<UNIT>(a, J, sr, p); });
     }
     uint32_t at = wave_append(a.shadow_count, shadow);
     if (shadow) a.shadow_queue[at] = a.Pc + p;
@@ -335,8 +467,8 @@ __global__ void k_generate(BakeArgs a) {
 
 // Hit shading at depth d (ADJ: the replay form of the adjoint, with the
 // block's gradient sums in dynamic LDS; SPLIT: batches with spare slots for
-// alpha splits).
-template <bool ADJ, bool SPLIT>
+// alpha splits; UNIT: every light and albedo scale is 1).
+template <bool ADJ, bool SPLIT, bool UNIT>
 __global__ void k_shade(BakeArgs a, int depth, const uint32_t* in_queue,
                         const uint32_t* in_count, uint32_t* out_queue,
                         uint32_t* out_count) {
@@ -353,7 +485,7 @@ __global__ void k_shade(BakeArgs a, int depth, const uint32_t* in_queue,
       p = in_queue ? in_queue[i] : i;
       const SlotRef sr = slot_ref(a, SPLIT ? root_slot(a, p) : p);
       with_job(a, sr.job, [&](const JobDev& J) {
-        shade_path<ADJ, SPLIT>(a, J, sr, depth, p, shadow, extend, child, &acc);
+        shade_path<ADJ, SPLIT, UNIT>(a, J, sr, depth, p, shadow, extend, child, &acc);
       });
     }
     uint32_t at = wave_append(a.shadow_count, shadow);
@@ -464,9 +596,10 @@ static hipError_t launch_trace_t(const TraceArgs& a, const Epi& e, uint32_t capa
                                  hipStream_t st) {
   if (capacity == 0) return hipSuccess;
   if (a.stack_entries < 1 || a.stack_entries > (uint32_t)kStackLds) return hipErrorInvalidValue;
-  const size_t lds = (size_t)a.stack_entries * block * sizeof(uint2);
+  size_t lds = (size_t)a.stack_entries * block * sizeof(uint2);
   uint32_t grid = (capacity + block - 1) / block;
   if (a.head && a.refill >= 1 && a.refill <= 64) {
+    if (a.wave_tris) lds += (size_t)(block / 64) * sizeof(WaveTriLds);
     // persistent: as many blocks as fit on the device at once (occupancy of
     // this kernel with its LDS stack), no more than the queue could use
     int per_cu = 0;
@@ -520,7 +653,10 @@ hipError_t launch_hit_details(const SceneView& sc, const float4* rays, const flo
 }
 
 hipError_t launch_generate(const BakeArgs& b, int grid, hipStream_t st) {
-  hipLaunchKernelGGL(k_generate, dim3(grid), dim3(256), 0, st, b);
+  if (b.unit_scales)
+    hipLaunchKernelGGL(k_generate<true>, dim3(grid), dim3(256), 0, st, b);
+  else
+    hipLaunchKernelGGL(k_generate<false>, dim3(grid), dim3(256), 0, st, b);
   return hipGetLastError();
 }
 
@@ -533,21 +669,21 @@ hipError_t launch_shade(const BakeArgs& b, int depth, const uint32_t* in_q,
                         const uint32_t* in_n, uint32_t* out_q, uint32_t* out_n, int grid,
                         hipStream_t st) {
   const size_t lds = b.adj_grad ? adj_lds(b) : 0;
-  if (b.adj_grad) {
-    if (b.S)
-      hipLaunchKernelGGL((k_shade<true, true>), dim3(grid), dim3(256), lds, st, b, depth, in_q,
-                         in_n, out_q, out_n);
-    else
-      hipLaunchKernelGGL((k_shade<true, false>), dim3(grid), dim3(256), lds, st, b, depth, in_q,
-                         in_n, out_q, out_n);
-  } else {
-    if (b.S)
-      hipLaunchKernelGGL((k_shade<false, true>), dim3(grid), dim3(256), 0, st, b, depth, in_q,
-                         in_n, out_q, out_n);
-    else
-      hipLaunchKernelGGL((k_shade<false, false>), dim3(grid), dim3(256), 0, st, b, depth, in_q,
-                         in_n, out_q, out_n);
+#define SHB_SHADE(A, S, U)                                                                 \
+  hipLaunchKernelGGL((k_shade<A, S, U>), dim3(grid), dim3(256), lds, st, b, depth, in_q, in_n, \
+                     out_q, out_n)
+  const int v = (b.adj_grad ? 4 : 0) | (b.S ? 2 : 0) | (b.unit_scales ? 1 : 0);
+  switch (v) {
+    case 0: SHB_SHADE(false, false, false); break;
+    case 1: SHB_SHADE(false, false, true); break;
+    case 2: SHB_SHADE(false, true, false); break;
+    case 3: SHB_SHADE(false, true, true); break;
+    case 4: SHB_SHADE(true, false, false); break;
+    case 5: SHB_SHADE(true, false, true); break;
+    case 6: SHB_SHADE(true, true, false); break;
+    default: SHB_SHADE(true, true, true); break;
   }
+#undef SHB_SHADE
   return hipGetLastError();
 }
 

@@ -65,7 +65,8 @@ __device__ __forceinline__ void with_job(const BakeArgs& a, uint32_t job, F&& f)
 
 // Primary ray and texel NEE of path slot p (baker.cpp:245-276,
 // light.cpp:191-293). Returns true when a texel shadow ray was made (stored
-// at shadow slot Pc + p).
+// at shadow slot Pc + p). UNIT: every light and albedo scale is 1.
+template <bool UNIT>
 __device__ __forceinline__ bool generate_path(const BakeArgs& a, const JobDev& J,
                                               const SlotRef& sr, uint32_t p) {
   uint32_t texel = J.texel_list[sr.rank];
@@ -102,7 +103,7 @@ __device__ __forceinline__ bool generate_path(const BakeArgs& a, const JobDev& J
   a.li[p] = make_float4(0.f, 0.f, 0.f, 0.f);
   NeeResult nr;
   f3 zero = {0.f, 0.f, 0.f};
-  if (nee_sample<false>(J.scene, P, N, zero, zero, false, rng, nr)) {
+  if (nee_sample<false, UNIT>(J.scene, P, N, zero, zero, false, rng, nr)) {
     a.nee0[p] = make_float4(nr.L.x, nr.L.y, nr.L.z, 0.f);
     a.nee0_dir[p] = make_float4(nr.dir.x, nr.dir.y, nr.dir.z, as_float((uint32_t)nr.light));
     size_t so = (size_t)a.Pc + p;
@@ -140,8 +141,9 @@ __device__ __forceinline__ uint32_t claim_branch_slot(const BakeArgs& a) {
 // gradient of the depth-0 emission term to `acc`.
 // SPLIT: the batch has spare slots (S > 0). Without them no material has a
 // texel with alpha < 255 (the host sets S from the texel pool), every alpha
-// is 1 and the split code is compiled out.
-template <bool ADJ, bool SPLIT>
+// is 1 and the split code is compiled out. UNIT: every light and albedo
+// scale is 1.
+template <bool ADJ, bool SPLIT, bool UNIT>
 __device__ __forceinline__ void shade_path(const BakeArgs& a, const JobDev& J, const SlotRef& sr,
                                            int depth, uint32_t p, bool& shadow, bool& extend,
                                            uint32_t& child, GradAcc* acc) {
@@ -247,7 +249,7 @@ __device__ __forceinline__ void shade_path(const BakeArgs& a, const JobDev& J, c
   f3 brdf = lambert(sc, mat, tu, tv, &brdf_w);
   Rng rng = site_rng(J.seed, texel, sr.smp, (uint32_t)depth, branch, kSiteVertex, J.stream);
   NeeResult nr;
-  if (nee_sample<true>(sc, hit_pos, nrm, brdf, brdf_w, front, rng, nr)) {
+  if (nee_sample<true, UNIT>(sc, hit_pos, nrm, brdf, brdf_w, front, rng, nr)) {
     f3 c = cmul(thr, nr.L);
     a.scontrib[p] = make_float4(c.x, c.y, c.z, as_float((uint32_t)nr.light));
     a.sray[2 * (size_t)p] = make_float4(hit_pos.x, hit_pos.y, hit_pos.z, kRayEps);

@@ -220,6 +220,12 @@ int run_pass(Ctx& x, std::vector<JobIn>& jobs, bool graph, BakeRun* run, const A
   }
   *S_out = S;
   const uint32_t Pcmax = Pmax + S;
+  // every scale 1 in every job: the kernels skip the scale products
+  bool unit = true;
+  for (const JobIn& j : jobs) {
+    for (float v : j.c->lscale_h) unit = unit && v == 1.0f;
+    for (float v : j.c->ascale_h) unit = unit && v == 1.0f;
+  }
   SHB_TRY(x, x.d_maps.reserve(maps.size() * sizeof(BatchMap)));
   SHB_TRY(x, hipMemcpyAsync(x.d_maps.ptr, maps.data(), maps.size() * sizeof(BatchMap),
                             hipMemcpyHostToDevice, st));
@@ -252,6 +258,7 @@ int run_pass(Ctx& x, std::vector<JobIn>& jobs, bool graph, BakeRun* run, const A
   tbase.scene.tris = ttris;
   tbase.scene.has_nodes = total_nodes ? 1 : 0;
   tbase.has_nodes = total_nodes ? 1 : 0;
+  tbase.wave_tris = total_tris < (1u << 26) ? 1u : 0u;
   tbase.max_steps = (uint32_t)total_nodes + 1;
   tbase.stack_entries = (uint32_t)std::max(1, std::min(depth - 1, kStackLds));
   tbase.refill = (uint32_t)x.tuning.queue_chunk;
@@ -303,6 +310,7 @@ int run_pass(Ctx& x, std::vector<JobIn>& jobs, bool graph, BakeRun* run, const A
       b.Pc = Pc;
       b.link = S ? x.d_link.as<uint4>() : nullptr;
       b.spawn_count = cnt + kCntSpawn;
+      b.unit_scales = unit ? 1u : 0u;
       b.ray = base;
       b.hit = base + 2 * (size_t)Pc;
       b.thr = base + 3 * (size_t)Pc;

@@ -79,6 +79,9 @@ struct TraceArgs {
   uint32_t P;
   uint32_t Pc;
   const uint4* link;
+  // persistent form: test the triangles of a step's nodes spread over the
+  // wave's lanes (needs fewer than 2^26 triangles)
+  uint32_t wave_tris;
 };
 
 struct BakeArgs {
@@ -98,6 +101,7 @@ struct BakeArgs {
   uint32_t Pc;
   uint4* link;
   uint32_t* spawn_count;
+  uint32_t unit_scales;       // every job's light and albedo scales are all 1
   float4* ray;                // 2 per slot: extension ray
   float4* hit;                // 1 per slot: (t, id, u, v)
   float4* thr;                // throughput, branch id in w

@@ -247,8 +247,8 @@ int build_host_scene(const shb_scene_desc* d, HostScene* out, std::string* err) 
             }
           }
           at.v0[0] = v[0].x, at.v0[1] = v[0].y, at.v0[2] = v[0].z;
-          at.v1[0] = v[1].x, at.v1[1] = v[1].y, at.v1[2] = v[1].z;
-          at.v2[0] = v[2].x, at.v2[1] = v[2].y, at.v2[2] = v[2].z;
+          at.e1[0] = v[1].x - v[0].x, at.e1[1] = v[1].y - v[0].y, at.e1[2] = v[1].z - v[0].z;
+          at.e2[0] = v[2].x - v[0].x, at.e2[1] = v[2].y - v[0].y, at.e2[2] = v[2].z - v[0].z;
           float area = std::sqrt(h3sq(h3cross(h3sub(v[0], v[1]), h3sub(v[0], v[2])))) / 2.f;
           at.pdf = std::max(1e-6f, 1.f / (float)nt * 1.f / area);
           at.has_uv = hasUV ? 1.f : 0.f;

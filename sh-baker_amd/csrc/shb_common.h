@@ -87,13 +87,15 @@ struct alignas(16) LightDev {
   int32_t pad;
 };
 
-// Area-light triangle record (object-space vertices, light.cpp:45-49).
+// Area-light triangle record (object-space vertices, light.cpp:45-49): v0
+// and the edges e1 = v1 - v0, e2 = v2 - v0 (float differences), so a sample
+// point is v0 + u1 e1 + u2 e2 with two fused multiply-adds per axis.
 struct alignas(16) AreaTri {
   float v0[3];
   float pdf;           // max(1e-6, 1/ntri * 1/area)  (light.cpp:75-76)
-  float v1[3];
+  float e1[3];
   float has_uv;
-  float v2[3];
+  float e2[3];
   float pad;
   float uv[6];
   float pad2[2];

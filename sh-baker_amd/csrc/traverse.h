@@ -136,14 +136,39 @@ __device__ __forceinline__ void trav_init(TravState& s, const TravRay& r, float 
   s.nodes = s.tris = 0;
 }
 
-// One traversal step: visit one node and test its triangles. Returns true
-// when the ray is finished (stack empty, or any-hit found).
+// Order-preserving 64-bit key of a candidate hit: (t, id) compares as the
+// closest-hit rule does (smaller t first, ties to the lower id). t + 0 maps
+// -0 to +0 so equal t give equal keys.
+__device__ __forceinline__ unsigned long long hit_key(float t, int id) {
+  uint32_t u = as_uint(t + 0.0f);
+  u = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+  return ((unsigned long long)u << 32) | (uint32_t)id;
+}
+__device__ __forceinline__ float key_t(unsigned long long k) {
+  uint32_t u = (uint32_t)(k >> 32);
+  u = (u & 0x80000000u) ? (u & 0x7fffffffu) : ~u;
+  return as_float(u);
+}
+
+// Candidate acceptance of the closest-hit rule (t in [tnear, tbest], ties in
+// t to the lower id).
+__device__ __forceinline__ bool better_hit(float t, int id, float tnear, float tbest, int best) {
+  return t >= tnear && (t < tbest || (t == tbest && id < best));
+}
+
+// First half of a traversal step: pop a group when the current one is
+// empty, visit its nearest child node and decode it; the node's child group
+// becomes current and the triangles to test are returned as (tri_base,
+// tmask). Returns true when the ray has nothing left to visit (tmask 0).
 // Stack: any type with get(int) / set(int, uint2) / cap() (entries; at most
 // kStackLds) — on the GPU a slice of the kernel's dynamic LDS, sized to the
 // uploaded tree's depth, on the host a local array.
-template <bool ANY, bool COUNT, class Stack>
-__device__ __forceinline__ bool trav_step(const SceneView& sc, const TravRay& r,
-                                          TravState& s, Stack& stack, bool& overflow) {
+template <bool COUNT, class Stack>
+__device__ __forceinline__ bool trav_node(const SceneView& sc, const TravRay& r, TravState& s,
+                                          Stack& stack, bool& overflow, uint32_t& tri_base,
+                                          uint32_t& tmask_out) {
+  tmask_out = 0u;
+  tri_base = 0u;
   if ((s.gbits & 0xffu) == 0u) {
     if (s.sp == 0) return true;
     --s.sp;
@@ -222,14 +247,22 @@ __device__ __forceinline__ bool trav_step(const SceneView& sc, const TravRay& r,
         tmask |= ((1u << (m >> 5)) - 1u) << (m & 31u);
     }
   }
-  // The next group is set before the triangle tests, and the test loop has a
-  // single exit (an any-hit ray finishes the node's few triangles and then
-  // stops): with an early return from inside this loop, hipcc's gfx950 code
-  // for the any-hit variant carried the old group base instead of the
-  // child base out of nodes whose triangles all missed.
   s.gbase = w1.x;
   s.gbits = hits | (imask << 8);
-  const uint32_t tri_base = w1.y;
+  tri_base = w1.y;
+  tmask_out = tmask;
+  return false;
+}
+
+// Second half of a traversal step, one lane alone: test the node's
+// triangles in ascending order. Returns true when ANY and a hit was taken.
+// The loop has a single exit (an any-hit ray finishes the node's few
+// triangles and then stops): with an early return from inside it, hipcc's
+// gfx950 code for the any-hit variant carried the old group base instead of
+// the child base out of nodes whose triangles all missed.
+template <bool ANY, bool COUNT>
+__device__ __forceinline__ bool trav_tris(const SceneView& sc, const TravRay& r, TravState& s,
+                                          uint32_t tri_base, uint32_t tmask) {
   bool found = false;
   while (tmask) {
     uint32_t j = __builtin_ctz(tmask);
@@ -240,7 +273,7 @@ __device__ __forceinline__ bool trav_step(const SceneView& sc, const TravRay& r,
     float t, u, v;
     if (tri_test(r, a, b, c, t, u, v)) {
       int id = (int)as_uint(a.w);
-      if (t >= r.tnear && (t < s.tbest || (t == s.tbest && id < s.best))) {
+      if (better_hit(t, id, r.tnear, s.tbest, s.best)) {
         s.tbest = t;
         s.best = id;
         s.bu = u;
@@ -250,6 +283,16 @@ __device__ __forceinline__ bool trav_step(const SceneView& sc, const TravRay& r,
     }
   }
   return ANY && found;
+}
+
+// One traversal step: visit one node and test its triangles. Returns true
+// when the ray is finished (stack empty, or any-hit found).
+template <bool ANY, bool COUNT, class Stack>
+__device__ __forceinline__ bool trav_step(const SceneView& sc, const TravRay& r,
+                                          TravState& s, Stack& stack, bool& overflow) {
+  uint32_t tri_base, tmask;
+  if (trav_node<COUNT>(sc, r, s, stack, overflow, tri_base, tmask)) return true;
+  return trav_tris<ANY, COUNT>(sc, r, s, tri_base, tmask);
 }
 
 }  // namespace shb
