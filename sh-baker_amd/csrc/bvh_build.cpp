@@ -2,9 +2,14 @@
 //
 // 1. Binary BVH by binned SAH (32 bins per axis on centroids, leaves of at
 //    most 3 triangles).
-// 2. Greedy collapse into nodes of up to `width` children: repeatedly open the
-//    largest-area internal child (Ylitie et al. 2017 use an optimal DP; the
-//    greedy form is simpler and close in practice).
+// 2. Collapse into nodes of up to `width` children by the surface-area cost
+//    optimisation of Ylitie, Karras, Laine (HPG 2017, section 3.1): for every
+//    binary node n and slot budget i <= width, the cheapest way to represent
+//    n's subtree in at most i child slots of a wide node -- as one leaf (at
+//    most 3 triangles), as one wide internal node, or split between its two
+//    children -- with cost area x (c_node per internal node, c_tri per
+//    triangle test). A greedy collapse (open the largest child) left most
+//    nodes half empty (C2 room: 40 % of nodes with 2 children).
 // 3. Each node's children are placed in octant slots: slot s stands for the
 //    direction with sign bit a set when the child lies on the negative side of
 //    axis a from the parent centre, assigned greedily by the best dot product
@@ -19,6 +24,7 @@
 #include <cmath>
 #include <cstring>
 #include <deque>
+#include <functional>
 
 namespace shb {
 namespace {
@@ -197,8 +203,99 @@ void build_cwbvh(const float* tri_xyz, uint32_t n, int width, BvhBuildOut* out) 
   Builder b;
   b.build(tri_xyz, n);
 
+  // ---- cost-optimal collapse (see the file comment)
+  // cost[n * W + i - 1]: subtree n in at most i slots; how[...]: 0 leaf,
+  // 1 wide internal node, 2 split (children get k and i - k slots, k in
+  // split[...]), 3 as with i - 1 slots.
+  const int W = width;
+  const size_t N2 = b.nodes.size();
+  constexpr float kNodeCost = 1.0f, kTriCost = 0.35f;
+  std::vector<float> cost(N2 * W);
+  std::vector<uint8_t> how(N2 * W), split(N2 * W);
+  std::vector<uint8_t> wide_split(N2, 1);  // as a wide node: left child's share of W slots
+  {
+    // children after parents in `nodes`: a reverse sweep visits children first
+    for (size_t ni = N2; ni-- > 0;) {
+      const B2Node& nd = b.nodes[ni];
+      const float A = nd.box.area();
+      float* c = &cost[ni * W];
+      uint8_t* h = &how[ni * W];
+      uint8_t* sp = &split[ni * W];
+      if (nd.leaf()) {
+        for (int i = 0; i < W; ++i) {
+          c[i] = A * kTriCost * (float)nd.count;
+          h[i] = 0;
+          sp[i] = 0;
+        }
+        continue;
+      }
+      const float* cl = &cost[(size_t)nd.left * W];
+      const float* cr = &cost[(size_t)nd.right * W];
+      // distribute over j slots (j >= 2): best k for the left child
+      float dist[9];
+      uint8_t dk[9];
+      for (int j = 2; j <= W; ++j) {
+        float best = INFINITY;
+        int bk = 1;
+        for (int k = 1; k < j; ++k) {
+          float v = cl[k - 1] + cr[j - k - 1];
+          if (v < best) {
+            best = v;
+            bk = k;
+          }
+        }
+        dist[j] = best;
+        dk[j] = (uint8_t)bk;
+      }
+      const float c_int = A * kNodeCost + dist[W];
+      wide_split[ni] = dk[W];
+      float c1 = c_int;
+      uint8_t h1 = 1;
+      if (nd.count <= 3) {
+        const float c_leaf = A * kTriCost * (float)nd.count;
+        if (c_leaf <= c_int) {
+          c1 = c_leaf;
+          h1 = 0;
+        }
+      }
+      c[0] = c1;
+      h[0] = h1;
+      sp[0] = 0;
+      for (int i = 2; i <= W; ++i) {
+        if (dist[i] < c[i - 2]) {
+          c[i - 1] = dist[i];
+          h[i - 1] = 2;
+          sp[i - 1] = dk[i];
+        } else {
+          c[i - 1] = c[i - 2];
+          h[i - 1] = 3;
+          sp[i - 1] = 0;
+        }
+      }
+    }
+  }
+  // the slots of a wide internal node rooted at binary node n: its subtree
+  // split over W slots; each part is a leaf (kid, true) or a wide node
+  // (kid, false)
+  struct Kid {
+    int b2;
+    bool leaf;
+  };
+  std::function<void(int, int, std::vector<Kid>&)> gather = [&](int n, int i,
+                                                                 std::vector<Kid>& out_kids) {
+    const uint8_t* h = &how[(size_t)n * W];
+    while (i > 1 && h[i - 1] == 3) --i;
+    if (i == 1 || h[i - 1] != 2) {
+      out_kids.push_back({n, h[0] == 0});
+      return;
+    }
+    const int k = split[(size_t)n * W + i - 1];
+    gather(b.nodes[n].left, k, out_kids);
+    gather(b.nodes[n].right, i - k, out_kids);
+  };
+
   struct Pending {
-    int b2;       // BVH2 node collapsed into this wide node
+    int b2;       // binary node collapsed into this wide node
     uint32_t at;  // index in out->nodes
     int depth;
   };
@@ -209,31 +306,22 @@ void build_cwbvh(const float* tri_xyz, uint32_t n, int width, BvhBuildOut* out) 
     Pending pd = q.front();
     q.pop_front();
     out->max_depth = std::max(out->max_depth, pd.depth);
-    // gather up to `width` children by opening the largest internal child
-    std::vector<int> kids;
+    std::vector<Kid> parts;
     const B2Node& root = b.nodes[pd.b2];
     if (root.leaf()) {
-      kids.push_back(pd.b2);  // single-leaf tree
+      parts.push_back({pd.b2, true});  // the whole tree is one leaf
     } else {
-      kids.push_back(root.left);
-      kids.push_back(root.right);
-      while ((int)kids.size() < width) {
-        int best = -1;
-        float ba = -1.f;
-        for (int k = 0; k < (int)kids.size(); ++k) {
-          const B2Node& c = b.nodes[kids[k]];
-          if (c.leaf()) continue;
-          float a = c.box.area();
-          if (a > ba) {
-            ba = a;
-            best = k;
-          }
-        }
-        if (best < 0) break;
-        int opened = kids[best];
-        kids[best] = b.nodes[opened].left;
-        kids.push_back(b.nodes[opened].right);
-      }
+      // a wide node: its subtree over W slots, the left child taking the
+      // share that minimised the cost
+      const int k = wide_split[pd.b2];
+      gather(root.left, k, parts);
+      gather(root.right, W - k, parts);
+    }
+    std::vector<int> kids;
+    std::vector<bool> kid_leaf;
+    for (const Kid& kd : parts) {
+      kids.push_back(kd.b2);
+      kid_leaf.push_back(kd.leaf);
     }
     Box nb;
     for (int k : kids) nb.grow(b.nodes[k].box);
@@ -266,9 +354,9 @@ void build_cwbvh(const float* tri_xyz, uint32_t n, int width, BvhBuildOut* out) 
       slot_used[bs] = true;
       slot_of[bk] = bs;
     }
-    int slot_kid[8];
+    int slot_kid[8];  // index into kids
     for (int s = 0; s < 8; ++s) slot_kid[s] = -1;
-    for (int k = 0; k < m; ++k) slot_kid[slot_of[k]] = kids[k];
+    for (int k = 0; k < m; ++k) slot_kid[slot_of[k]] = k;
 
     CWNode node;
     std::memset(&node, 0, sizeof(node));
@@ -287,12 +375,12 @@ void build_cwbvh(const float* tri_xyz, uint32_t n, int width, BvhBuildOut* out) 
     for (int s = 0; s < 8; ++s) {
       int k = slot_kid[s];
       if (k < 0) continue;
-      const B2Node& c = b.nodes[k];
+      const B2Node& c = b.nodes[kids[k]];
       quantize_axis(node.px, ex, c.box.lo[0], c.box.hi[0], &node.qlox[s], &node.qhix[s]);
       quantize_axis(node.py, ey, c.box.lo[1], c.box.hi[1], &node.qloy[s], &node.qhiy[s]);
       quantize_axis(node.pz, ez, c.box.lo[2], c.box.hi[2], &node.qloz[s], &node.qhiz[s]);
-      // The binary build makes leaves of at most 3 triangles.
-      bool as_leaf = c.leaf();
+      // leaves hold at most 3 triangles (the collapse only makes those)
+      bool as_leaf = kid_leaf[k];
       if (as_leaf) {
         node.meta[s] = (uint8_t)((c.count << 5) | tri_off);
         for (uint32_t t = 0; t < c.count; ++t) {
@@ -315,7 +403,7 @@ void build_cwbvh(const float* tri_xyz, uint32_t n, int width, BvhBuildOut* out) 
       if (k < 0 || !(node.imask & (1u << s))) continue;
       uint32_t at = (uint32_t)out->nodes.size();
       out->nodes.push_back(CWNode());
-      q.push_back({k, at, pd.depth + 1});
+      q.push_back({kids[k], at, pd.depth + 1});
     }
     out->nodes[pd.at] = node;
   }
