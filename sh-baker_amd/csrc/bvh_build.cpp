@@ -20,11 +20,12 @@
 #include "bvh_build.h"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
 #include <deque>
-#include <functional>
+#include <thread>
 
 namespace shb {
 namespace {
@@ -52,118 +53,330 @@ struct B2Node {
   bool leaf() const { return left < 0; }
 };
 
+// A triangle reference: its box and id, 32 bytes, partitioned in place so
+// the binning passes read memory in order.
+struct Ref {
+  float lo[3];
+  float hi[3];
+  uint32_t id;
+  uint32_t pad;
+  float c(int a) const { return 0.5f * (lo[a] + hi[a]); }
+};
+
 struct Builder {
-  std::vector<Box> tbox;
-  std::vector<float> cen;  // 3 per triangle
-  std::vector<uint32_t> order;
+  std::vector<Ref> refs;
+  std::vector<Ref> scratch;  // the parallel partition's buffer (refs.size())
+  std::vector<uint32_t> order;  // triangle ids in leaf order (refs[k].id)
   std::vector<B2Node> nodes;
 
+  struct Item {
+    int node;  // index in the node array it is built into
+    uint32_t first, count;
+  };
+
+  // Bounds of item `it`, stored in nodes[it.node]; unless it becomes a leaf,
+  // partitions its range of `order` by the binned-SAH split and appends its
+  // two children to `nodes` and `work`.
+  static constexpr int NB = 32;
+  struct Bins {
+    uint32_t cnt[3][NB];
+    Box bb[3][NB];
+  };
+
+  // Bounds and centroid bounds of order[first, first + count) (min / max:
+  // the result does not depend on how the range is cut into chunks).
+  static void grow_ref(Box& b, const Ref& r) {
+    for (int a = 0; a < 3; ++a) {
+      b.lo[a] = std::min(b.lo[a], r.lo[a]);
+      b.hi[a] = std::max(b.hi[a], r.hi[a]);
+    }
+  }
+  void bounds(uint32_t first, uint32_t count, Box& box, Box& cbox) const {
+    for (uint32_t k = first; k < first + count; ++k) {
+      const Ref& r = refs[k];
+      grow_ref(box, r);
+      for (int a = 0; a < 3; ++a) {
+        float c = r.c(a);
+        cbox.lo[a] = std::min(cbox.lo[a], c);
+        cbox.hi[a] = std::max(cbox.hi[a], c);
+      }
+    }
+  }
+  void bin(uint32_t first, uint32_t count, const Box& cbox, Bins& B) const {
+    float kscale[3];
+    bool live[3];
+    for (int a = 0; a < 3; ++a) {
+      for (int b = 0; b < NB; ++b) B.cnt[a][b] = 0;
+      float span = cbox.hi[a] - cbox.lo[a];
+      live[a] = span > 0.f;
+      kscale[a] = live[a] ? NB / span : 0.f;
+    }
+    for (uint32_t k = first; k < first + count; ++k) {
+      const Ref& r = refs[k];
+      for (int a = 0; a < 3; ++a) {
+        if (!live[a]) continue;
+        int b = std::min(NB - 1, (int)((r.c(a) - cbox.lo[a]) * kscale[a]));
+        B.cnt[a][b]++;
+        grow_ref(B.bb[a][b], r);
+      }
+    }
+  }
+
+  // Bounds and bins of a range, cut into `threads` chunks when large.
+  void bounds_and_bins(const Item& it, int threads, Box& box, Box& cbox, Bins& B) const {
+    if (threads <= 1) {
+      bounds(it.first, it.count, box, cbox);
+      bin(it.first, it.count, cbox, B);
+      return;
+    }
+    const uint32_t chunk = (it.count + threads - 1) / threads;
+    std::vector<Box> bx(threads), cb(threads);
+    std::vector<Bins> bs(threads);
+    auto run = [&](auto&& f) {
+      std::vector<std::thread> pool;
+      for (int t = 0; t < threads; ++t) {
+        const uint32_t f0 = it.first + std::min(it.count, t * chunk);
+        const uint32_t f1 = it.first + std::min(it.count, (t + 1) * chunk);
+        pool.emplace_back([&, t, f0, f1]() { f(t, f0, f1 - f0); });
+      }
+      for (auto& th : pool) th.join();
+    };
+    run([&](int t, uint32_t f0, uint32_t c) { bounds(f0, c, bx[t], cb[t]); });
+    for (int t = 0; t < threads; ++t) {
+      box.grow(bx[t]);
+      cbox.grow(cb[t]);
+    }
+    run([&](int t, uint32_t f0, uint32_t c) { bin(f0, c, cbox, bs[t]); });
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < NB; ++b) {
+        B.cnt[a][b] = 0;
+        for (int t = 0; t < threads; ++t) {
+          B.cnt[a][b] += bs[t].cnt[a][b];
+          B.bb[a][b].grow(bs[t].bb[a][b]);
+        }
+      }
+  }
+
+  // Stable partition of refs[it.first, +count) by `left`, cut into
+  // `threads` chunks when threads > 1 (counts, offsets, scatter through a
+  // buffer): the same order for any thread count. Returns the left count.
+  template <class Pred>
+  uint32_t stable_split(const Item& it, int threads, Pred left) {
+    Ref* base = refs.data() + it.first;
+    if (threads <= 1) {
+      thread_local std::vector<Ref> tmp;
+      tmp.clear();
+      uint32_t l = 0;
+      for (uint32_t k = 0; k < it.count; ++k) {
+        if (left(base[k]))
+          base[l++] = base[k];
+        else
+          tmp.push_back(base[k]);
+      }
+      std::copy(tmp.begin(), tmp.end(), base + l);
+      return l;
+    }
+    const uint32_t chunk = (it.count + threads - 1) / threads;
+    std::vector<uint32_t> nl(threads, 0), offl(threads), offr(threads);
+    Ref* buf = scratch.data() + it.first;  // this range's part of the buffer
+    auto run = [&](auto&& f) {
+      std::vector<std::thread> pool;
+      for (int t = 0; t < threads; ++t) pool.emplace_back([&, t]() { f(t); });
+      for (auto& th : pool) th.join();
+    };
+    auto range = [&](int t, uint32_t& a, uint32_t& b) {
+      a = std::min(it.count, t * chunk);
+      b = std::min(it.count, (t + 1) * chunk);
+    };
+    run([&](int t) {
+      uint32_t a, b;
+      range(t, a, b);
+      uint32_t c = 0;
+      for (uint32_t k = a; k < b; ++k) c += left(base[k]) ? 1u : 0u;
+      nl[t] = c;
+    });
+    uint32_t L = 0;
+    for (int t = 0; t < threads; ++t) {
+      offl[t] = L;
+      L += nl[t];
+    }
+    uint32_t R = L;
+    for (int t = 0; t < threads; ++t) {
+      uint32_t a, b;
+      range(t, a, b);
+      offr[t] = R;
+      R += (b - a) - nl[t];
+    }
+    run([&](int t) {
+      uint32_t a, b;
+      range(t, a, b);
+      uint32_t pl = offl[t], pr = offr[t];
+      for (uint32_t k = a; k < b; ++k) buf[left(base[k]) ? pl++ : pr++] = base[k];
+    });
+    run([&](int t) {
+      uint32_t a, b;
+      range(t, a, b);
+      std::copy(buf + a, buf + b, base + a);
+    });
+    return L;
+  }
+
+  void split_item(const Item& it, std::vector<B2Node>& nodes_out, std::vector<Item>& work,
+                  int threads = 1) {
+    Box box, cbox;
+    Bins B;
+    bounds_and_bins(it, threads, box, cbox, B);
+    nodes_out[it.node].box = box;
+    nodes_out[it.node].first = it.first;
+    nodes_out[it.node].count = it.count;
+    if (it.count <= 1) return;
+    int best_axis = -1, best_bin = -1;
+    float best_cost = INFINITY;
+    for (int a = 0; a < 3; ++a) {
+      float span = cbox.hi[a] - cbox.lo[a];
+      if (!(span > 0.f)) continue;
+      const uint32_t* cnt = B.cnt[a];
+      const Box* bb = B.bb[a];
+      float la[NB];
+      uint32_t lc[NB];
+      Box acc;
+      uint32_t c = 0;
+      for (int b = 0; b < NB; ++b) {
+        acc.grow(bb[b]);
+        c += cnt[b];
+        la[b] = acc.area();
+        lc[b] = c;
+      }
+      acc = Box();
+      c = 0;
+      for (int b = NB - 1; b >= 1; --b) {
+        acc.grow(bb[b]);
+        c += cnt[b];
+        if (lc[b - 1] == 0 || c == 0) continue;
+        float cost = la[b - 1] * lc[b - 1] + acc.area() * c;
+        if (cost < best_cost) {
+          best_cost = cost;
+          best_axis = a;
+          best_bin = b - 1;
+        }
+      }
+    }
+    float leaf_cost = box.area() * it.count;
+    // SAH with traversal cost ~ 1 triangle test per unit of parent area.
+    if (it.count <= 3 && !(best_cost + box.area() < leaf_cost)) return;
+    uint32_t mid;
+    if (best_axis < 0) {
+      mid = it.first + it.count / 2;
+    } else {
+      float span = cbox.hi[best_axis] - cbox.lo[best_axis];
+      float kscale = NB / span;
+      const float lo = cbox.lo[best_axis];
+      mid = it.first + stable_split(it, threads, [&](const Ref& r) {
+              int b = std::min(NB - 1, (int)((r.c(best_axis) - lo) * kscale));
+              return b <= best_bin;
+            });
+      if (mid == it.first || mid == it.first + it.count) mid = it.first + it.count / 2;
+    }
+    int l = (int)nodes_out.size();
+    nodes_out.push_back(B2Node());
+    int r = (int)nodes_out.size();
+    nodes_out.push_back(B2Node());
+    nodes_out[it.node].left = l;
+    nodes_out[it.node].right = r;
+    work.push_back({r, mid, it.first + it.count - mid});
+    work.push_back({l, it.first, mid - it.first});
+  }
+
+  // Top-down build. The top levels are split here until the open ranges
+  // are small or numerous enough; the remaining subtrees are built by
+  // worker threads into their own node arrays and appended (children
+  // always after their parent). Every range sees the same permutation of
+  // `order` as in a serial build, so the tree does not depend on the
+  // thread count.
   void build(const float* xyz, uint32_t n) {
-    tbox.resize(n);
-    cen.resize(3 * (size_t)n);
-    order.resize(n);
+    refs.resize(n);
+    scratch.resize(n);
     for (uint32_t i = 0; i < n; ++i) {
       const float* t = xyz + 9 * (size_t)i;
-      Box b;
+      Ref& r = refs[i];
+      for (int a = 0; a < 3; ++a) {
+        r.lo[a] = INFINITY;
+        r.hi[a] = -INFINITY;
+      }
       for (int k = 0; k < 3; ++k)
         for (int a = 0; a < 3; ++a) {
-          b.lo[a] = std::min(b.lo[a], t[3 * k + a]);
-          b.hi[a] = std::max(b.hi[a], t[3 * k + a]);
+          r.lo[a] = std::min(r.lo[a], t[3 * k + a]);
+          r.hi[a] = std::max(r.hi[a], t[3 * k + a]);
         }
-      tbox[i] = b;
-      for (int a = 0; a < 3; ++a) cen[3 * (size_t)i + a] = 0.5f * (b.lo[a] + b.hi[a]);
-      order[i] = i;
+      r.id = i;
+      r.pad = 0;
     }
     nodes.reserve(2 * (size_t)n + 1);
     nodes.push_back(B2Node());
-    struct Item {
-      int node;
-      uint32_t first, count;
-    };
-    std::vector<Item> work{{0, 0, n}};
-    constexpr int NB = 32;
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const unsigned T = std::min(16u, hw);
+    // ranges below this many triangles go to the subtree workers (at least
+    // 4 ranges per worker); larger ranges are split here, their bins made
+    // by all threads
+    const uint32_t hand_off = std::max<uint32_t>(1024u, n / (4u * T));
+    const uint32_t kParallelBins = 1u << 16;
+    std::vector<Item> work{{0, 0, n}}, open;
     while (!work.empty()) {
       Item it = work.back();
       work.pop_back();
-      Box box, cbox;
-      for (uint32_t k = it.first; k < it.first + it.count; ++k) {
-        uint32_t id = order[k];
-        box.grow(tbox[id]);
-        for (int a = 0; a < 3; ++a) {
-          float c = cen[3 * (size_t)id + a];
-          cbox.lo[a] = std::min(cbox.lo[a], c);
-          cbox.hi[a] = std::max(cbox.hi[a], c);
-        }
+      if (T > 1 && it.count < hand_off) {
+        open.push_back(it);
+        continue;
       }
-      nodes[it.node].box = box;
-      nodes[it.node].first = it.first;
-      nodes[it.node].count = it.count;
-      if (it.count <= 1) continue;
-      int best_axis = -1, best_bin = -1;
-      float best_cost = INFINITY;
-      for (int a = 0; a < 3; ++a) {
-        float span = cbox.hi[a] - cbox.lo[a];
-        if (!(span > 0.f)) continue;
-        float kscale = NB / span;
-        uint32_t cnt[NB] = {0};
-        Box bb[NB];
-        for (uint32_t k = it.first; k < it.first + it.count; ++k) {
-          uint32_t id = order[k];
-          int b = std::min(NB - 1, (int)((cen[3 * (size_t)id + a] - cbox.lo[a]) * kscale));
-          cnt[b]++;
-          bb[b].grow(tbox[id]);
-        }
-        float la[NB];
-        uint32_t lc[NB];
-        Box acc;
-        uint32_t c = 0;
-        for (int b = 0; b < NB; ++b) {
-          acc.grow(bb[b]);
-          c += cnt[b];
-          la[b] = acc.area();
-          lc[b] = c;
-        }
-        acc = Box();
-        c = 0;
-        for (int b = NB - 1; b >= 1; --b) {
-          acc.grow(bb[b]);
-          c += cnt[b];
-          if (lc[b - 1] == 0 || c == 0) continue;
-          float cost = la[b - 1] * lc[b - 1] + acc.area() * c;
-          if (cost < best_cost) {
-            best_cost = cost;
-            best_axis = a;
-            best_bin = b - 1;
-          }
-        }
-      }
-      float leaf_cost = box.area() * it.count;
-      // SAH with traversal cost ~ 1 triangle test per unit of parent area.
-      if (it.count <= 3 && !(best_cost + box.area() < leaf_cost)) continue;
-      uint32_t mid;
-      if (best_axis < 0) {
-        mid = it.first + it.count / 2;
-      } else {
-        float span = cbox.hi[best_axis] - cbox.lo[best_axis];
-        float kscale = NB / span;
-        auto pv = std::partition(order.begin() + it.first,
-                                 order.begin() + it.first + it.count, [&](uint32_t id) {
-                                   int b = std::min(NB - 1, (int)((cen[3 * (size_t)id + best_axis] -
-                                                                   cbox.lo[best_axis]) * kscale));
-                                   return b <= best_bin;
-                                 });
-        mid = (uint32_t)(pv - order.begin());
-        if (mid == it.first || mid == it.first + it.count) mid = it.first + it.count / 2;
-      }
-      int l = (int)nodes.size();
-      nodes.push_back(B2Node());
-      int r = (int)nodes.size();
-      nodes.push_back(B2Node());
-      nodes[it.node].left = l;
-      nodes[it.node].right = r;
-      work.push_back({r, mid, it.first + it.count - mid});
-      work.push_back({l, it.first, mid - it.first});
+      split_item(it, nodes, work, T > 1 && it.count >= kParallelBins ? (int)T : 1);
     }
+    if (open.empty()) {
+      finish_order();
+      return;
+    }
+    std::sort(open.begin(), open.end(),
+              [](const Item& a, const Item& b) { return a.count > b.count; });
+    std::vector<std::vector<B2Node>> local(open.size());
+    std::atomic<size_t> next{0};
+    auto worker = [&]() {
+      for (;;) {
+        const size_t j = next.fetch_add(1);
+        if (j >= open.size()) return;
+        std::vector<B2Node>& ln = local[j];
+        ln.reserve(2 * (size_t)open[j].count);
+        ln.push_back(B2Node());
+        std::vector<Item> w{{0, open[j].first, open[j].count}};
+        while (!w.empty()) {
+          Item it = w.back();
+          w.pop_back();
+          split_item(it, ln, w);
+        }
+      }
+    };
+    std::vector<std::thread> pool;
+    for (unsigned t = 0; t < T; ++t) pool.emplace_back(worker);
+    for (auto& th : pool) th.join();
+    for (size_t j = 0; j < open.size(); ++j) {
+      const std::vector<B2Node>& ln = local[j];
+      const int base = (int)nodes.size() - 1;  // local index k >= 1 -> base + k
+      auto remap = [&](int k) { return k < 0 ? -1 : (k == 0 ? open[j].node : base + k); };
+      B2Node root = ln[0];
+      root.left = remap(root.left);
+      root.right = remap(root.right);
+      nodes[open[j].node] = root;
+      for (size_t k = 1; k < ln.size(); ++k) {
+        B2Node nd = ln[k];
+        nd.left = remap(nd.left);
+        nd.right = remap(nd.right);
+        nodes.push_back(nd);
+      }
+    }
+    finish_order();
+  }
+
+  void finish_order() {
+    order.resize(refs.size());
+    for (size_t k = 0; k < refs.size(); ++k) order[k] = refs[k].id;
   }
 };
 
@@ -277,21 +490,29 @@ void build_cwbvh(const float* tri_xyz, uint32_t n, int width, BvhBuildOut* out) 
   // the slots of a wide internal node rooted at binary node n: its subtree
   // split over W slots; each part is a leaf (kid, true) or a wide node
   // (kid, false)
-  struct Kid {
-    int b2;
-    bool leaf;
-  };
-  std::function<void(int, int, std::vector<Kid>&)> gather = [&](int n, int i,
-                                                                 std::vector<Kid>& out_kids) {
-    const uint8_t* h = &how[(size_t)n * W];
-    while (i > 1 && h[i - 1] == 3) --i;
-    if (i == 1 || h[i - 1] != 2) {
-      out_kids.push_back({n, h[0] == 0});
-      return;
+  // gather(n, i, ...) appends the parts of subtree n over at most i slots,
+  // left subtree first (an explicit stack; at most 8 parts)
+  auto gather = [&](int n0, int i0, int* kid, bool* leaf, int& m) {
+    int st_n[16], st_i[16], sp = 0;
+    st_n[sp] = n0;
+    st_i[sp++] = i0;
+    while (sp > 0) {
+      const int n = st_n[--sp];
+      int i = st_i[sp];
+      const uint8_t* h = &how[(size_t)n * W];
+      while (i > 1 && h[i - 1] == 3) --i;
+      if (i == 1 || h[i - 1] != 2) {
+        kid[m] = n;
+        leaf[m] = h[0] == 0;
+        ++m;
+        continue;
+      }
+      const int k = split[(size_t)n * W + i - 1];
+      st_n[sp] = b.nodes[n].right;  // popped after the left part
+      st_i[sp++] = i - k;
+      st_n[sp] = b.nodes[n].left;
+      st_i[sp++] = k;
     }
-    const int k = split[(size_t)n * W + i - 1];
-    gather(b.nodes[n].left, k, out_kids);
-    gather(b.nodes[n].right, i - k, out_kids);
   };
 
   struct Pending {
@@ -299,6 +520,8 @@ void build_cwbvh(const float* tri_xyz, uint32_t n, int width, BvhBuildOut* out) 
     uint32_t at;  // index in out->nodes
     int depth;
   };
+  out->tris.reserve(3 * (size_t)n);
+  out->nodes.reserve(N2 / 2 + 1);
   out->nodes.push_back(CWNode());
   std::deque<Pending> q;
   q.push_back({0, 0, 1});
@@ -306,34 +529,38 @@ void build_cwbvh(const float* tri_xyz, uint32_t n, int width, BvhBuildOut* out) 
     Pending pd = q.front();
     q.pop_front();
     out->max_depth = std::max(out->max_depth, pd.depth);
-    std::vector<Kid> parts;
+    int kids[8];
+    bool kid_leaf[8];
+    int m = 0;
     const B2Node& root = b.nodes[pd.b2];
     if (root.leaf()) {
-      parts.push_back({pd.b2, true});  // the whole tree is one leaf
+      kids[0] = pd.b2;  // the whole tree is one leaf
+      kid_leaf[0] = true;
+      m = 1;
     } else {
       // a wide node: its subtree over W slots, the left child taking the
       // share that minimised the cost
       const int k = wide_split[pd.b2];
-      gather(root.left, k, parts);
-      gather(root.right, W - k, parts);
-    }
-    std::vector<int> kids;
-    std::vector<bool> kid_leaf;
-    for (const Kid& kd : parts) {
-      kids.push_back(kd.b2);
-      kid_leaf.push_back(kd.leaf);
+      gather(root.left, k, kids, kid_leaf, m);
+      gather(root.right, W - k, kids, kid_leaf, m);
     }
     Box nb;
-    for (int k : kids) nb.grow(b.nodes[k].box);
+    for (int k = 0; k < m; ++k) nb.grow(b.nodes[kids[k]].box);
     // octant slot assignment
     float pc[3], cc[8][3];
     for (int a = 0; a < 3; ++a) pc[a] = 0.5f * (nb.lo[a] + nb.hi[a]);
-    int m = (int)kids.size();
     for (int k = 0; k < m; ++k)
       for (int a = 0; a < 3; ++a)
         cc[k][a] = 0.5f * (b.nodes[kids[k]].box.lo[a] + b.nodes[kids[k]].box.hi[a]) - pc[a];
     int slot_of[8];
     bool kid_done[8] = {false}, slot_used[8] = {false};
+    float score[8][8];  // child k in slot s: (child centre - node centre) . slot sign vector
+    for (int k = 0; k < m; ++k)
+      for (int s = 0; s < 8; ++s) {
+        float sc = 0.f;
+        for (int a = 0; a < 3; ++a) sc += ((s >> a) & 1) ? -cc[k][a] : cc[k][a];
+        score[k][s] = sc;
+      }
     for (int round = 0; round < m; ++round) {
       float best = -INFINITY;
       int bk = -1, bs = -1;
@@ -341,8 +568,7 @@ void build_cwbvh(const float* tri_xyz, uint32_t n, int width, BvhBuildOut* out) 
         if (kid_done[k]) continue;
         for (int s = 0; s < 8; ++s) {
           if (slot_used[s]) continue;
-          float sc = 0.f;
-          for (int a = 0; a < 3; ++a) sc += ((s >> a) & 1) ? -cc[k][a] : cc[k][a];
+          const float sc = score[k][s];
           if (sc > best) {
             best = sc;
             bk = k;
