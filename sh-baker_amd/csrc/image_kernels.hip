@@ -326,6 +326,77 @@ __global__ void __launch_bounds__(256) k_dilate_pass(const float* src, float* ds
   if (inside) mdst[(size_t)y * w + x] = (self || count > 0) ? 1 : 0;
 }
 
+// The same pass with every channel of the tile and its halo staged in LDS
+// at once: rows of the interleaved image are contiguous (18 texels x ch
+// floats), so loads and stores are coalesced and each texel is read from
+// memory once per pass. Same arithmetic and order as k_dilate_pass.
+constexpr int kDilateMaxCh = 32;  // LDS: 18 x 18 x ch floats (40.5 KB at 32)
+__global__ void __launch_bounds__(256) k_dilate_tile(const float* src, float* dst,
+                                                     const uint8_t* msrc, uint8_t* mdst, int w,
+                                                     int h, int ch) {
+  extern __shared__ float tile[];  // (kTile + 2)^2 texels x ch
+  __shared__ uint8_t m[kTile + 2][kTile + 2];
+  constexpr int E = kTile + 2;
+  const int tx = threadIdx.x % kTile, ty = threadIdx.x / kTile;
+  const int x0 = blockIdx.x * kTile, y0 = blockIdx.y * kTile;
+  const int x = x0 + tx, y = y0 + ty;
+  for (int k = threadIdx.x; k < E * E; k += blockDim.x) {
+    int lx = k % E, ly = k / E;
+    int gx = x0 + lx - 1, gy = y0 + ly - 1;
+    m[ly][lx] = (gx >= 0 && gx < w && gy >= 0 && gy < h) ? msrc[(size_t)gy * w + gx] : 0;
+  }
+  // rows of the tile + halo: E texels x ch floats each, contiguous in src
+  const int rowf = E * ch;
+  for (int k = threadIdx.x; k < E * rowf; k += blockDim.x) {
+    const int ly = k / rowf, r = k - ly * rowf;
+    const int lx = r / ch;
+    const int gx = x0 + lx - 1, gy = y0 + ly - 1;
+    tile[k] = (gx >= 0 && gx < w && gy >= 0 && gy < h)
+                  ? src[((size_t)gy * w + (size_t)(x0 - 1)) * ch + r]
+                  : 0.f;
+  }
+  __syncthreads();
+  const bool inside = x < w && y < h;
+  const bool self = inside && m[ty + 1][tx + 1];
+  int count = 0;
+  if (inside && !self) {
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dx = -1; dx <= 1; ++dx) {
+        if (dx == 0 && dy == 0) continue;
+        count += m[ty + 1 + dy][tx + 1 + dx] ? 1 : 0;
+      }
+  }
+  float out[kDilateMaxCh];
+  const float* cen = tile + ((ty + 1) * E + (tx + 1)) * ch;
+  for (int c = 0; c < ch; ++c) {
+    float o = cen[c];
+    if (count > 0) {
+      float acc = 0.f;
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+          if (dx == 0 && dy == 0) continue;
+          if (m[ty + 1 + dy][tx + 1 + dx]) acc += tile[((ty + 1 + dy) * E + (tx + 1 + dx)) * ch + c];
+        }
+      o = acc * (1.0f / count);
+    }
+    out[c] = o;
+  }
+  __syncthreads();  // every thread has read its neighbours
+  float* own = tile + ((ty + 1) * E + (tx + 1)) * ch;
+  for (int c = 0; c < ch; ++c) own[c] = out[c];
+  __syncthreads();
+  // interior rows back to dst, contiguous kTile texels x ch floats
+  const int rin = kTile * ch;
+  for (int k = threadIdx.x; k < kTile * rin; k += blockDim.x) {
+    const int ly = k / rin, r = k - ly * rin;
+    const int lx = r / ch;
+    const int gx = x0 + lx, gy = y0 + ly;
+    if (gx < w && gy < h)
+      dst[((size_t)gy * w + (size_t)x0) * ch + r] = tile[((ly + 1) * E + 1) * ch + r];
+  }
+  if (inside) mdst[(size_t)y * w + x] = (self || count > 0) ? 1 : 0;
+}
+
 // Ping-pongs between (a, ma) and (b, mb); *result_in_a tells where the final
 // pass landed. Passes after the first unchanged pass would copy only, so
 // running them all gives the reference's early-exit result.
@@ -334,11 +405,18 @@ hipError_t launch_dilate(float* a, float* b, uint8_t* ma, uint8_t* mb, int w, in
   (void)changed;
   bool in_a = true;
   dim3 grid((w + kTile - 1) / kTile, (h + kTile - 1) / kTile);
+  const bool tiled = ch >= 1 && ch <= kDilateMaxCh;
+  const size_t lds = tiled ? (size_t)(kTile + 2) * (kTile + 2) * ch * sizeof(float) : 0;
   for (int p = 0; p < passes; ++p) {
-    if (in_a)
-      hipLaunchKernelGGL(k_dilate_pass, grid, dim3(kTile * kTile), 0, st, a, b, ma, mb, w, h, ch);
+    float* s0 = in_a ? a : b;
+    float* d0 = in_a ? b : a;
+    uint8_t* ms = in_a ? ma : mb;
+    uint8_t* md = in_a ? mb : ma;
+    if (tiled)
+      hipLaunchKernelGGL(k_dilate_tile, grid, dim3(kTile * kTile), lds, st, s0, d0, ms, md, w, h,
+                         ch);
     else
-      hipLaunchKernelGGL(k_dilate_pass, grid, dim3(kTile * kTile), 0, st, b, a, mb, ma, w, h, ch);
+      hipLaunchKernelGGL(k_dilate_pass, grid, dim3(kTile * kTile), 0, st, s0, d0, ms, md, w, h, ch);
     in_a = !in_a;
   }
   *result_in_a = in_a;
