@@ -78,7 +78,10 @@ def test_downsample_bit_exact(baker):
     assert (baker.downsample(ones, 1, 1, 2) == 1.0).all()
 
 
-@pytest.mark.parametrize("ch,passes", [(27, 1), (27, 4), (1, 2), (1, 16)])
+# 28 = SH + visibility, 32 the largest count staged in LDS at once, 40 the
+# one-channel-at-a-time kernel
+@pytest.mark.parametrize("ch,passes", [(27, 1), (27, 4), (1, 2), (1, 16), (28, 3), (32, 2),
+                                       (40, 2)])
 def test_dilate_bit_exact(baker, ch, passes):
     rng = np.random.default_rng(passes * 31 + ch)
     w, h = 67, 45
