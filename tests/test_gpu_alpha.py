@@ -247,3 +247,26 @@ def test_batches_mixing_jobs_with_branches():
     finally:
         for b in bakers:
             b.close()
+
+
+def test_adjoint_reruns_when_spare_slots_run_out():
+    # a fresh context's first pass has too few spare slots for this room;
+    # the re-run must start from zeroed gradient sums: the same gradients as
+    # a second call that fits in one pass, up to the rounding of the blocks'
+    # FP32 partial sums (a sum carried over would be off by a whole factor)
+    s = see_through_room(32)
+    spp, B, seed = 8, 4, 5
+    b = shb.Baker(0)
+    try:
+        b.upload_scene(s)
+        b.rasterize(32, 32, 1, want=False)
+        w = np.random.default_rng(3).normal(size=(32 * 32, 27)).astype(np.float32)
+        dl1, dm1, st1 = b.bake_adjoint(spp, B, w, seed=seed)
+        assert st1["passes"] >= 2 and st1["branches"] > 0
+        dl2, dm2, st2 = b.bake_adjoint(spp, B, w, seed=seed)
+        assert st2["passes"] == 1 and st2["branches"] == st1["branches"]
+        for a, c in ((dl1, dl2), (dm1, dm2)):
+            np.testing.assert_allclose(a, c, rtol=1e-5, atol=1e-8 * np.abs(c).max())
+        assert np.abs(dm2).max() > 0
+    finally:
+        b.close()
