@@ -5,9 +5,12 @@
 // up to kStackLds groups in LDS (laid out entry-major, lane-minor, so a
 // wave's accesses hit 64 distinct banks); the group stack never holds more
 // than one entry per tree level, and the upload rejects deeper trees.
-// Triangles hit in a node are tested as soon as the node is decoded. Ray/triangle: the watertight test of Woop,
-// Benthin, Wald (JCGT 2013), double sided, ties in t resolved to the lowest
-// triangle id so the closest hit does not depend on traversal order.
+// A step decodes one node (trav_node) and then tests the triangles of its
+// leaf children before the next step: per lane (trav_tris, trav_step) or,
+// in the persistent kernel, spread over the wave. Ray/triangle: the
+// watertight test of Woop, Benthin, Wald (JCGT 2013), double sided, ties in
+// t resolved to the lowest triangle id so the closest hit does not depend
+// on traversal order.
 #ifndef SHB_TRAVERSE_H_
 #define SHB_TRAVERSE_H_
 

@@ -5,7 +5,7 @@ from scipy's spherical harmonics, and from closed forms; nothing here runs on
 a GPU and nothing comes from the reference's sources at run time.
 
 Files (numpy .npz, no pickles):
-  c1_box_room.npz  the 256x256 box-room scene (4 x 3 x 5 m room + 0.5 m cube,
+  box_room_256.npz  the 256x256 box-room scene (4 x 3 x 5 m room + 0.5 m cube,
                    one 50 W point light) as flat arrays; 4096 seeded rays
                    with the oracle's FP32 watertight closest hits, its FP32
                    any-hit flags and the FP64 brute-force closest hits; the
@@ -98,7 +98,7 @@ def real_sh_fp64(d):
 
 
 def main():
-    # ---------------------------------------------------------------- C1
+    # ------------------------------------------------ box room, 256x256
     s = scenes.box_room_scene(256, 256)
     o = orc.Oracle(s)
     rays = seeded_rays(s, 4096, 20240601)
@@ -110,7 +110,7 @@ def main():
     cov = pts["material_id"] >= 0
     sample_idx = np.sort(rng.choice(np.flatnonzero(cov), 512, replace=False)).astype(np.int64)
     np.savez_compressed(
-        os.path.join(HERE, "c1_box_room.npz"), **scene_arrays(s), rays=rays,
+        os.path.join(HERE, "box_room_256.npz"), **scene_arrays(s), rays=rays,
         hits_fp32=h32, anyhit_fp32=(any_["prim_id"] >= 0).astype(np.uint8),
         hits_fp64_prim=h64["prim_id"], hits_fp64_t=h64["t"],
         raster_coverage=np.packbits(cov), raster_sha256=surface_digest(pts),

@@ -14,7 +14,7 @@ from shb import SURFACE_DTYPE, scenes
 
 @pytest.fixture(scope="module")
 def c1():
-    z = fx.load("c1_box_room")
+    z = fx.load("box_room_256")
     return z, fx.scene(z)
 
 

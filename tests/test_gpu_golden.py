@@ -16,7 +16,7 @@ pytestmark = pytest.mark.gpu
 
 @pytest.fixture(scope="module")
 def c1(baker):
-    z = fx.load("c1_box_room")
+    z = fx.load("box_room_256")
     baker.upload_scene(fx.scene(z))
     return z
 

@@ -45,9 +45,9 @@ def compare_exact(hg, ho):
     np.testing.assert_array_equal(hg["geom_id"], ho["geom_id"])
 
 
-@pytest.mark.parametrize("which", ["c1", "synth20k"])
+@pytest.mark.parametrize("which", ["box_room", "synth20k"])
 def test_closest_hit_bit_exact_vs_oracle(baker, which):
-    s = scenes.box_room_scene() if which == "c1" else scenes.scaled_synthetic_scene(20000, 256, 7)
+    s = scenes.box_room_scene() if which == "box_room" else scenes.scaled_synthetic_scene(20000, 256, 7)
     baker.upload_scene(s)
     rays = seeded_rays(s, 20000, 1)
     hg = baker.trace_rays(rays)
@@ -146,13 +146,13 @@ def test_bvh_widths_agree(baker):
     baker.set_tuning(bvh_width=8)
 
 
-@pytest.mark.parametrize("which,width", [("c1", 8), ("synth20k", 8), ("synth20k", 4),
-                                         ("c2", 8)])
+@pytest.mark.parametrize("which,width", [("box_room", 8), ("synth20k", 8), ("synth20k", 4),
+                                         ("room_262k", 8)])
 def test_visit_counts_equal_host_recount(baker, which, width):
     # the per-ray node visits and triangle tests behind the roofline's
     # algorithmic bytes equal a host traversal of the same exported BVH on
     # the same rays (closest hit and any hit); hits agree bit for bit
-    if which == "c1":
+    if which == "box_room":
         s = scenes.box_room_scene()
     elif which == "synth20k":
         s = scenes.scaled_synthetic_scene(20000, 256, 7)
