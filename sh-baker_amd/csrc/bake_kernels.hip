@@ -451,7 +451,7 @@ __global__ void k_hit_details(SceneView sc, const float4* rays, const float4* hi
 // Primary ray + texel NEE for every path slot of the batch (UNIT: every
 // light and albedo scale is 1).
 template <bool UNIT>
-__global__ void k_generate(BakeArgs a) {
+__global__ void __launch_bounds__(256, 5) k_generate(BakeArgs a) {
   const uint32_t n = a.P;
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
     uint32_t p = base + threadIdx.x;
@@ -469,9 +469,9 @@ __global__ void k_generate(BakeArgs a) {
 // block's gradient sums in dynamic LDS; SPLIT: batches with spare slots for
 // alpha splits; UNIT: every light and albedo scale is 1).
 template <bool ADJ, bool SPLIT, bool UNIT>
-__global__ void k_shade(BakeArgs a, int depth, const uint32_t* in_queue,
-                        const uint32_t* in_count, uint32_t* out_queue,
-                        uint32_t* out_count) {
+__device__ __forceinline__ void shade_stage(const BakeArgs& a, int depth, const uint32_t* in_queue,
+                                            const uint32_t* in_count, uint32_t* out_queue,
+                                            uint32_t* out_count) {
   extern __shared__ float grad_lds[];
   GradAcc acc;
   if (ADJ)
@@ -499,6 +499,23 @@ __global__ void k_shade(BakeArgs a, int depth, const uint32_t* in_queue,
     }
   }
   if (ADJ) acc.flush();
+}
+
+// The common form (no spare slots, every scale 1) runs at 5 waves per SIMD
+// (a register cap with a small spill, faster: the light loop waits on a
+// scalar load per light); the others keep the registers they need.
+template <bool ADJ>
+__global__ void __launch_bounds__(256, 5) k_shade_unit(BakeArgs a, int depth,
+                                                       const uint32_t* in_queue,
+                                                       const uint32_t* in_count,
+                                                       uint32_t* out_queue, uint32_t* out_count) {
+  shade_stage<ADJ, false, true>(a, depth, in_queue, in_count, out_queue, out_count);
+}
+template <bool ADJ, bool SPLIT, bool UNIT>
+__global__ void __launch_bounds__(256) k_shade(BakeArgs a, int depth, const uint32_t* in_queue,
+                                               const uint32_t* in_count, uint32_t* out_queue,
+                                               uint32_t* out_count) {
+  shade_stage<ADJ, SPLIT, UNIT>(a, depth, in_queue, in_count, out_queue, out_count);
 }
 
 // Adjoint of the visible vertex NEE terms of one depth: the shadow queue's
@@ -675,11 +692,17 @@ hipError_t launch_shade(const BakeArgs& b, int depth, const uint32_t* in_q,
   const int v = (b.adj_grad ? 4 : 0) | (b.S ? 2 : 0) | (b.unit_scales ? 1 : 0);
   switch (v) {
     case 0: SHB_SHADE(false, false, false); break;
-    case 1: SHB_SHADE(false, false, true); break;
+    case 1:
+      hipLaunchKernelGGL(k_shade_unit<false>, dim3(grid), dim3(256), lds, st, b, depth, in_q, in_n,
+                         out_q, out_n);
+      break;
     case 2: SHB_SHADE(false, true, false); break;
     case 3: SHB_SHADE(false, true, true); break;
     case 4: SHB_SHADE(true, false, false); break;
-    case 5: SHB_SHADE(true, false, true); break;
+    case 5:
+      hipLaunchKernelGGL(k_shade_unit<true>, dim3(grid), dim3(256), lds, st, b, depth, in_q, in_n,
+                         out_q, out_n);
+      break;
     case 6: SHB_SHADE(true, true, false); break;
     default: SHB_SHADE(true, true, true); break;
   }
