@@ -37,17 +37,27 @@ def log(*a):
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_k_trace_synthetic_room_1024_ss3_spp128.md")
 
 
+VALU_FILE = os.path.join(ROOT, "profiles",
+                         "valu_issue_ceiling_probe_and_synthetic_room_1024_ss3_spp128.md")
+
+
+def read_profile_values(path, prefix):
+    """`<prefix> <key>: <number>` lines of a committed profile summary."""
+    out = {}
+    if os.path.exists(path):
+        for line in open(path):
+            if line.startswith(prefix + " "):
+                k, v = line[len(prefix) + 1:].split(":")
+                out[k.strip()] = float(v)
+    return out
+
+
 def read_traffic():
     """HBM bytes per launch of each traversal kernel kind, measured with
     rocprofv3 FETCH_SIZE + WRITE_SIZE (scaled by a copy kernel of known size)
     and recorded as `traffic_bytes_per_launch <kind>: <bytes>` lines."""
-    out = {}
-    if os.path.exists(TRAFFIC_FILE):
-        for line in open(TRAFFIC_FILE):
-            if line.startswith("traffic_bytes_per_launch "):
-                k, v = line[len("traffic_bytes_per_launch "):].split(":")
-                out[k.strip()] = int(float(v))
-    return out
+    return {k: int(v) for k, v in read_profile_values(TRAFFIC_FILE,
+                                                      "traffic_bytes_per_launch").items()}
 
 
 def copy_probe(mib):
@@ -61,6 +71,59 @@ def copy_probe(mib):
         c.copy_(a)
     torch.cuda.synchronize()
     print(json.dumps({"copy_probe_bytes_each_way": n * 4, "copies": 3}), flush=True)
+
+
+VALU_PROBE_KINDS = ("fp32_fma", "int_add_rotate_xor", "mixed_4fma_6int")
+
+
+def probe_lib():
+    """libshb_probe.so: the VALU issue probe and a CU count, on the same HIP
+    runtime as libshb_hip.so (torch's CUDA init reported no GPUs once
+    libshb_hip.so had opened the device)."""
+    import ctypes
+    lib = ctypes.CDLL(os.path.join(ROOT, "sh-baker_amd", "shb", "libshb_probe.so"))
+    lib.shb_probe_cus.restype = ctypes.c_int
+    lib.shb_probe_cus.argtypes = [ctypes.POINTER(ctypes.c_int)]
+    return lib
+
+
+def device_cus():
+    import ctypes
+    n = ctypes.c_int(0)
+    if probe_lib().shb_probe_cus(ctypes.byref(n)) != 0 or n.value <= 0:
+        raise RuntimeError("shb_probe_cus failed")
+    return n.value
+
+
+def valu_probe(iters=20000, blocks_per_cu=8, reps=3):
+    """VALU issue ceiling (sh-baker_amd/csrc/probe/valu_probe.hip): wave64
+    VALU instructions issued per SIMD per shader clock for independent
+    chains of FP32 fma, of integer add/rotate/xor, and of both, with 8 waves
+    per SIMD. The traversal and shading kernels are compared with it using
+    their SQ_INSTS_VALU counts."""
+    import ctypes
+    lib = probe_lib()
+    f = lib.shb_probe_valu
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_int] * 5 + [ctypes.POINTER(ctypes.c_float),
+                                       ctypes.POINTER(ctypes.c_double),
+                                       ctypes.POINTER(ctypes.c_double)]
+    cus = device_cus()
+    simds = 4 * cus
+    res = []
+    for kind, name in enumerate(VALU_PROBE_KINDS):
+        ms, wi, mhz = ctypes.c_float(), ctypes.c_double(), ctypes.c_double()
+        rc = f(kind, cus, blocks_per_cu, iters, reps, ctypes.byref(ms), ctypes.byref(wi),
+               ctypes.byref(mhz))
+        if rc != 0:
+            raise RuntimeError(f"shb_probe_valu({name}) failed: {rc}")
+        rate = wi.value / (ms.value * 1e-3) / simds
+        res.append({"kind": name, "ms": round(ms.value, 4), "wave_instr": wi.value,
+                    "shader_mhz": round(mhz.value, 1),
+                    "valu_per_simd_per_clk": round(rate / (mhz.value * 1e6), 4),
+                    "valu_per_simd_per_clk_at_2400mhz": round(rate / 2.4e9, 4)})
+    print(json.dumps({"valu_probe": res, "cus": cus, "waves_per_simd": blocks_per_cu,
+                      "iters": iters}), flush=True)
 
 
 C3_SIZES = (64, 128, 256, 512, 1024, 2048)
@@ -335,7 +398,12 @@ def main():
     ap.add_argument("--stress-spp", type=int, default=512)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device copy of this size (counter calibration)")
+    ap.add_argument("--valu-probe", action="store_true",
+                    help="only measure the VALU issue ceiling (its own JSON line)")
     args = ap.parse_args()
+    if args.valu_probe:
+        valu_probe()
+        return
     if args.copy_probe_mib:
         copy_probe(args.copy_probe_mib)
         return
@@ -405,7 +473,7 @@ def main():
     host_s = time.perf_counter() - ts
     log(f"with outputs copied to host memory: {host_s:.3f}s")
 
-    roofline, kernels = None, None
+    roofline, kernels, valu_roofline = None, None, None
     if not args.no_count:
         _, _, stc = b.bake(args.spp, args.bounces, seed=12345, flags=shb.BAKE_COUNT_VISITS)
         n_sh = stc["rays_shadow"]
@@ -437,6 +505,25 @@ def main():
                     "unit": "GB/s", "frac": round(kd["achieved_GBs"] / HBM_PEAK_GBS, 4),
                     "traffic": traffic.get(kd["kind"]),
                     "kernel": dom}
+        # VALU issue roofline of the same kernel: SQ_INSTS_VALU per launch
+        # (counted on this workload) over the live launch time, against the
+        # probe's mixed-stream ceiling (bench.py --valu-probe)
+        vi = read_profile_values(VALU_FILE, "valu_instr_per_launch")
+        vc = read_profile_values(VALU_FILE, "valu_issue_ceiling")
+        default_workload = (args.res, args.ss, args.spp, args.bounces, args.tris) == (
+            1024, 3, 128, 3, 262144)
+        if default_workload and kd["kind"] in vi and "mixed_4fma_6int" in vc:
+            simds = 4 * device_cus()
+            rate = vi[kd["kind"]] / (kd["avg_launch_ms"] * 1e-3 * simds * 2.4e9)
+            ceil = vc["mixed_4fma_6int"]
+            valu_roofline = {"bound": "valu-issue", "achieved": round(rate, 4), "peak": ceil,
+                             "unit": "wave64 VALU instructions / SIMD / clock at 2400 MHz",
+                             "frac": round(rate / ceil, 4),
+                             "valu_instr_per_launch": vi[kd["kind"]],
+                             "peak_source": "mixed FP32+integer probe stream (measured); "
+                                            f"FP32-only {vc.get('fp32_fma')}, integer-only "
+                                            f"{vc.get('int_add_rotate_xor')}",
+                             "kernel": dom}
         log(f"counted bake: {json.dumps(kernels)}")
 
     adjoint = None if args.no_adjoint else bench_adjoint(shb, scenes)
@@ -502,6 +589,7 @@ def main():
                    "rays_per_step": rays // args.steps},
         "roofline": roofline,
         "roofline_kernels": kernels,
+        "valu_roofline": valu_roofline,
         "adjoint_forward_ratio": adjoint["adjoint_forward_ratio"] if adjoint else None,
         "adjoint_512": adjoint,
         "batched_16_jobs": batched,
