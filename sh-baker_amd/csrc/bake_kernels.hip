@@ -301,7 +301,7 @@ __device__ __forceinline__ bool wave_tris(const SceneView& sc, const TravRay& r,
 constexpr uint32_t kFetch = 128;  // >= 64: one claim serves every idle lane
 
 template <bool ANY, bool COUNT, class Epi>
-__global__ void __launch_bounds__(256) k_trace_persistent(TraceArgs a, Epi epi) {
+__global__ void __launch_bounds__(256, 6) k_trace_persistent(TraceArgs a, Epi epi) {
   // a.stack_entries * blockDim.x stack entries, then (with a.wave_tris) one
   // WaveTriLds per wave of the block
   extern __shared__ __align__(16) uint2 stack_mem[];
