@@ -105,7 +105,7 @@ int shb_autotune(shb_ctx* h, const shb_bake_cfg* cfg, shb_tuning* best) {
   shb_tuning best_t = start;
   const int widths[] = {8, 4};
   const int blocks[] = {64, 256};
-  const int chunks[] = {0, 8, 16, 32};
+  const int chunks[] = {0, 8, 12, 16, 32};
   for (int w : widths) {
     cur.bvh_width = w;
     rc = shb_set_tuning(h, &cur);

@@ -41,7 +41,7 @@ struct Ctx {
   hipStream_t stream = nullptr;
   int num_cus = 0;
   std::string err;
-  shb_tuning tuning{64, 1 << 25, 8, 8};
+  shb_tuning tuning{256, 1 << 25, 12, 8};
   // device-resident result of the last shb_bake_lightmap (base resolution)
   const float* lm_sh = nullptr;
   const float* lm_vis = nullptr;
