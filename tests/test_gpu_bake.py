@@ -106,7 +106,10 @@ def test_bake_batch_size_invariant(baker, room):
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
 
 
-def test_bake_persistent_traversal_bit_identical(baker, room):
+@pytest.mark.parametrize("chunk,block", [(16, 64), (12, 256), (1, 128), (64, 256)])
+def test_bake_persistent_traversal_bit_identical(baker, room, chunk, block):
+    """The persistent traversal (refill threshold `chunk`, including the
+    default 12 / 256) gives the one-ray-per-thread kernel's bits."""
     s, _ = room
     baker.upload_scene(s)
     baker.rasterize(48, 48, 1, want=False)
@@ -114,7 +117,7 @@ def test_bake_persistent_traversal_bit_identical(baker, room):
     try:
         baker.set_tuning(queue_chunk=0, block_size=256)
         a, av, _ = baker.bake(16, 3, seed=21)
-        baker.set_tuning(queue_chunk=16, block_size=64)
+        baker.set_tuning(queue_chunk=chunk, block_size=block)
         b, bv, _ = baker.bake(16, 3, seed=21)
     finally:
         baker.set_tuning(**old)
