@@ -511,6 +511,15 @@ __global__ void __launch_bounds__(256, 5) k_shade_unit(BakeArgs a, int depth,
                                                        uint32_t* out_queue, uint32_t* out_count) {
   shade_stage<ADJ, false, true>(a, depth, in_queue, in_count, out_queue, out_count);
 }
+// The scaled form without spare slots (the fitting loop's bakes) with the
+// same cap: 40-44 bytes per lane spilled.
+template <bool ADJ>
+__global__ void __launch_bounds__(256, 5) k_shade_scaled(BakeArgs a, int depth,
+                                                         const uint32_t* in_queue,
+                                                         const uint32_t* in_count,
+                                                         uint32_t* out_queue, uint32_t* out_count) {
+  shade_stage<ADJ, false, false>(a, depth, in_queue, in_count, out_queue, out_count);
+}
 template <bool ADJ, bool SPLIT, bool UNIT>
 __global__ void __launch_bounds__(256) k_shade(BakeArgs a, int depth, const uint32_t* in_queue,
                                                const uint32_t* in_count, uint32_t* out_queue,
@@ -691,14 +700,20 @@ hipError_t launch_shade(const BakeArgs& b, int depth, const uint32_t* in_q,
                      out_q, out_n)
   const int v = (b.adj_grad ? 4 : 0) | (b.S ? 2 : 0) | (b.unit_scales ? 1 : 0);
   switch (v) {
-    case 0: SHB_SHADE(false, false, false); break;
+    case 0:
+      hipLaunchKernelGGL(k_shade_scaled<false>, dim3(grid), dim3(256), lds, st, b, depth, in_q,
+                         in_n, out_q, out_n);
+      break;
     case 1:
       hipLaunchKernelGGL(k_shade_unit<false>, dim3(grid), dim3(256), lds, st, b, depth, in_q, in_n,
                          out_q, out_n);
       break;
     case 2: SHB_SHADE(false, true, false); break;
     case 3: SHB_SHADE(false, true, true); break;
-    case 4: SHB_SHADE(true, false, false); break;
+    case 4:
+      hipLaunchKernelGGL(k_shade_scaled<true>, dim3(grid), dim3(256), lds, st, b, depth, in_q, in_n,
+                         out_q, out_n);
+      break;
     case 5:
       hipLaunchKernelGGL(k_shade_unit<true>, dim3(grid), dim3(256), lds, st, b, depth, in_q, in_n,
                          out_q, out_n);
