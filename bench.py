@@ -396,6 +396,8 @@ def main():
     ap.add_argument("--stress", action="store_true",
                     help="only the 4.2M-triangle / 256-light stress bake (its own JSON line)")
     ap.add_argument("--stress-spp", type=int, default=512)
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="only the adjoint / fitting measurement (its own JSON line)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device copy of this size (counter calibration)")
     ap.add_argument("--valu-probe", action="store_true",
@@ -420,6 +422,9 @@ def main():
         return
     if args.stress:
         print(json.dumps({"stress": bench_stress(shb, scenes, args.stress_spp)}), flush=True)
+        return
+    if args.adjoint_only:
+        print(json.dumps({"adjoint_512": bench_adjoint(shb, scenes)}), flush=True)
         return
 
     t0 = time.time()
