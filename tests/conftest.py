@@ -15,7 +15,9 @@ def pytest_configure(config):
     # compilers only; on a GPU box the prebuilt files travel with the tree).
     if not os.path.exists(os.path.join(ROOT, "oracle", "liboracle.so")):
         subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
-    if not os.path.exists(os.path.join(ROOT, "sh-baker_amd", "shb", "libshb_hip.so")):
+    shb_dir = os.path.join(ROOT, "sh-baker_amd", "shb")
+    if not all(os.path.exists(os.path.join(shb_dir, n))
+               for n in ("libshb_hip.so", "libshb_probe.so")):
         subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "sh-baker_amd"), "-j8"],
                        check=True)
     if not os.path.exists(os.path.join(ROOT, "tests", "cpp", "shb_baker_test")):
