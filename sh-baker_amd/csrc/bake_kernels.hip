@@ -501,29 +501,13 @@ __device__ __forceinline__ void shade_stage(const BakeArgs& a, int depth, const 
   if (ADJ) acc.flush();
 }
 
-// The common form (no spare slots, every scale 1) runs at 5 waves per SIMD
-// (a register cap with a small spill, faster: the light loop waits on a
-// scalar load per light); the others keep the registers they need.
-template <bool ADJ>
-__global__ void __launch_bounds__(256, 5) k_shade_unit(BakeArgs a, int depth,
-                                                       const uint32_t* in_queue,
-                                                       const uint32_t* in_count,
-                                                       uint32_t* out_queue, uint32_t* out_count) {
-  shade_stage<ADJ, false, true>(a, depth, in_queue, in_count, out_queue, out_count);
-}
-// The scaled form without spare slots (the fitting loop's bakes) with the
-// same cap: 40-44 bytes per lane spilled.
-template <bool ADJ>
-__global__ void __launch_bounds__(256, 5) k_shade_scaled(BakeArgs a, int depth,
-                                                         const uint32_t* in_queue,
-                                                         const uint32_t* in_count,
-                                                         uint32_t* out_queue, uint32_t* out_count) {
-  shade_stage<ADJ, false, false>(a, depth, in_queue, in_count, out_queue, out_count);
-}
+// Every form runs at 5 waves per SIMD: a register cap with a small spill
+// (24-88 bytes per lane by form) is faster than 4 spill-free waves, since
+// the light loop waits on a scalar load per light.
 template <bool ADJ, bool SPLIT, bool UNIT>
-__global__ void __launch_bounds__(256) k_shade(BakeArgs a, int depth, const uint32_t* in_queue,
-                                               const uint32_t* in_count, uint32_t* out_queue,
-                                               uint32_t* out_count) {
+__global__ void __launch_bounds__(256, 5) k_shade(BakeArgs a, int depth, const uint32_t* in_queue,
+                                                  const uint32_t* in_count, uint32_t* out_queue,
+                                                  uint32_t* out_count) {
   shade_stage<ADJ, SPLIT, UNIT>(a, depth, in_queue, in_count, out_queue, out_count);
 }
 
@@ -700,24 +684,12 @@ hipError_t launch_shade(const BakeArgs& b, int depth, const uint32_t* in_q,
                      out_q, out_n)
   const int v = (b.adj_grad ? 4 : 0) | (b.S ? 2 : 0) | (b.unit_scales ? 1 : 0);
   switch (v) {
-    case 0:
-      hipLaunchKernelGGL(k_shade_scaled<false>, dim3(grid), dim3(256), lds, st, b, depth, in_q,
-                         in_n, out_q, out_n);
-      break;
-    case 1:
-      hipLaunchKernelGGL(k_shade_unit<false>, dim3(grid), dim3(256), lds, st, b, depth, in_q, in_n,
-                         out_q, out_n);
-      break;
+    case 0: SHB_SHADE(false, false, false); break;
+    case 1: SHB_SHADE(false, false, true); break;
     case 2: SHB_SHADE(false, true, false); break;
     case 3: SHB_SHADE(false, true, true); break;
-    case 4:
-      hipLaunchKernelGGL(k_shade_scaled<true>, dim3(grid), dim3(256), lds, st, b, depth, in_q, in_n,
-                         out_q, out_n);
-      break;
-    case 5:
-      hipLaunchKernelGGL(k_shade_unit<true>, dim3(grid), dim3(256), lds, st, b, depth, in_q, in_n,
-                         out_q, out_n);
-      break;
+    case 4: SHB_SHADE(true, false, false); break;
+    case 5: SHB_SHADE(true, false, true); break;
     case 6: SHB_SHADE(true, true, false); break;
     default: SHB_SHADE(true, true, true); break;
   }
