@@ -468,10 +468,13 @@ __global__ void __launch_bounds__(256, 5) k_generate(BakeArgs a) {
 // Hit shading at depth d (ADJ: the replay form of the adjoint, with the
 // block's gradient sums in dynamic LDS; SPLIT: batches with spare slots for
 // alpha splits; UNIT: every light and albedo scale is 1).
+// Every form runs at 5 waves per SIMD: a register cap with a small spill
+// (24-76 bytes per lane by form) is faster than 4 spill-free waves, since
+// the light loop waits on a scalar load per light.
 template <bool ADJ, bool SPLIT, bool UNIT>
-__device__ __forceinline__ void shade_stage(const BakeArgs& a, int depth, const uint32_t* in_queue,
-                                            const uint32_t* in_count, uint32_t* out_queue,
-                                            uint32_t* out_count) {
+__global__ void __launch_bounds__(256, 5) k_shade(BakeArgs a, int depth, const uint32_t* in_queue,
+                                                  const uint32_t* in_count, uint32_t* out_queue,
+                                                  uint32_t* out_count) {
   extern __shared__ float grad_lds[];
   GradAcc acc;
   if (ADJ)
@@ -499,16 +502,6 @@ __device__ __forceinline__ void shade_stage(const BakeArgs& a, int depth, const 
     }
   }
   if (ADJ) acc.flush();
-}
-
-// Every form runs at 5 waves per SIMD: a register cap with a small spill
-// (24-88 bytes per lane by form) is faster than 4 spill-free waves, since
-// the light loop waits on a scalar load per light.
-template <bool ADJ, bool SPLIT, bool UNIT>
-__global__ void __launch_bounds__(256, 5) k_shade(BakeArgs a, int depth, const uint32_t* in_queue,
-                                                  const uint32_t* in_count, uint32_t* out_queue,
-                                                  uint32_t* out_count) {
-  shade_stage<ADJ, SPLIT, UNIT>(a, depth, in_queue, in_count, out_queue, out_count);
 }
 
 // Adjoint of the visible vertex NEE terms of one depth: the shadow queue's
