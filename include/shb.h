@@ -9,7 +9,10 @@
  * Threading: one context per host thread. A context is not safe to use from
  * several threads at once. All GPU work of a context runs on its own stream.
  * Results are deterministic bit for bit for a given (scene, surface, config,
- * seed), whatever the batch size, launch shapes, BVH width or job packing.
+ * seed, BVH width), whatever the batch size, launch shapes or job packing.
+ * Another BVH width builds another tree, which can change the hit of a ray
+ * whose t lies within rounding of a box bound (3 of 68.6 M extension and 2
+ * of 80.2 M shadow rays on a 262k-triangle room, no lightmap bit changed).
  *
  * Each entry point names the reference interface it replaces
  * (DaviesX/sh-baker, paths relative to the repository root). Host buffers are
@@ -175,7 +178,9 @@ typedef struct shb_bake_stats {
   uint32_t reserved;
 } shb_bake_stats;
 
-/* Launch parameters of the wavefront kernels (chosen by shb_autotune). */
+/* Launch parameters of the wavefront kernels (chosen by shb_autotune).
+ * A new context starts with block_size 256, batch_paths 2^25, queue_chunk
+ * 12 and bvh_width 8. Results do not depend on the first three.        */
 typedef struct shb_tuning {
   int32_t block_size;       /* threads per block of the traversal kernels   */
   int32_t batch_paths;      /* path slots resident per wavefront batch      */
