@@ -408,6 +408,23 @@ int shb_env_sh_latlong(shb_ctx* ctx, int32_t width, int32_t height,
                        int32_t channels, const float* pixels, float* sh_out,
                        double* seconds_device);
 
+/* shb_env_sh_latlong with no allocation: the map is already in device
+ * memory (pixels_device, width * height * channels floats) and the 27 SH
+ * floats go to device memory (sh_out_device); both must be device memory of
+ * ctx's GPU. The kernels' per-block partial sums live in a fixed 2 MiB array
+ * of the library, one per GPU, held until the process ends, so the call
+ * makes no hipMalloc / hipFree: maps needing more than 9708 blocks of
+ * partial sums (above about 16384 x 8192 texels) are rejected with
+ * SHB_ERR_LIMIT. Same kernels and summation order as shb_env_sh_latlong,
+ * so the result is the same bit for bit. Runs on ctx's stream and returns
+ * once it has drained. Calls that overlap on one GPU (from several host
+ * threads or contexts) share the array and race.
+ * No reference counterpart: for callers that cannot allocate device memory
+ * (code under stream capture, tight serving loops). */
+int shb_env_sh_latlong_noalloc(shb_ctx* ctx, int32_t width, int32_t height,
+                               int32_t channels, const float* pixels_device,
+                               float* sh_out_device);
+
 /* SH9 of the clamped Preetham/Perez clear sky for a sun direction (any
  * length; normalised first): 64 x 128 midpoint quadrature of the upper
  * hemisphere, radiance (0.2, 0.5, 0.9) * min(F, 30 F_zenith) / F_zenith.

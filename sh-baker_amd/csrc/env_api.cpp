@@ -3,6 +3,7 @@
 // sky is a fixed 64 x 128 quadrature evaluated on the host.
 #include <cmath>
 #include <cstring>
+#include <string>
 
 #include "api_util.h"
 #include "kernels.h"
@@ -80,7 +81,8 @@ int shb_env_sh_latlong(shb_ctx* h, int32_t width, int32_t height, int32_t channe
   hipError_t err = hipEventCreate(&e1);
   if (err == hipSuccess) err = hipEventRecord(e0, c.stream);
   if (err == hipSuccess)
-    err = launch_env_latlong(c.d_env.as<float>(), width, height, channels, g, part, out, c.stream);
+    err = launch_env_latlong(c.d_env.as<float>(), width, height, channels, g, part, out,
+                             nullptr, c.stream);
   if (err == hipSuccess) err = hipEventRecord(e1, c.stream);
   double sums[27];
   if (err == hipSuccess)
@@ -93,6 +95,43 @@ int shb_env_sh_latlong(shb_ctx* h, int32_t width, int32_t height, int32_t channe
   SHB_TRY(c, err);
   for (int i = 0; i < 27; ++i) sh_out[i] = (float)sums[i];
   if (seconds_device) *seconds_device = ms * 1e-3;
+  return SHB_OK;
+}
+
+int shb_env_sh_latlong_noalloc(shb_ctx* h, int32_t width, int32_t height, int32_t channels,
+                               const float* pixels_device, float* sh_out_device) {
+  if (!h) return SHB_ERR_INVALID_ARG;
+  Ctx& c = h->c;
+  if (!pixels_device || !sh_out_device)
+    return fail(c, SHB_ERR_INVALID_ARG, "pixels_device and sh_out_device must not be NULL");
+  if (width <= 0 || height <= 0 || channels < 3)
+    return fail(c, SHB_ERR_INVALID_ARG, "bad environment map size or channel count (< 3)");
+  if ((size_t)width * height > (size_t)1 << 31)
+    return fail(c, SHB_ERR_LIMIT, "environment map above 2^31 texels");
+  const EnvGrid g = env_grid(width, height);
+  const size_t nblocks = (size_t)g.gx * g.gy;
+  if ((nblocks + 1) * 27 * sizeof(double) > kEnvScratchBytes)
+    return fail(c, SHB_ERR_LIMIT,
+                "environment map needs " + std::to_string(nblocks) +
+                    " partial-sum blocks; the fixed scratch holds " +
+                    std::to_string(kEnvScratchBytes / (27 * sizeof(double)) - 1) +
+                    " (use shb_env_sh_latlong)");
+  SHB_TRY(c, hipSetDevice(c.device));
+  // both buffers must be device memory of this GPU (a host pointer would fault)
+  for (const void* p : {(const void*)pixels_device, (const void*)sh_out_device}) {
+    hipPointerAttribute_t at;
+    if (hipPointerGetAttributes(&at, p) != hipSuccess || at.type != hipMemoryTypeDevice ||
+        at.device != c.device) {
+      (void)hipGetLastError();  // clear the lookup's error state
+      return fail(c, SHB_ERR_INVALID_ARG,
+                  "pixels_device and sh_out_device must be device memory of the context's GPU");
+    }
+  }
+  double* part = static_cast<double*>(env_scratch_ptr(c.device));
+  if (!part) return fail(c, SHB_ERR_DEVICE, "the fixed scratch of this GPU was not set up");
+  SHB_TRY(c, launch_env_latlong(pixels_device, width, height, channels, g, part, nullptr,
+                                sh_out_device, c.stream));
+  SHB_TRY(c, hipStreamSynchronize(c.stream));
   return SHB_OK;
 }
 

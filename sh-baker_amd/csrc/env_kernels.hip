@@ -23,6 +23,7 @@
 // HBM-bound: each texel's three floats are read once; loads of 8 rows are
 // issued before their first use.
 #include <algorithm>
+#include <mutex>
 
 #include "kernels.h"
 
@@ -137,10 +138,12 @@ __global__ void __launch_bounds__(kEnvThreads)
   }
 }
 
-// Block i sums partials[i * nblocks + b] over b in a fixed order.
+// Block i sums partials[i * nblocks + b] over b in a fixed order into out[i]
+// (FP64) and / or out_f[i] (rounded to float); either may be null.
 constexpr int kFinishThreads = 256;
 __global__ void __launch_bounds__(kFinishThreads)
-    k_env_finish(const double* __restrict__ partials, int nblocks, double* __restrict__ out) {
+    k_env_finish(const double* __restrict__ partials, int nblocks, double* __restrict__ out,
+                 float* __restrict__ out_f) {
   __shared__ double red[kFinishThreads / 64];
   const double* p = partials + (size_t)blockIdx.x * nblocks;
   double s = 0.0;
@@ -151,7 +154,8 @@ __global__ void __launch_bounds__(kFinishThreads)
   if (threadIdx.x == 0) {
     double t = 0.0;
     for (int k = 0; k < kFinishThreads / 64; ++k) t += red[k];
-    out[blockIdx.x] = t;
+    if (out) out[blockIdx.x] = t;
+    if (out_f) out_f[blockIdx.x] = (float)t;
   }
 }
 
@@ -166,14 +170,52 @@ EnvGrid env_grid(int w, int h) {
 }
 
 hipError_t launch_env_latlong(const float* px, int w, int h, int ch, const EnvGrid& g,
-                              double* partials, double* out, hipStream_t st) {
+                              double* partials, double* out, float* out_f, hipStream_t st) {
   hipLaunchKernelGGL(k_env_latlong, dim3(g.gx, g.gy), dim3(kEnvThreads), 0, st, px, w, h, ch,
                      g.rows_per_block, partials);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_env_finish, dim3(27), dim3(kFinishThreads), 0, st, partials, g.gx * g.gy,
-                     out);
+                     out, out_f);
   return hipGetLastError();
+}
+
+// ------------------------------------------------ fixed scratch (no allocation)
+// The per-block partial sums of shb_env_sh_latlong_noalloc live in this
+// array, one copy per GPU for the life of the process. In host code its name
+// is only a stand-in: kernels get the device address that
+// hipGetSymbolAddress returns for the current device (env_scratch_init).
+// Every partial is written by k_env_latlong before k_env_finish reads it, so
+// nothing in the array has to start at zero; calls that overlap on one GPU
+// share it and race.
+static __device__ __align__(256) unsigned char scratch[kEnvScratchBytes];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxScratchDevices] = {};
+}  // namespace
+
+hipError_t env_scratch_init(int device) {
+  if (device < 0 || device >= kMaxScratchDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return hipSuccess;
+  int cur = -1;
+  hipError_t e = hipGetDevice(&cur);
+  if (e != hipSuccess) return e;
+  if (cur != device) return hipErrorInvalidDevice;  // the caller makes `device` current
+  void* p = nullptr;
+  e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  if (!p) return hipErrorInvalidSymbol;
+  g_scratch_ptr[device] = p;
+  return hipSuccess;
+}
+
+void* env_scratch_ptr(int device) {
+  if (device < 0 || device >= kMaxScratchDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return g_scratch_ptr[device];
 }
 
 }  // namespace shb

@@ -218,11 +218,20 @@ struct EnvGrid {
   int gx = 0, gy = 0, rows_per_block = 0;
 };
 EnvGrid env_grid(int w, int h);
+// Bytes of the fixed partial-sum scratch (2 MiB): (gx * gy + 1) * 27 FP64
+// must fit, i.e. up to 9708 blocks (a 16384 x 8192 map takes 8192).
+constexpr size_t kEnvScratchBytes = (size_t)2 << 20;
 
 #ifndef SHB_HOST_ONLY
-// SH9 of an RGB(A) equirectangular map (channels >= 3); out: 27 FP64.
+// SH9 of an RGB(A) equirectangular map (channels >= 3) into out (27 FP64)
+// and / or out_f (27 floats); partials: g.gx * g.gy * 27 FP64.
 hipError_t launch_env_latlong(const float* px, int w, int h, int ch, const EnvGrid& g,
-                              double* partials, double* out, hipStream_t st);
+                              double* partials, double* out, float* out_f, hipStream_t st);
+// Fixed device scratch of shb_env_sh_latlong_noalloc: kEnvScratchBytes per
+// GPU, its address looked up once per device (with that device current) by
+// env_scratch_init; env_scratch_ptr returns it (null before init).
+hipError_t env_scratch_init(int device);
+void* env_scratch_ptr(int device);
 // counts: num_tris + 1 uint32 scratch; scan: num_tris + 1 uint64 scratch.
 hipError_t launch_raster(const RasterArgs& a, uint32_t* counts, unsigned long long* scan,
                          hipStream_t st);

@@ -79,6 +79,12 @@ int shb_create(shb_ctx** out, int device) {
     return SHB_ERR_DEVICE;
   }
   c.num_cus = prop.multiProcessorCount;
+  // address of this GPU's copy of the fixed scratch (once per device)
+  if (env_scratch_init(device) != hipSuccess) {
+    (void)hipStreamDestroy(c.stream);
+    delete h;
+    return SHB_ERR_DEVICE;
+  }
   // default parameters: all scales 1 (reproduces the reference bake)
   *out = h;
   return SHB_OK;

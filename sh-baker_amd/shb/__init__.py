@@ -119,7 +119,7 @@ assert HIT_DTYPE.itemsize == 56
 EXPORTED = [
     "shb_create", "shb_destroy", "shb_last_error", "shb_version", "shb_upload_scene",
     "shb_rasterize", "shb_set_surface", "shb_validity_mask", "shb_bake", "shb_bake_lightmap", "shb_read_lightmap", "shb_save_lightmap",
-    "shb_env_sh_latlong", "shb_env_sh_preetham",
+    "shb_env_sh_latlong", "shb_env_sh_latlong_noalloc", "shb_env_sh_preetham",
     "shb_load_gltf", "shb_gltf_scene", "shb_gltf_environment", "shb_gltf_material_name",
     "shb_gltf_albedo_path", "shb_gltf_free", "shb_atlas_geometry_scales", "shb_atlas_create",
     "shb_atlas_scene", "shb_atlas_info", "shb_atlas_free",
@@ -154,6 +154,7 @@ def _load():
                                       C.c_char_p, C.c_size_t]
     lib.shb_env_sh_latlong.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, _fp, _fp,
                                        C.POINTER(C.c_double)]
+    lib.shb_env_sh_latlong_noalloc.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, vp, vp]
     lib.shb_env_sh_preetham.argtypes = [_fp, _fp]
     lib.shb_load_gltf.argtypes = [C.c_char_p, C.POINTER(vp), C.c_char_p, C.c_size_t]
     lib.shb_gltf_scene.restype = C.POINTER(ShbSceneDesc)
@@ -412,6 +413,13 @@ class Baker:
         sec = C.c_double(0.0)
         self._chk(lib.shb_env_sh_latlong(self.h, w, h, ch, ptr, _f(out), C.byref(sec)))
         return out, sec.value
+
+    def env_sh_latlong_noalloc(self, pixels_device, width, height, channels, sh_out_device):
+        """shb_env_sh_latlong_noalloc: pixels_device and sh_out_device are
+        device addresses (int) of width*height*channels and 27 floats."""
+        self._chk(lib.shb_env_sh_latlong_noalloc(self.h, width, height, channels,
+                                                 C.c_void_p(pixels_device),
+                                                 C.c_void_p(sh_out_device)))
 
     def set_params(self, light_scale=None, albedo_scale=None):
         ls = None if light_scale is None else np.ascontiguousarray(light_scale, np.float32)
