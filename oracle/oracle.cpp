@@ -596,6 +596,18 @@ static void AccumulateRadiance(V3 L, V3 d, float* sh) {
 }
 
 // ---------------------------------------------------------------- lights
+// Planted mistakes, for tests that show a check fails on them (0: none).
+// Never set by anything but those tests.
+enum {
+  FAULT_NONE = 0,
+  FAULT_CONT_NO_COSINE = 1,   // continuation weight brdf / pdf (no cos)
+  FAULT_NEE_NO_COSINE = 2,    // vertex NEE radiance x brdf (no receiver cos)
+  FAULT_SPOT_NO_FALLOFF = 3,  // spot cone ignored
+  FAULT_AREA_PDF_TOTAL = 4    // area pdf 1 / (sum of areas) with a uniform triangle pick
+};
+static int g_fault = FAULT_NONE;
+static inline float nee_cos(float c) { return g_fault == FAULT_NEE_NO_COSINE ? 1.0f : c; }
+
 struct AreaSample {
   V3 point, normal, radiance;
   float pdf = 0.f;
@@ -632,6 +644,14 @@ static AreaSample SampleAreaLightAt(const Scene& s, const Light& l,
   V3 emission = GetEmission(s.mats[l.mat], uv);
   float area = norm(cross(v0 - v1, v0 - v2)) / 2.f;
   float pdf = std::max(1e-6f, 1.f / num_triangles * 1.f / area);
+  if (g_fault == FAULT_AREA_PDF_TOTAL) {
+    float total = 0.f;
+    for (size_t t = 0; t < num_triangles; ++t) {
+      V3 a = geo.v[geo.idx[3 * t]], b = geo.v[geo.idx[3 * t + 1]], c = geo.v[geo.idx[3 * t + 2]];
+      total += norm(cross(a - b, a - c)) / 2.f;
+    }
+    pdf = std::max(1e-6f, 1.f / total);
+  }
   return {p, n, emission, pdf};
 }
 
@@ -664,7 +684,7 @@ static V3 LightRadiance(const Scene& s, const Light& l, const AreaSample* as,
       if (cos_n < 0.f) return V3{};
       V3 rad = l.intensity * l.color;
       if (!with_brdf) return rad;
-      return cmul(rad, cos_n * brdf(-l.dir));
+      return cmul(rad, nee_cos(cos_n) * brdf(-l.dir));
     }
     case SHB_LIGHT_POINT:  // light.h:72-109
     case SHB_LIGHT_SPOT: { // light.h:117-158
@@ -687,12 +707,13 @@ static V3 LightRadiance(const Scene& s, const Light& l, const AreaSample* as,
         float cos_l = dot(l.dir, -L);
         float falloff = (cos_l - l.cout) / (l.cin - l.cout);
         falloff = std::clamp(falloff, 0.0f, 1.0f);
+        if (g_fault == FAULT_SPOT_NO_FALLOFF) falloff = 1.0f;
         if (cos_n < 0.f) return V3{};
         rad = gpu_arith ? l.intensity * l.color * inv2 * falloff
                         : l.intensity * l.color / (dist * dist) * falloff;
       }
       if (!with_brdf) return rad;
-      return cmul(rad, cos_n * brdf(L));
+      return cmul(rad, nee_cos(cos_n) * brdf(L));
     }
     case SHB_LIGHT_AREA: {  // light.h:166-203
       V3 L = as->point - P;
@@ -708,7 +729,7 @@ static V3 LightRadiance(const Scene& s, const Light& l, const AreaSample* as,
       if (cos_n < 0.f) return V3{};
       V3 rad = gpu_arith ? as->radiance * (inv * inv) : as->radiance / (dist * dist);
       if (!with_brdf) return rad;
-      return cmul(rad, cos_n * brdf(L));
+      return cmul(rad, nee_cos(cos_n) * brdf(L));
     }
     default:
       return V3{};
@@ -835,7 +856,8 @@ static V3 TraceRef(const TraceCtx& cx, V3 origin, V3 dir, int depth, RefRng& r) 
   V3 incoming = TraceRef(cx, hit_pos, smp.dir, depth + 1, r);
   V3 brdf = EvalMaterial(mat, occ.uv, occ.normal, smp.dir, -dir);
   float cosine = dot(occ.normal, smp.dir);
-  V3 Lind = cmul(incoming, brdf) * (cosine / smp.pdf);
+  V3 Lind = cmul(incoming, brdf) *
+             ((g_fault == FAULT_CONT_NO_COSINE ? 1.0f : cosine) / smp.pdf);
   color += alpha * Lind;
   return color;
 }
@@ -969,7 +991,8 @@ static V3 TraceGpu(const GpuCtx& cx, V3 origin, V3 dir, int depth,
   if (is_zero(brdf) || depth + 1 > cx.max_depth) return color;
   V3 incoming = TraceGpu(cx, hit_pos, smp.dir, depth + 1, next_branch);
   float cosine = dot(occ.normal, smp.dir);
-  V3 Lind = cmul(incoming, brdf) * (cosine / smp.pdf);
+  V3 Lind = cmul(incoming, brdf) *
+             ((g_fault == FAULT_CONT_NO_COSINE ? 1.0f : cosine) / smp.pdf);
   color += alpha * Lind;
   return color;
 }
@@ -1213,6 +1236,9 @@ void orc_sh_accumulate(const float* radiance, const float* dir, float* sh27) {
 }
 
 float orc_srgb_to_linear(uint8_t x) { return SRGBToLinear(x); }
+
+// Plants mistake `fault` (FAULT_*) in every later call; 0 removes it.
+void orc_set_fault(int fault) { g_fault = fault; }
 
 // material.cpp:113-146 with given uniforms: out = dir[3], pdf
 void orc_sample_material(const float* n, float r1, float r2, float* out4) {

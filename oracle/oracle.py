@@ -58,6 +58,7 @@ def lib():
         L.orc_env_latlong.argtypes = [fp, C.c_int, C.c_int, C.c_int, fp,
                                       C.POINTER(C.c_double)]
         L.orc_env_preetham.argtypes = [fp, fp]
+        L.orc_set_fault.argtypes = [C.c_int]
         L.orc_srgb_to_linear.argtypes = [C.c_uint8]
         L.orc_srgb_to_linear.restype = C.c_float
         L.orc_sample_material.argtypes = [fp, C.c_float, C.c_float, fp]
@@ -134,6 +135,15 @@ def env_preetham(sun):
     out = np.zeros(27, np.float32)
     lib().orc_env_preetham(_f(_vec(sun, 3)), _f(out))
     return out
+
+
+FAULT_CONT_NO_COSINE, FAULT_NEE_NO_COSINE, FAULT_SPOT_NO_FALLOFF, FAULT_AREA_PDF_TOTAL = 1, 2, 3, 4
+
+
+def set_fault(fault):
+    """Plants a known mistake in the oracle (tests that show a check catches
+    it); 0 removes it."""
+    lib().orc_set_fault(fault)
 
 
 def srgb_to_linear(x):

@@ -280,6 +280,30 @@ class SceneBuilder:
         return self.add_geometry(p, np.repeat(n[None], 3, 0), np.repeat(np.append(t, 1)[None], 3, 0),
                                  [0, 1, 2], np.full(3, cid), loc, material)
 
+    def add_emitter_fan(self, center, size, k, material, rng):
+        """k triangles around `center` sharing it (a fan of uneven wedges and
+        radii, so the triangles' areas differ), horizontal, facing -y, with
+        texture uvs over their extent (for a textured emission)."""
+        c = np.asarray(center, np.float64)
+        cuts = np.sort(rng.uniform(0, 2 * math.pi, k))
+        cuts = np.append(cuts, cuts[0] + 2 * math.pi)
+        rad = size * rng.uniform(0.4, 1.0, k + 1)
+        rad[-1] = rad[0]
+        ring = np.stack([c[0] + rad * np.cos(cuts), np.full(k + 1, c[1]),
+                         c[2] + rad * np.sin(cuts)], 1)
+        pos = np.concatenate([c[None], ring[:-1]])
+        tri = []
+        for i in range(k):
+            # centre, next, this: counter-clockwise seen from below (normal -y)
+            tri += [0, 1 + (i + 1) % k, 1 + i]
+        n = np.repeat(np.array([[0.0, -1.0, 0.0]]), len(pos), 0)
+        tan = np.repeat(np.array([[1.0, 0.0, 0.0, 1.0]]), len(pos), 0)
+        lo = pos[:, [0, 2]].min(0)
+        ext = np.maximum(pos[:, [0, 2]].max(0) - lo, 1e-6)
+        tuv = (pos[:, [0, 2]] - lo) / ext
+        cid, loc = self.new_chart(pos[:, [0, 2]])
+        return self.add_geometry(pos, n, tan, tri, np.full(len(pos), cid), loc, material, tuv=tuv)
+
     def add_light(self, **kw):
         self.lights.append(kw)
         return len(self.lights) - 1
@@ -398,7 +422,8 @@ SYNTH_SEED = 0x5EED0002
 
 def synthetic_room_scene(width=1024, height=1024, seed=SYNTH_SEED, n_pillars=400, n_boxes=2000,
                          n_emissive=64, n_point=4, n_materials=128, segments=32, rings=8,
-                         target_tris=262144, padding=2, n_spot=0) -> Scene:
+                         target_tris=262144, padding=2, n_spot=0, emitter_tris=1,
+                         textured_emission=False, emitter_y=(13.0, 14.5)) -> Scene:
     """Sponza-scale synthetic room: closed room 40 x 15 x 20 m
     (x in [-20,20], y in [0,15], z in [-10,10]) with `n_pillars` 32-segment
     cylinders (radius U[0.2,0.6], full height, `rings` rings, 30-triangle fan
@@ -407,7 +432,11 @@ def synthetic_room_scene(width=1024, height=1024, seed=SYNTH_SEED, n_pillars=400
     albedos U[0.2,0.8]^3 (sRGB8), `n_point` point lights (log-U[10,200]),
     one directional light and `n_emissive` emissive triangles (white,
     emission U[1,20]) near the ceiling, each its own area light; `n_spot`
-    downward spot lights (log-U[20,400], cones 20-35 / 35-50 degrees)."""
+    downward spot lights (log-U[20,400], cones 20-35 / 35-50 degrees).
+    emitter_tris > 1 makes each emitter a fan of that many triangles of
+    uneven areas (one area light each); textured_emission gives each
+    emitter a random 4x4 sRGB texture instead of white; emitters sit at
+    heights U[emitter_y]."""
     rng = np.random.default_rng(seed)
     b = SceneBuilder()
     mats = []
@@ -433,8 +462,19 @@ def synthetic_room_scene(width=1024, height=1024, seed=SYNTH_SEED, n_pillars=400
         c = (rng.uniform(-19, 19), rng.uniform(1, 14), rng.uniform(-9, 9))
         b.add_box(c, half, random_rotation(rng), material=mats[int(rng.integers(0, n_materials))])
     for _ in range(n_emissive):
+        if emitter_tris > 1:
+            c = np.array([rng.uniform(-19, 19), rng.uniform(*emitter_y), rng.uniform(-9, 9)])
+            em_i = float(rng.uniform(1, 20))
+            if textured_emission:
+                px = rng.integers(60, 256, (4, 4, 3)).astype(np.uint8)
+                em = b.add_material(pixels=px, emission=em_i)
+            else:
+                em = b.add_material((255, 255, 255, 255), emission=em_i)
+            g = b.add_emitter_fan(c, rng.uniform(0.3, 1.0), emitter_tris, em, rng)
+            b.add_light(type=LIGHT_AREA, geometry_index=g, material_index=em, intensity=em_i)
+            continue
         size = rng.uniform(0.3, 1.0)
-        c = np.array([rng.uniform(-19, 19), rng.uniform(13.0, 14.5), rng.uniform(-9, 9)])
+        c = np.array([rng.uniform(-19, 19), rng.uniform(*emitter_y), rng.uniform(-9, 9)])
         ang = rng.uniform(0, 2 * math.pi)
         pts = [c + size * np.array([math.cos(ang + k * 2 * math.pi / 3), 0,
                                     math.sin(ang + k * 2 * math.pi / 3)]) for k in range(3)]

@@ -595,3 +595,20 @@ def test_cwbvh_recount_on_a_hand_built_tree():
     ha, nva, tta = orc.cwbvh_trace(nodes, tris, rays, any_hit=True)
     assert list(ha["prim_id"] >= 0) == [True, True, False, True]
     assert list(nva) == [1, 2, 1, 1] and list(tta) == [2, 1, 0, 1]
+
+
+def test_two_streams_agree_with_spot_and_area_lights():
+    """The two streams on the room with spot lights and 3-triangle textured
+    emitters (the rng.index triangle pick of the GPU stream against the
+    reference's uniform_int_distribution)."""
+    s = scenes.scaled_synthetic_scene(10000, 96, 77, n_spot=4, emitter_tris=3,
+                                      textured_emission=True, emitter_y=(14.3, 14.8))
+    o = orc.Oracle(s)
+    pts = o.rasterize(96, 96, 1)
+    v = pts["material_id"] >= 0
+    g, _, gv, _ = o.bake(pts, 96, 96, 64, 3, mode=orc.MODE_GPU, seed=99)
+    r, _, rv, _ = o.bake(pts, 96, 96, 64, 3, mode=orc.MODE_REF)
+    live = (gv[v] + rv[v]) > 0
+    z = (g[v] - r[v])[live] / np.sqrt(gv[v] + rv[v])[live]
+    assert 0.6 < np.abs(z).mean() < 1.0, np.abs(z).mean()
+    assert (np.abs(z) > 4.5).mean() < 2e-3
