@@ -368,6 +368,46 @@ def bench_adjoint(shb, scenes, res=512, spp=64, bounces=3, iters=50, reps=3):
     return res_d
 
 
+def cpu_share():
+    """(threads, description) of the host cores this job may use: the CPU
+    share the environment declares for it (OMP_NUM_THREADS, as a one-GPU
+    job's share of a shared host is set), else every core of the process's
+    affinity mask; nproc and the affinity are reported beside it."""
+    aff = len(os.sched_getaffinity(0))
+    declared = os.environ.get("OMP_NUM_THREADS")
+    threads = int(declared) if declared and declared.isdigit() and int(declared) > 0 else aff
+    threads = min(threads, aff)
+    return threads, {"nproc": os.cpu_count(), "affinity_cores": aff,
+                     "omp_num_threads": declared, "threads_used": threads}
+
+
+def per_texel_agreement(g_sh, r_sh, r_var, g_var, pts, width, stride):
+    """z = (GPU - reference stream) / sqrt(var_ref + var_gpu) per texel and
+    coefficient on the valid texels of every `stride`-th row, each var the
+    stream's own sample variance of its per-texel mean. Both are needed: a
+    texel lit by rare, large terms (an emitter seen by few primary rays) can
+    have a sample variance of 0 in one stream and not in the other.
+    Coefficients with zero variance in both (zero or deterministic terms)
+    are left out and counted."""
+    rows = (np.arange(pts.size) // width) % stride == 0
+    v = (pts["material_id"] >= 0) & rows
+    var = r_var[v].astype(np.float64) + g_var[v].astype(np.float64)
+    live = var > 0
+    z = (g_sh[v].astype(np.float64) - r_sh[v])[live] / np.sqrt(var[live])
+    az = np.abs(z)
+    dead = ~live
+    return {"texels": int(v.sum()), "coefficients": int(live.sum()),
+            "coefficients_zero_variance": int(dead.sum()),
+            "zero_variance_equal": bool(np.allclose(g_sh[v][dead], r_sh[v][dead], rtol=1e-5, atol=0)),
+            "texels_black_in_both": int(((np.abs(g_sh[v]).max(1) == 0) &
+                                         (np.abs(r_sh[v]).max(1) == 0)).sum()),
+            "share_within_3_sigma": round(float((az <= 3.0).mean()), 5),
+            "mean_abs_z": round(float(az.mean()), 4),
+            "share_above_4_5_sigma": round(float((az > 4.5).mean()), 6),
+            "gaussian_expectation": {"share_within_3_sigma": 0.9973, "mean_abs_z": 0.7979},
+            "reference": "oracle, std::mt19937 reference stream, same texels and spp"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -541,15 +581,17 @@ def main():
         small = bench_batched(shb, scenes, small_jobs())
 
     cpu = None
+    agreement = None
     if args.cpu_seconds > 0:
         from oracle import oracle as orc
-        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        threads, share = cpu_share()
+        threads = args.cpu_threads or threads
         o = orc.Oracle(scene)
         W = args.res * args.ss
         stride = 1024
         while True:
-            _, _, _, cst = o.bake(pts, W, W, args.spp, args.bounces, mode=orc.MODE_REF,
-                                  threads=threads, row_stride=stride)
+            r_sh, _, r_var, cst = o.bake(pts, W, W, args.spp, args.bounces,
+                                         mode=orc.MODE_REF, threads=threads, row_stride=stride)
             if cst["seconds"] > args.cpu_seconds / 8 or stride == 1:
                 break
             stride = max(1, stride // 4)
@@ -557,8 +599,8 @@ def main():
         want = max(1, int(stride * cst["seconds"] / args.cpu_seconds))
         if want < stride:
             stride = want
-            _, _, _, cst = o.bake(pts, W, W, args.spp, args.bounces, mode=orc.MODE_REF,
-                                  threads=threads, row_stride=stride)
+            r_sh, _, r_var, cst = o.bake(pts, W, W, args.spp, args.bounces,
+                                         mode=orc.MODE_REF, threads=threads, row_stride=stride)
         crays = cst["rays_extension"] + cst["rays_shadow"]
         cpu_mrays = crays / cst["seconds"] / 1e6
         frac_texels = cst["valid_texels"] / max(valid, 1)
@@ -568,8 +610,20 @@ def main():
                          f"({cst['valid_texels']} texels, {frac_texels:.4f} of the valid ones), "
                          f"{cst['seconds']:.1f}s; std::mt19937 reference stream, host BVH2 "
                          f"(Embree unavailable)",
-               "bake_seconds_extrapolated": round(cst["seconds"] / max(frac_texels, 1e-12), 1)}
+               "bake_seconds_extrapolated": round(cst["seconds"] / max(frac_texels, 1e-12), 1),
+               "host": share}
         log(f"cpu baseline: {cpu}")
+        # per-texel agreement of the GPU bake with the reference-stream bake
+        # on the texels the CPU leg baked (its last, timed run); the GPU's
+        # per-texel variance comes from the oracle's GPU-stream mode, which
+        # draws the GPU's samples
+        g_sh = b.bake(args.spp, args.bounces, seed=12345)[0]
+        ts = time.perf_counter()
+        _, _, g_var, _ = o.bake(pts, W, W, args.spp, args.bounces, mode=orc.MODE_GPU,
+                                seed=12345, threads=threads, row_stride=stride)
+        agreement = per_texel_agreement(g_sh, r_sh, r_var, g_var, pts, W, stride)
+        log(f"agreement ({time.perf_counter() - ts:.1f}s for the GPU-stream variances): "
+            f"{agreement}")
 
     out = {
         "metric": "Mrays/s + bake seconds @1024^2, 128spp, 3 bounces; adjoint/forward time ratio",
@@ -602,6 +656,7 @@ def main():
         "environment_sh": env,
         "stress_4m_tris_256_lights": stress,
         "cpu_baseline": cpu,
+        "agreement": agreement,
     }
     if cpu:
         out["speedup_vs_cpu"] = round(mrays / cpu["value"], 1)

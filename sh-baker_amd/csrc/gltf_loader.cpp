@@ -222,6 +222,43 @@ int type_count(const std::string& t) {
   return 0;
 }
 
+// Sizes and offsets of the file are JSON numbers (doubles). They are taken
+// only when finite, integral, non-negative and at most kMaxFileSize, so the
+// byte arithmetic below (sums of a few such values, products of a count with
+// a stride of at most kMaxStride) stays far from 64-bit wrap-around.
+constexpr double kMaxFileSize = 1099511627776.0;  // 2^40 bytes
+constexpr size_t kMaxStride = 1u << 16;
+constexpr size_t kMaxZeroElements = (size_t)1 << 27;  // accessors without a bufferView
+
+bool size_field(Ctx& c, const Value& v, const char* key, size_t dflt, size_t* out,
+                const std::string& what) {
+  const Value* m = v.get(key);
+  if (!m) {
+    *out = dflt;
+    return true;
+  }
+  const double x = m->is_number() ? m->num : -1.0;
+  if (!(x >= 0.0) || !(x <= kMaxFileSize) || x != std::floor(x)) {
+    c.err = what + " has an invalid " + key;
+    return false;
+  }
+  *out = (size_t)x;
+  return true;
+}
+
+// [offset, offset + length) of a bufferView, checked against its buffer.
+bool view_range(Ctx& c, const Value& bv, size_t buf_size, size_t* off, size_t* len,
+                const std::string& what) {
+  if (!size_field(c, bv, "byteOffset", 0, off, what) ||
+      !size_field(c, bv, "byteLength", 0, len, what))
+    return false;
+  if (*off > buf_size || *len > buf_size - *off) {
+    c.err = what + " lies beyond its buffer";
+    return false;
+  }
+  return true;
+}
+
 // accessor elements as doubles (ncomp per element); normalized integers are
 // mapped to [0, 1] / [-1, 1] as the glTF specification defines
 bool read_accessor(Ctx& c, int ai, int want_comps, std::vector<double>* out, size_t* count,
@@ -235,10 +272,12 @@ bool read_accessor(Ctx& c, int ai, int want_comps, std::vector<double>* out, siz
     c.err = "sparse accessors are not supported";
     return false;
   }
+  const std::string what = "accessor " + std::to_string(ai);
   const int ct = shb::json::integer(*a, "componentType", 0);
   const int nc = type_count(shb::json::string(*a, "type"));
   const int cs = comp_size(ct);
-  const size_t n = (size_t)shb::json::num(*a, "count", 0);
+  size_t n = 0;
+  if (!size_field(c, *a, "count", 0, &n, what)) return false;
   if (!cs || nc < want_comps || (want_comps && nc != want_comps)) {
     c.err = "accessor " + std::to_string(ai) + " has an unexpected type";
     return false;
@@ -248,10 +287,16 @@ bool read_accessor(Ctx& c, int ai, int want_comps, std::vector<double>* out, siz
     c.err = "accessor " + std::to_string(ai) + " is not FLOAT";
     return false;
   }
-  out->assign(n * nc, 0.0);
-  *count = n;
   const int bvi = shb::json::integer(*a, "bufferView", -1);
-  if (bvi < 0) return true;  // all zeros
+  if (bvi < 0) {  // all zeros
+    if (n > kMaxZeroElements) {
+      c.err = what + " has too many elements for an accessor without a bufferView";
+      return false;
+    }
+    out->assign(n * nc, 0.0);
+    *count = n;
+    return true;
+  }
   const Value* bv = arr_at(c.root, "bufferViews", bvi);
   if (!bv) {
     c.err = "bufferView out of range";
@@ -264,14 +309,26 @@ bool read_accessor(Ctx& c, int ai, int want_comps, std::vector<double>* out, siz
   }
   const std::vector<uint8_t>& buf = c.buffers[bi];
   const size_t elem = (size_t)cs * nc;
-  const size_t stride = (size_t)shb::json::num(*bv, "byteStride", 0) ? (size_t)shb::json::num(*bv, "byteStride", 0) : elem;
-  const size_t base = (size_t)shb::json::num(*bv, "byteOffset", 0) + (size_t)shb::json::num(*a, "byteOffset", 0);
-  const size_t view_len = (size_t)shb::json::num(*bv, "byteLength", 0);
-  const size_t view_end = (size_t)shb::json::num(*bv, "byteOffset", 0) + view_len;
-  if (n && (base + (n - 1) * stride + elem > std::min(view_end, buf.size()))) {
-    c.err = "accessor " + std::to_string(ai) + " reads past its buffer view";
+  size_t view_off = 0, view_len = 0, acc_off = 0, stride = 0;
+  if (!view_range(c, *bv, buf.size(), &view_off, &view_len, "bufferView " + std::to_string(bvi)) ||
+      !size_field(c, *a, "byteOffset", 0, &acc_off, what) ||
+      !size_field(c, *bv, "byteStride", 0, &stride, "bufferView " + std::to_string(bvi)))
+    return false;
+  if (stride == 0) stride = elem;
+  if (stride < elem || stride > kMaxStride) {
+    c.err = "bufferView " + std::to_string(bvi) + " has a byteStride below its element size "
+            "or above 65536";
     return false;
   }
+  // elements [0, n) at acc_off + i * stride, each elem bytes, inside the view
+  if (acc_off > view_len ||
+      (n && (elem > view_len - acc_off || n - 1 > (view_len - acc_off - elem) / stride))) {
+    c.err = what + " reads past its buffer view";
+    return false;
+  }
+  const size_t base = view_off + acc_off;
+  out->assign(n * nc, 0.0);
+  *count = n;
   for (size_t i = 0; i < n; ++i) {
     const uint8_t* e = &buf[base + i * stride];
     for (int k = 0; k < nc; ++k) {
@@ -378,12 +435,12 @@ bool load_image(Ctx& c, int img_index, shb::Image8* img, std::string* path) {
       return false;
     }
     const int bi = shb::json::integer(*bv, "buffer", -1);
-    const size_t off = (size_t)shb::json::num(*bv, "byteOffset", 0);
-    const size_t len = (size_t)shb::json::num(*bv, "byteLength", 0);
-    if (bi < 0 || (size_t)bi >= c.buffers.size() || off + len > c.buffers[bi].size()) {
+    size_t off = 0, len = 0;
+    if (bi < 0 || (size_t)bi >= c.buffers.size()) {
       c.err = "image bufferView out of range";
       return false;
     }
+    if (!view_range(c, *bv, c.buffers[bi].size(), &off, &len, "image bufferView")) return false;
     bytes.assign(c.buffers[bi].begin() + off, c.buffers[bi].begin() + off + len);
     path->clear();
   }

@@ -45,9 +45,12 @@ inline double num(const Value& v, const char* key, double d) {
   const Value* m = v.get(key);
   return m && m->is_number() ? m->num : d;
 }
+// An int field; absent, not a number, NaN or outside the int range: d.
 inline int integer(const Value& v, const char* key, int d) {
   const Value* m = v.get(key);
-  return m && m->is_number() ? (int)m->num : d;
+  if (!m || !m->is_number()) return d;
+  const double x = m->num;
+  return x >= -2147483648.0 && x <= 2147483647.0 ? (int)x : d;
 }
 inline std::string string(const Value& v, const char* key, const std::string& d = "") {
   const Value* m = v.get(key);

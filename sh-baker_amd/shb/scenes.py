@@ -233,6 +233,59 @@ class SceneBuilder:
                                  max(1, round(np.linalg.norm(e2) / cell))))
         return self.add_quads(faces, material, grid if cell else None)
 
+    def add_boxes(self, centers, halves, Rs, materials):
+        """add_box for many unsubdivided outward boxes at once: the same
+        vertices, normals, tangents, charts and triangles, bit for bit (the
+        lengths are taken with the per-vector dot that np.linalg.norm uses),
+        with the arithmetic done over all faces together."""
+        centers = np.asarray(centers, np.float64)
+        halves = np.asarray(halves, np.float64)
+        Rs = np.asarray(Rs, np.float64)
+        n = len(centers)
+        P0, E1, E2 = [], [], []
+        for a in range(3):
+            b, c = (a + 1) % 3, (a + 2) % 3
+            ax, bx, cx = Rs[:, :, a], Rs[:, :, b], Rs[:, :, c]
+            for sgn in (1, -1):
+                p0 = (centers + (sgn * halves[:, a])[:, None] * ax - halves[:, b][:, None] * bx
+                      - halves[:, c][:, None] * cx)
+                eb, ec = (2 * halves[:, b])[:, None] * bx, (2 * halves[:, c])[:, None] * cx
+                e1, e2 = (eb, ec) if sgn > 0 else (ec, eb)
+                P0.append(p0)
+                E1.append(e1)
+                E2.append(e2)
+        P0, E1, E2 = (np.stack(v, 1) for v in (P0, E1, E2))  # (n, 6, 3)
+
+        def lengths(v):
+            f = v.reshape(-1, 3)
+            return np.sqrt(np.array([x.dot(x) for x in f])).reshape(v.shape[:-1])
+
+        nrm = np.cross(E1, E2)
+        nrm /= lengths(nrm)[..., None]
+        l1, l2 = lengths(E1), lengths(E2)
+        t = E1 / l1[..., None]
+        uu = np.array([0.0, 1.0, 0.0, 1.0])
+        vv = np.array([0.0, 0.0, 1.0, 1.0])
+        pos = P0[:, :, None] + uu[None, None, :, None] * E1[:, :, None] + \
+            vv[None, None, :, None] * E2[:, :, None]  # (n, 6, 4, 3)
+        loc = np.stack([uu[None, None] * l1[..., None], vv[None, None] * l2[..., None]], -1)
+        tri = []
+        for fi in range(6):
+            a4 = 4 * fi
+            tri += [a4, a4 + 1, a4 + 3, a4, a4 + 3, a4 + 2]
+        nrm4 = np.repeat(nrm[:, :, None], 4, 2)
+        tan4 = np.concatenate([np.repeat(t[:, :, None], 4, 2), np.ones((n, 6, 4, 1))], -1)
+        first = []
+        for k in range(n):
+            base_chart = len(self.charts)
+            for fi in range(6):
+                self.charts.append((float(l1[k, fi]), float(l2[k, fi])))
+            ch = np.repeat(np.arange(base_chart, base_chart + 6), 4)
+            first.append(self.add_geometry(pos[k].reshape(24, 3), nrm4[k].reshape(24, 3),
+                                           tan4[k].reshape(24, 4), tri, ch,
+                                           loc[k].reshape(24, 2), materials[k]))
+        return first
+
     def add_cylinder(self, cx, cz, r, y0, y1, segments, rings, material):
         th = np.arange(segments + 1) / segments * 2 * math.pi
         ys = y0 + (y1 - y0) * np.arange(rings + 1) / rings
@@ -323,16 +376,21 @@ class SceneBuilder:
             if ws.max() > width:
                 return None
             x = y = row_h = 0
-            pos = np.zeros((len(ext), 2), np.int64)
-            for ci in order:
-                w, h = ws[ci], hs[ci]
+            wl, hl = ws[order].tolist(), hs[order].tolist()  # plain ints: a fast loop
+            px, py = [0] * len(wl), [0] * len(wl)
+            for k in range(len(wl)):
+                w, h = wl[k], hl[k]
                 if x + w > width:
                     x, y, row_h = 0, y + row_h, 0
                 if y + h > height:
                     return None
-                pos[ci] = (x, y)
+                px[k], py[k] = x, y
                 x += w
-                row_h = max(row_h, h)
+                if h > row_h:
+                    row_h = h
+            pos = np.zeros((len(ext), 2), np.int64)
+            pos[order, 0] = px
+            pos[order, 1] = py
             return pos
 
         lo = 0.0
@@ -457,10 +515,14 @@ def synthetic_room_scene(width=1024, height=1024, seed=SYNTH_SEED, n_pillars=400
         cx = rng.uniform(-20 + r, 20 - r)
         cz = rng.uniform(-10 + r, 10 - r)
         b.add_cylinder(cx, cz, r, 0.0, 15.0, segments, rings, mats[int(rng.integers(0, n_materials))])
+    centers, halves, Rs, bm = [], [], [], []
     for _ in range(n_boxes):
-        half = rng.uniform(0.2, 2.0, 3) / 2
-        c = (rng.uniform(-19, 19), rng.uniform(1, 14), rng.uniform(-9, 9))
-        b.add_box(c, half, random_rotation(rng), material=mats[int(rng.integers(0, n_materials))])
+        halves.append(rng.uniform(0.2, 2.0, 3) / 2)
+        centers.append((rng.uniform(-19, 19), rng.uniform(1, 14), rng.uniform(-9, 9)))
+        Rs.append(random_rotation(rng))
+        bm.append(mats[int(rng.integers(0, n_materials))])
+    if n_boxes:
+        b.add_boxes(centers, halves, Rs, bm)
     for _ in range(n_emissive):
         if emitter_tris > 1:
             c = np.array([rng.uniform(-19, 19), rng.uniform(*emitter_y), rng.uniform(-9, 9)])

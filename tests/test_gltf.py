@@ -356,3 +356,40 @@ def test_hdr_run_length_scanlines(tmp_path):
         fh.write(b"#?RGBE\n\n" + f"-Y {h} +X {w}\n".encode() + b"".join(rows))
     _, env = shb.load_gltf(quad_gltf(tmp_path, UV_PLAIN, extra={"skybox": "sky.hdr"}))
     np.testing.assert_array_equal(env["pixels"], exp)
+
+
+@pytest.mark.parametrize("field,where,value", [
+    ("count", "accessor", 2 ** 62),            # count * components would wrap
+    ("count", "accessor", -4),
+    ("count", "accessor", 3.5),
+    ("byteStride", "view", 2 ** 61),           # (count - 1) * stride would wrap
+    ("byteStride", "view", 4),                 # below the element size (12)
+    ("byteOffset", "accessor", 2 ** 63),
+    ("byteOffset", "view", -8),
+    ("byteLength", "view", 10 ** 300),
+    ("byteOffset", "view", 120),               # in range, but the view then ends past the buffer
+])
+def test_malformed_accessor_sizes_fail_cleanly(tmp_path, field, where, value):
+    """Sizes from the file are checked before any arithmetic: a crafted count,
+    stride or offset fails the load with a message instead of wrapping a
+    product and writing past a vector."""
+    p = quad_gltf(tmp_path, UV_PLAIN)
+    g = json.load(open(p))
+    acc = g["accessors"][0]  # POSITION: VEC3 float, 12-byte elements
+    target = acc if where == "accessor" else g["bufferViews"][acc["bufferView"]]
+    target[field] = value
+    open(p, "w").write(json.dumps(g))
+    with pytest.raises(shb.ShbError) as e:
+        shb.load_gltf(p)
+    assert e.value.code == -1 and str(e.value)
+
+
+def test_accessor_without_buffer_view_is_bounded(tmp_path):
+    p = quad_gltf(tmp_path, UV_PLAIN)
+    g = json.load(open(p))
+    del g["accessors"][0]["bufferView"]
+    g["accessors"][0]["count"] = 2 ** 31
+    open(p, "w").write(json.dumps(g))
+    with pytest.raises(shb.ShbError) as e:
+        shb.load_gltf(p)
+    assert "too many elements" in str(e.value)
