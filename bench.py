@@ -284,6 +284,16 @@ def bench_stress(shb, scenes, spp=512):
     t0 = time.perf_counter()
     _, _, _, st = b.bake_lightmap(spp, 5, 4, seed=12345, to_host=False)
     bake_s = time.perf_counter() - t0
+    # the configuration's batched form: 64 tile jobs (8 x 8 tiles of 1024^2
+    # supersampled texels) in one batched run, against the plain raw bake
+    # (both leave the result on the device)
+    tiles = [(x, y, 1024, 1024) for y in range(0, 8192, 1024) for x in range(0, 8192, 1024)]
+    t0 = time.perf_counter()
+    _, _, pst = b.bake(spp, 5, seed=12345, want=False)
+    plain_raw_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    _, _, tst = b.bake_tiled(spp, 5, tiles, seed=12345, want=False)
+    tiled_s = time.perf_counter() - t0
     b.close()
     rays = st["rays_extension"] + st["rays_shadow"]
     res = {"config": "stress: 4.2M-triangle room, 256 lights, 4096^2 ss2, 5 bounces, dilation 4",
@@ -293,7 +303,13 @@ def bench_stress(shb, scenes, spp=512):
            "bake_seconds_scaled_to_512spp": round(bake_s * 512 / spp, 1),
            "upload_bvh_seconds": round(up_s, 2), "raster_seconds": round(ra_s, 3),
            "scene_generation_seconds": round(gen_s, 1),
-           "note": "one context (no 64-tile batched form: tiles of one surface are not jobs)"}
+           "tiled_64_jobs": {"plain_bake_seconds": round(plain_raw_s, 3),
+                             "tiled_bake_seconds": round(tiled_s, 3),
+                             "speedup": round(plain_raw_s / tiled_s, 3),
+                             "rays_equal": pst["rays_extension"] + pst["rays_shadow"] ==
+                             tst["rays_extension"] + tst["rays_shadow"],
+                             "mrays_s": round((tst["rays_extension"] + tst["rays_shadow"]) /
+                                              tiled_s / 1e6, 1)}}
     log(f"stress: {json.dumps(res)}")
     return res
 

@@ -443,6 +443,23 @@ int shb_bake_batched(shb_ctx* const* ctxs, int32_t num_jobs,
                      const shb_bake_cfg* cfgs, float* const* sh_outs,
                      float* const* vis_outs, shb_bake_stats* stats);
 
+/* A tile of a tiled bake: supersampled texels [x0, x0 + w) x [y0, y0 + h). */
+typedef struct shb_tile {
+  int32_t x0, y0, w, h;
+} shb_tile;
+
+/* Bakes the current surface of ctx as num_tiles (at most 64) tile jobs in
+ * one batched run: the tiles share the context's scene and BVH and the
+ * wavefront batches, and the launch sequence is captured in a hipGraph.
+ * Tiles must lie inside the supersampled surface and must not overlap;
+ * texels outside every tile stay 0. A texel in a tile gets the same bits as
+ * from shb_bake (same random stream). sh_out / vis_out as in shb_bake (NULL
+ * keeps that output on the device only). No reference counterpart: the
+ * reference bakes one whole lightmap per call (src/baker.cpp:186-304). */
+int shb_bake_tiled(shb_ctx* ctx, const shb_bake_cfg* cfg, const shb_tile* tiles,
+                   int32_t num_tiles, float* sh_out, float* vis_out,
+                   shb_bake_stats* stats);
+
 /* Chooses block size, batch size, refill threshold and BVH width by timing
  * short bakes of the current scene and surface; applies and returns them. */
 int shb_autotune(shb_ctx* ctx, const shb_bake_cfg* cfg, shb_tuning* best);

@@ -3,6 +3,7 @@
 // lightmaps through the same batches.
 #include "bake_run.h"
 
+#include <algorithm>
 #include <cstring>
 #include <string>
 
@@ -127,32 +128,41 @@ int run_pass(Ctx& x, std::vector<JobIn>& jobs, bool graph, BakeRun* run, const A
   run->valid = 0;
   for (const JobIn& j : jobs) run->valid += j.valid;
 
-  // traversal arrays: one job uses its own BVH; several jobs get their trees
-  // concatenated (child and triangle bases shifted), each ray starting at its
-  // job's root
-  std::vector<uint32_t> roots(nj, kNoRoot);
+  // traversal arrays: the distinct scenes of the jobs (a context may carry
+  // several jobs, e.g. the tiles of one surface). One scene uses its own
+  // BVH; several get their trees concatenated (child and triangle bases
+  // shifted), each ray starting at its job's root.
+  std::vector<Ctx*> scenes_u;
+  std::vector<uint32_t> scene_of(nj);
+  for (uint32_t k = 0; k < nj; ++k) {
+    auto it = std::find(scenes_u.begin(), scenes_u.end(), jobs[k].c);
+    scene_of[k] = (uint32_t)(it - scenes_u.begin());
+    if (it == scenes_u.end()) scenes_u.push_back(jobs[k].c);
+  }
+  const uint32_t nu = (uint32_t)scenes_u.size();
+  std::vector<uint32_t> roots(nj, kNoRoot), uroot(nu, kNoRoot);
   const CWNode* tnodes = nullptr;
   const float4* ttris = nullptr;
   size_t total_nodes = 0, total_tris = 0;
   int depth = 1;
-  for (const JobIn& j : jobs) {
-    total_nodes += j.c->bvh.nodes.size();
-    total_tris += j.c->bvh.tris.size() / 3;
-    depth = std::max(depth, j.c->bvh.max_depth);
+  for (const Ctx* c : scenes_u) {
+    total_nodes += c->bvh.nodes.size();
+    total_tris += c->bvh.tris.size() / 3;
+    depth = std::max(depth, c->bvh.max_depth);
   }
-  if (nj == 1) {
-    tnodes = jobs[0].c->d_nodes.as<CWNode>();
-    ttris = jobs[0].c->d_tris.as<float4>();
-    if (!jobs[0].c->bvh.nodes.empty()) roots[0] = 0;
+  if (nu == 1) {
+    tnodes = scenes_u[0]->d_nodes.as<CWNode>();
+    ttris = scenes_u[0]->d_tris.as<float4>();
+    if (!scenes_u[0]->bvh.nodes.empty()) uroot[0] = 0;
   } else {
     SHB_TRY(x, x.d_cnodes.reserve(std::max<size_t>(total_nodes * sizeof(CWNode), 16)));
     SHB_TRY(x, x.d_ctris.reserve(std::max<size_t>(total_tris * 3 * sizeof(float4), 16)));
     size_t no = 0, to = 0;
-    for (uint32_t k = 0; k < nj; ++k) {
-      const Ctx& c = *jobs[k].c;
+    for (uint32_t u = 0; u < nu; ++u) {
+      const Ctx& c = *scenes_u[u];
       const size_t nn = c.bvh.nodes.size(), nt = c.bvh.tris.size() / 3;
       if (nn) {
-        roots[k] = (uint32_t)no;
+        uroot[u] = (uint32_t)no;
         SHB_TRY(x, launch_relocate_nodes(c.d_nodes.as<CWNode>(), x.d_cnodes.as<CWNode>() + no,
                                          (uint32_t)nn, (uint32_t)no, (uint32_t)to, st));
         SHB_TRY(x, hipMemcpyAsync(x.d_ctris.as<float4>() + 3 * to, c.d_tris.ptr,
@@ -164,6 +174,7 @@ int run_pass(Ctx& x, std::vector<JobIn>& jobs, bool graph, BakeRun* run, const A
     tnodes = x.d_cnodes.as<CWNode>();
     ttris = x.d_ctris.as<float4>();
   }
+  for (uint32_t k = 0; k < nj; ++k) roots[k] = uroot[scene_of[k]];
 
   // job table
   std::vector<JobDev> jt(nj);
@@ -173,7 +184,7 @@ int run_pass(Ctx& x, std::vector<JobIn>& jobs, bool graph, BakeRun* run, const A
     std::memset(&J, 0, sizeof(J));
     J.scene = c.view();
     J.surface = c.d_surface.as<float4>();
-    J.texel_list = c.d_list.as<uint32_t>();
+    J.texel_list = c.d_list.as<uint32_t>() + jobs[k].list_offset;
     J.sh_out = c.d_sh.as<float>();
     J.vis_out = c.d_vis.as<float>();
     J.seed = jobs[k].cfg.seed;
@@ -343,9 +354,12 @@ int run_pass(Ctx& x, std::vector<JobIn>& jobs, bool graph, BakeRun* run, const A
       else SHB_TRY(x, launch_generate(b, sgrid, st));
       if (!r.dry) SHB_TRY(x, stage_done(st, "generate", bi, -1));
       for (uint32_t d = 0; d <= maxB[bi]; ++d) {
+        // this bounce's extension queue (depth 0 traces the primary rays in
+        // slot order)
+        const uint32_t* ext_q = d == 0 ? nullptr : qext[d & 1];
         TraceArgs ta = tb;
         ta.rays = b.ray;
-        ta.queue = d == 0 ? nullptr : qext[d & 1];
+        ta.queue = ext_q;
         ta.count_ptr = d == 0 ? nullptr : cnt + kCntExt + d;
         ta.count_const = P;
         ta.totals = count ? totals : nullptr;
@@ -364,8 +378,8 @@ int run_pass(Ctx& x, std::vector<JobIn>& jobs, bool graph, BakeRun* run, const A
           r.add(4, b);
           r.add(4, d);
         } else {
-          SHB_TRY(x, launch_shade(b, (int)d, d == 0 ? nullptr : qext[d & 1], cnt + kCntExt + d,
-                                  qext[(d + 1) & 1], cnt + kCntExt + d + 1, sgrid, st));
+          SHB_TRY(x, launch_shade(b, (int)d, ext_q, cnt + kCntExt + d, qext[(d + 1) & 1],
+                                  cnt + kCntExt + d + 1, sgrid, st));
           SHB_TRY(x, stage_done(st, "shade", bi, (int)d));
         }
         TraceArgs sa = tb;

@@ -16,6 +16,9 @@ struct JobIn {
   shb_bake_cfg cfg{};
   uint32_t stream = 0;  // random-stream id (0 for the plain entry points)
   uint32_t valid = 0;   // valid texels (set by prepare_job)
+  // the job's texels: entries [list_offset, list_offset + valid) of its
+  // context's texel list (a tile of a tiled bake; 0 for a whole surface)
+  uint32_t list_offset = 0;
 };
 
 // Validates job j's context and config, zeroes its outputs and builds its
@@ -28,10 +31,10 @@ int prepare_job_async(JobIn& j, hipStream_t st, uint32_t* count_dev);
 int prepare_job(JobIn& j);
 
 // Runs the bake of every job on x's stream: one set of wavefront batches
-// whose path slots may mix jobs (several jobs traverse their concatenated
-// BVHs). With `graph`, the launch sequence is captured once in a hipGraph
-// and replayed while the plan and buffers are unchanged. Errors are
-// reported on x.
+// whose path slots may mix jobs (jobs of several scenes traverse their
+// concatenated BVHs; jobs that share a context share its BVH). With
+// `graph`, the launch sequence is captured once in a hipGraph and replayed
+// while the plan and buffers are unchanged. Errors are reported on x.
 // Path-replay adjoint inputs: dL/dSH on the device (27 floats per
 // supersampled texel) and the zeroed FP64 gradient sums (3 per light, then 3
 // per material).

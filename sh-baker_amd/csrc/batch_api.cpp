@@ -73,6 +73,79 @@ int shb_bake_batched(shb_ctx* const* ctxs, int32_t num_jobs, const shb_bake_cfg*
   return SHB_OK;
 }
 
+int shb_bake_tiled(shb_ctx* h, const shb_bake_cfg* cfg, const shb_tile* tiles,
+                   int32_t num_tiles, float* sh_out, float* vis_out, shb_bake_stats* stats) {
+  if (!h) return SHB_ERR_INVALID_ARG;
+  Ctx& c = h->c;
+  c.err.clear();
+  if (!cfg) return fail(c, SHB_ERR_INVALID_ARG, "cfg is NULL");
+  if (!tiles || num_tiles <= 0) return fail(c, SHB_ERR_INVALID_ARG, "no tiles");
+  if (num_tiles > kMaxJobs)
+    return fail(c, SHB_ERR_LIMIT, "at most " + std::to_string(kMaxJobs) + " tiles per bake");
+  if (!c.has_surface)
+    return fail(c, SHB_ERR_NO_SURFACE, "no surface: call shb_rasterize or shb_set_surface");
+  const int64_t W = (int64_t)c.width * c.ss, H = (int64_t)c.height * c.ss;
+  for (int32_t i = 0; i < num_tiles; ++i) {
+    const shb_tile& t = tiles[i];
+    if (t.w <= 0 || t.h <= 0 || t.x0 < 0 || t.y0 < 0 || (int64_t)t.x0 + t.w > W ||
+        (int64_t)t.y0 + t.h > H)
+      return fail(c, SHB_ERR_INVALID_ARG,
+                  "tile " + std::to_string(i) + " is empty or outside the " + std::to_string(W) +
+                      " x " + std::to_string(H) + " supersampled surface");
+    for (int32_t k = 0; k < i; ++k) {
+      const shb_tile& u = tiles[k];
+      if (t.x0 < u.x0 + u.w && u.x0 < t.x0 + t.w && t.y0 < u.y0 + u.h && u.y0 < t.y0 + t.h)
+        return fail(c, SHB_ERR_INVALID_ARG,
+                    "tiles " + std::to_string(k) + " and " + std::to_string(i) + " overlap");
+    }
+  }
+  const double t0 = now_s();
+  // the whole surface's valid texels (ascending), then each tile's share of
+  // them, row by row: a tile row's texels are one range of the ascending list
+  std::vector<JobIn> jobs((size_t)num_tiles);
+  jobs[0].c = &c;
+  jobs[0].cfg = *cfg;
+  int rc = prepare_job(jobs[0]);
+  if (rc) return rc;
+  const uint32_t V = jobs[0].valid;
+  std::vector<uint32_t> list(V), tiled;
+  tiled.reserve(V);
+  if (V)
+    SHB_TRY(c, hipMemcpy(list.data(), c.d_list.ptr, (size_t)V * 4, hipMemcpyDeviceToHost));
+  for (int32_t i = 0; i < num_tiles; ++i) {
+    const shb_tile& t = tiles[i];
+    JobIn& j = jobs[i];
+    j.c = &c;
+    j.cfg = *cfg;
+    j.stream = 0;  // the plain bake's stream: identical results
+    j.list_offset = (uint32_t)tiled.size();
+    for (int64_t y = t.y0; y < (int64_t)t.y0 + t.h; ++y) {
+      const uint32_t lo = (uint32_t)(y * W + t.x0), hi = (uint32_t)(y * W + t.x0 + t.w);
+      auto a = std::lower_bound(list.begin(), list.end(), lo);
+      auto b = std::lower_bound(a, list.end(), hi);
+      tiled.insert(tiled.end(), a, b);
+    }
+    j.valid = (uint32_t)tiled.size() - j.list_offset;
+  }
+  if (!tiled.empty())
+    SHB_TRY(c, hipMemcpy(c.d_list.ptr, tiled.data(), tiled.size() * 4, hipMemcpyHostToDevice));
+  BakeRun run;
+  rc = run_bake(c, jobs, true, &run);
+  if (rc) return rc;
+  const size_t npx = (size_t)W * H;
+  if (sh_out)
+    SHB_TRY(c, hipMemcpyAsync(sh_out, c.d_sh.ptr, npx * 27 * 4, hipMemcpyDeviceToHost, c.stream));
+  if (vis_out)
+    SHB_TRY(c, hipMemcpyAsync(vis_out, c.d_vis.ptr, npx * 4, hipMemcpyDeviceToHost, c.stream));
+  SHB_TRY(c, hipStreamSynchronize(c.stream));
+  if (stats) {
+    fill_stats(stats, run, 0, t0);
+    stats->valid_texels = tiled.size();
+    stats->paths = (uint64_t)tiled.size() * (uint64_t)cfg->samples;
+  }
+  return SHB_OK;
+}
+
 int shb_autotune(shb_ctx* h, const shb_bake_cfg* cfg, shb_tuning* best) {
   if (!h) return SHB_ERR_INVALID_ARG;
   Ctx& c = h->c;

@@ -25,6 +25,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <deque>
@@ -75,7 +76,11 @@ struct Builder {
   struct Item {
     int node;  // index in the node array it is built into
     uint32_t first, count;
+    int depth;  // binary depth (root 0)
   };
+  // SAH splits above this binary depth; deeper ranges split at their object
+  // median, which bounds the tree's depth (kept deterministic: stable sort)
+  int sah_depth = INT_MAX;
 
   // Bounds of item `it`, stored in nodes[it.node]; unless it becomes a leaf,
   // partitions its range of `order` by the binned-SAH split and appends its
@@ -233,6 +238,25 @@ struct Builder {
     nodes_out[it.node].first = it.first;
     nodes_out[it.node].count = it.count;
     if (it.count <= 1) return;
+    if (it.depth >= sah_depth) {
+      // depth budget spent: halve the range along its widest centroid axis
+      if (it.count <= 3) return;  // a leaf
+      int ax = 0;
+      for (int a = 1; a < 3; ++a)
+        if (cbox.hi[a] - cbox.lo[a] > cbox.hi[ax] - cbox.lo[ax]) ax = a;
+      std::stable_sort(refs.begin() + it.first, refs.begin() + it.first + it.count,
+                       [ax](const Ref& p, const Ref& q) { return p.c(ax) < q.c(ax); });
+      const uint32_t mid = it.first + it.count / 2;
+      int l = (int)nodes_out.size();
+      nodes_out.push_back(B2Node());
+      int r = (int)nodes_out.size();
+      nodes_out.push_back(B2Node());
+      nodes_out[it.node].left = l;
+      nodes_out[it.node].right = r;
+      work.push_back({r, mid, it.first + it.count - mid, it.depth + 1});
+      work.push_back({l, it.first, mid - it.first, it.depth + 1});
+      return;
+    }
     int best_axis = -1, best_bin = -1;
     float best_cost = INFINITY;
     for (int a = 0; a < 3; ++a) {
@@ -286,8 +310,8 @@ struct Builder {
     nodes_out.push_back(B2Node());
     nodes_out[it.node].left = l;
     nodes_out[it.node].right = r;
-    work.push_back({r, mid, it.first + it.count - mid});
-    work.push_back({l, it.first, mid - it.first});
+    work.push_back({r, mid, it.first + it.count - mid, it.depth + 1});
+    work.push_back({l, it.first, mid - it.first, it.depth + 1});
   }
 
   // Top-down build. The top levels are split here until the open ranges
@@ -323,7 +347,7 @@ struct Builder {
     // by all threads
     const uint32_t hand_off = std::max<uint32_t>(1024u, n / (4u * T));
     const uint32_t kParallelBins = 1u << 16;
-    std::vector<Item> work{{0, 0, n}}, open;
+    std::vector<Item> work{{0, 0, n, 0}}, open;
     while (!work.empty()) {
       Item it = work.back();
       work.pop_back();
@@ -348,7 +372,7 @@ struct Builder {
         std::vector<B2Node>& ln = local[j];
         ln.reserve(2 * (size_t)open[j].count);
         ln.push_back(B2Node());
-        std::vector<Item> w{{0, open[j].first, open[j].count}};
+        std::vector<Item> w{{0, open[j].first, open[j].count, open[j].depth}};
         while (!w.empty()) {
           Item it = w.back();
           w.pop_back();
@@ -404,20 +428,8 @@ int axis_exponent(float lo, float hi) {
   return std::max(-126, std::min(127, e));
 }
 
-}  // namespace
-
-void build_cwbvh(const float* tri_xyz, uint32_t n, int width, BvhBuildOut* out) {
-  auto t0 = std::chrono::steady_clock::now();
-  width = std::max(2, std::min(8, width));
-  out->nodes.clear();
-  out->tris.clear();
-  out->max_depth = 0;
-  if (n == 0) {
-    out->seconds = 0;
-    return;
-  }
-  Builder b;
-  b.build(tri_xyz, n);
+// The wide tree of binary tree b (see the file comment, steps 2-4).
+void collapse(const Builder& b, const float* tri_xyz, uint32_t n, int width, BvhBuildOut* out) {
 
   // ---- cost-optimal collapse (see the file comment)
   // cost[n * W + i - 1]: subtree n in at most i slots; how[...]: 0 leaf,
@@ -635,6 +647,29 @@ void build_cwbvh(const float* tri_xyz, uint32_t n, int width, BvhBuildOut* out) 
       q.push_back({kids[k], at, pd.depth + 1});
     }
     out->nodes[pd.at] = node;
+  }
+}
+
+}  // namespace
+
+void build_cwbvh(const float* tri_xyz, uint32_t n, int width, BvhBuildOut* out) {
+  auto t0 = std::chrono::steady_clock::now();
+  width = std::max(2, std::min(8, width));
+  // SAH everywhere first; when the wide tree comes out deeper than the
+  // traversal stack allows, again with the SAH limited to the top levels
+  // (object-median splits below), down to a median tree
+  static const int kBudgets[] = {INT_MAX, 32, 24, 16, 12, 8, 4, 0};
+  for (int k = 0;; ++k) {
+    out->nodes.clear();
+    out->tris.clear();
+    out->max_depth = 0;
+    out->sah_depth = kBudgets[k];
+    if (n == 0) break;
+    Builder b;
+    b.sah_depth = kBudgets[k];
+    b.build(tri_xyz, n);
+    collapse(b, tri_xyz, n, width, out);
+    if (out->max_depth <= kMaxBvhDepth || k + 1 == (int)(sizeof(kBudgets) / sizeof(int))) break;
   }
   auto t1 = std::chrono::steady_clock::now();
   out->seconds = std::chrono::duration<double>(t1 - t0).count();

@@ -28,10 +28,11 @@ int build_scene(Ctx& c, const shb_scene_desc* d) {
 int upload_bvh(Ctx& c) {
   build_cwbvh(c.hs.world_xyz.data(), c.hs.num_tris, c.tuning.bvh_width, &c.bvh);
   // the group stack holds at most one entry per level below the root
-  if (c.bvh.max_depth - 1 > kStackLds)
+  if (c.bvh.max_depth > kMaxBvhDepth)
     return fail(c, SHB_ERR_LIMIT,
                 "BVH depth " + std::to_string(c.bvh.max_depth) + " exceeds the traversal stack (" +
-                    std::to_string(kStackLds + 1) + " levels)");
+                    std::to_string(kMaxBvhDepth) + " levels) even with median splits: use a "
+                    "wider BVH (tuning.bvh_width)");
   SHB_TRY(c, upload(c.d_nodes, c.bvh.nodes, c.stream));
   SHB_TRY(c, upload(c.d_tris, c.bvh.tris, c.stream));
   return SHB_OK;

@@ -104,6 +104,10 @@ class ShbBakeStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class ShbTile(C.Structure):
+    _fields_ = [("x0", C.c_int32), ("y0", C.c_int32), ("w", C.c_int32), ("h", C.c_int32)]
+
+
 class ShbTuning(C.Structure):
     _fields_ = [("block_size", C.c_int32), ("batch_paths", C.c_int32),
                 ("queue_chunk", C.c_int32), ("bvh_width", C.c_int32)]
@@ -124,7 +128,7 @@ EXPORTED = [
     "shb_gltf_albedo_path", "shb_gltf_free", "shb_atlas_geometry_scales", "shb_atlas_create",
     "shb_atlas_scene", "shb_atlas_info", "shb_atlas_free",
     "shb_downsample", "shb_downsample_mask", "shb_dilate", "shb_trace_rays", "shb_bvh_export",
-    "shb_bake_batched", "shb_autotune", "shb_set_tuning", "shb_get_tuning", "shb_set_params",
+    "shb_bake_batched", "shb_bake_tiled", "shb_autotune", "shb_set_tuning", "shb_get_tuning", "shb_set_params",
     "shb_bake_adjoint",
 ]
 
@@ -184,6 +188,8 @@ def _load():
     lib.shb_bake_batched.argtypes = [C.POINTER(vp), C.c_int32, C.POINTER(ShbBakeCfg),
                                      C.POINTER(_fp), C.POINTER(_fp), C.POINTER(ShbBakeStats)]
     lib.shb_autotune.argtypes = [vp, C.POINTER(ShbBakeCfg), C.POINTER(ShbTuning)]
+    lib.shb_bake_tiled.argtypes = [vp, C.POINTER(ShbBakeCfg), C.POINTER(ShbTile), C.c_int32,
+                                   _fp, _fp, C.POINTER(ShbBakeStats)]
     lib.shb_set_tuning.argtypes = [vp, C.POINTER(ShbTuning)]
     lib.shb_get_tuning.argtypes = [vp, C.POINTER(ShbTuning)]
     lib.shb_set_params.argtypes = [vp, _fp, _fp]
@@ -292,6 +298,20 @@ class Baker:
         st = ShbBakeStats()
         cfg = self.cfg(samples, bounces, seed, flags, num_light_samples)
         self._chk(lib.shb_bake(self.h, C.byref(cfg), _f(sh), _f(vis), C.byref(st)))
+        return sh, vis, st.as_dict()
+
+    def bake_tiled(self, samples, bounces, tiles, seed=12345, flags=0, want=True):
+        """shb_bake_tiled: the surface baked as tile jobs (x0, y0, w, h in
+        supersampled texels) of one batched run; returns (sh, vis, stats)
+        like bake (texels outside every tile are 0)."""
+        n = self.width * self.ss * self.height * self.ss
+        sh = np.zeros((n, 27), np.float32) if want else None
+        vis = np.zeros(n, np.float32) if want else None
+        arr = (ShbTile * max(1, len(tiles)))(*[ShbTile(*map(int, t)) for t in tiles])
+        st = ShbBakeStats()
+        cfg = self.cfg(samples, bounces, seed, flags)
+        self._chk(lib.shb_bake_tiled(self.h, C.byref(cfg), arr, len(tiles), _f(sh), _f(vis),
+                                     C.byref(st)))
         return sh, vis, st.as_dict()
 
     def bake_lightmap(self, samples, bounces, dilation, seed=12345, flags=0, to_host=True):

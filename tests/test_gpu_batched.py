@@ -107,3 +107,50 @@ def test_autotune_keeps_results(jobs):
         assert np.array_equal(bits(b.bake(spp, B, seed=seed)[0]), bits(ref))
     finally:
         b.set_tuning(**old)
+
+
+def test_tiled_bake_bit_identical_to_plain():
+    """shb_bake_tiled: tiles of one surface as jobs of one batched run over
+    the context's single BVH; every texel in a tile gets the plain bake's
+    bits, texels outside every tile stay 0."""
+    b = shb.Baker(0)
+    try:
+        b.upload_scene(scenes.scaled_synthetic_scene(6000, 48, 21))
+        pts = b.rasterize(48, 48, 2)
+        ref, rvis, rst = b.bake(8, 3, seed=77)
+        grid = [(x, y, 24, 24) for y in range(0, 96, 24) for x in range(0, 96, 24)]
+        sh, vis, st = b.bake_tiled(8, 3, grid, seed=77)
+        assert np.array_equal(bits(sh), bits(ref)) and np.array_equal(bits(vis), bits(rvis))
+        assert st["rays_extension"] == rst["rays_extension"]
+        assert st["valid_texels"] == rst["valid_texels"]
+        # uneven tiles covering part of the surface
+        part = [(0, 0, 96, 10), (5, 30, 17, 41), (60, 20, 36, 76)]
+        sh, vis, st = b.bake_tiled(8, 3, part, seed=77)
+        inside = np.zeros((96, 96), bool)
+        for x, y, w, h in part:
+            inside[y:y + h, x:x + w] = True
+        inside = inside.ravel()
+        assert np.array_equal(bits(sh[inside]), bits(ref[inside]))
+        assert (sh[~inside] == 0).all() and (vis[~inside] == 0).all()
+        assert st["valid_texels"] == int(((pts["material_id"] >= 0) & inside).sum())
+        # one tile: the whole surface
+        sh, _, _ = b.bake_tiled(8, 3, [(0, 0, 96, 96)], seed=77)
+        assert np.array_equal(bits(sh), bits(ref))
+    finally:
+        b.close()
+
+
+def test_tiled_bake_rejects_bad_tiles():
+    b = shb.Baker(0)
+    try:
+        b.upload_scene(scenes.box_room_scene(32, 32))
+        b.rasterize(32, 32, 1, want=False)
+        for tiles in ([(0, 0, 16, 16), (8, 8, 16, 16)],   # overlap
+                      [(0, 0, 33, 4)],                     # outside
+                      [(0, 0, 0, 4)],                      # empty
+                      [(x, 0, 1, 1) for x in range(32)] * 3):  # more than 64
+            with pytest.raises(shb.ShbError) as e:
+                b.bake_tiled(4, 1, tiles)
+            assert e.value.code in (-1, -5)
+    finally:
+        b.close()

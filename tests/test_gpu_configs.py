@@ -14,7 +14,8 @@ and the post-bake steps of src/main.cpp:138-163):
       gradients against central finite differences under common random
       numbers;
   C5  4.2M triangles, 256 lights, 4096^2 x ss2 (8192^2 texels), 5 bounces
-      at 2 spp: same stream as the oracle on every 64th texel row.
+      at 2 spp: same stream as the oracle on every 64th texel row, and the
+      bake as 64 tile jobs bit-identical to the plain bake.
 The full sample counts run in bench.py."""
 import numpy as np
 import pytest
@@ -155,6 +156,14 @@ def test_c5_full_geometry_and_size():
         g_sh, g_vis, st = b.bake(2, 5, seed=12345)
         print(f"  GPU: {st['valid_texels']} texels, {st['rays_extension'] + st['rays_shadow']} "
               f"rays, {st['seconds_total']:.2f}s", flush=True)
+        # the configuration's form: 64 tile jobs (8 x 8 tiles of 1024^2) in one
+        # batched run, bit-identical to the plain bake
+        tiles = [(x, y, 1024, 1024) for y in range(0, 8192, 1024) for x in range(0, 8192, 1024)]
+        t_sh, t_vis, tst = b.bake_tiled(2, 5, tiles, seed=12345)
+        print(f"  64 tiles: {tst['seconds_total']:.2f}s", flush=True)
+        assert np.array_equal(t_sh.view(np.uint32), g_sh.view(np.uint32))
+        assert np.array_equal(t_vis.view(np.uint32), g_vis.view(np.uint32))
+        del t_sh, t_vis
     finally:
         b.close()
     o_sh, o_vis, _, _ = orc.Oracle(s).bake(pts, 8192, 8192, 2, 5, mode=orc.MODE_GPU,

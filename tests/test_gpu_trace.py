@@ -211,3 +211,39 @@ def test_persistent_traversal_matches(baker, chunk, block):
         assert (nv >= 1).all()
     finally:
         baker.set_tuning(**old)
+
+
+@pytest.mark.parametrize("width", [8, 2])
+def test_skewed_scene_uploads_and_traces(baker, width):
+    """A chain of triangles whose sizes and positions halve (120 scales): the
+    SAH alone builds a tree deeper than the traversal stack; the builder's
+    depth budget brings it within, and the hits stay bit-exact."""
+    n = 6000
+    i = np.arange(n)
+    sc = np.ldexp(1.0, -(i % 120)).astype(np.float32)
+    x = (sc + np.float32(1e-7) * (i // 120)).astype(np.float32)
+    tris = np.zeros((n, 3, 3), np.float32)
+    tris[:, 0, 0] = x
+    tris[:, 1, 0] = x + sc
+    tris[:, 2, 0] = x
+    tris[:, 2, 1] = sc
+    tris[:, :, 2] = (i % 7)[:, None] * np.float32(1e-3)
+    s = scenes.triangles_scene([t for t in tris])
+    old = baker.tuning()
+    try:
+        baker.set_tuning(bvh_width=width)
+        baker.upload_scene(s)
+        rng = np.random.default_rng(5)
+        r = np.zeros(20000, RAY_DTYPE)
+        r["origin"] = np.stack([rng.uniform(-0.1, 2.2, 20000), rng.uniform(-0.1, 1.1, 20000),
+                                np.full(20000, -1.0)], 1)
+        d = np.stack([rng.normal(0, 0.05, 20000), rng.normal(0, 0.05, 20000),
+                      np.ones(20000)], 1)
+        r["direction"] = d
+        r["tfar"] = np.inf
+        hg = baker.trace_rays(r)
+        ho = orc.Oracle(s).trace(r, mode=0)
+        compare_exact(hg, ho)
+        assert (ho["prim_id"] >= 0).mean() > 0.05
+    finally:
+        baker.set_tuning(**old)
