@@ -149,7 +149,8 @@ enum {
 typedef struct shb_bake_cfg {
   int32_t samples;           /* rays per texel                              */
   int32_t bounces;           /* max path depth                              */
-  int32_t num_light_samples; /* must be 1 (the reference CLI always uses 1) */
+  int32_t num_light_samples; /* light samples per NEE, 1..16 (the reference
+                                CLI always uses 1; the adjoint takes 1)      */
   uint32_t flags;            /* SHB_BAKE_*                                  */
   uint64_t seed;
 } shb_bake_cfg;

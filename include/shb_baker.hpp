@@ -171,7 +171,7 @@ struct SHTexture {
 struct BakeConfig {
   int samples = 128;          // Rays per texel
   int bounces = 3;            // Max path depth
-  int num_light_samples = 1;  // Number of light samples for NEE (must be 1)
+  int num_light_samples = 1;  // Number of light samples for NEE (1..16)
   uint64_t seed = 12345;      // counter-based stream key (no reference field)
 };
 

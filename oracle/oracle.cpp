@@ -29,7 +29,11 @@
 // (dilation_test.cpp:22-32,50-57); downsample KAT (baker_test.cpp:80-87); light
 // closed forms (light_test.cpp:108,196-200,226,259,274,300-305); material KATs
 // (material_test.cpp:51-53,96-98,109,120); Philox and xoshiro known answers;
-// the BakeSimpleQuad closed form (baker_test.cpp:10-65) in both modes.
+// the BakeSimpleQuad closed form (baker_test.cpp:10-65) in both modes; the
+// bounce >= 1 estimator (Trace, vertex and texel NEE for point, spot and
+// area lights, one or several light samples) against FP64 quadratures of
+// plate scenes (tests/plate_closed_forms.py), each shown to fail with the
+// mistake it targets planted through orc_set_fault.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -884,11 +888,89 @@ static Xoshiro128ss site_rng(uint64_t seed, uint32_t texel, uint32_t sample,
   return x;
 }
 
-// One NEE sample with the reservoir pick over lights: for each light in
-// order, draw (area: triangle, u1, u2) then u; W += w; pick light if u*W < w.
-// The pick is light i with probability w_i / W, as discrete_distribution's.
+// Unshadowed radiance, shadow ray and area pdf of light l at P in the GPU
+// stream: an area light first draws its triangle (when it has several) and
+// its two barycentric uniforms.
+static V3 GpuLightEval(const Scene& s, const Light& l, V3 P, V3 N, V3 wo, const Mat* mat,
+                       V2 uv, Xoshiro128ss& x, bool texel, ShadowRay* ray, float* ap) {
+  *ap = 1.0f;
+  if (l.type != SHB_LIGHT_AREA)
+    return LightRadiance(s, l, nullptr, P, N, !texel, mat, uv, wo, ray, true);
+  AreaSample as{};
+  if (area_valid(s, l)) {
+    uint32_t nt = (uint32_t)(s.geos[l.geom].idx.size() / 3);
+    uint32_t tri = nt > 1 ? x.index(nt) : 0u;  // one triangle: no draw
+    float u1 = x.uniform();
+    float u2 = x.uniform();
+    as = SampleAreaLightAt(s, l, tri, u1, u2, true);
+  }
+  V3 rd = LightRadiance(s, l, &as, P, N, !texel, mat, uv, wo, ray, true);
+  *ap = as.pdf;
+  return rd;
+}
+
+// NEE in the GPU stream (light.cpp:82-189 at a vertex, 191-293 at a texel).
+// One sample: a reservoir pick over lights: for each light in order, its
+// area draws then u; W += w; pick light if u*W < w -- light i with
+// probability w_i / W, as discrete_distribution's.
+// n > 1 samples: the reference's n picks with replacement from one set of
+// per-light samples, as an inverse CDF: a first pass sums W, then n
+// uniforms give targets u_j W, and a second pass (the same area draws,
+// replayed from a copy of the state) assigns target j to the light whose
+// interval of the running sum holds it. Picks are traced and summed in the
+// order j; the vertex result is their sum / n, a texel adds L / (joint n)
+// along each visible pick's direction.
 static V3 NeeGpu(const Scene& s, V3 P, V3 N, V3 wo, const Mat* mat, V2 uv,
-                 Xoshiro128ss& x, bool texel, float* sh, Counters& c) {
+                 Xoshiro128ss& x, bool texel, float* sh, Counters& c, unsigned n = 1) {
+  if (n > 1) {
+    Xoshiro128ss x0 = x;
+    float W = 0.f;
+    for (const Light& l : s.lights) {
+      ShadowRay ray;
+      float ap;
+      W += maxc(GpuLightEval(s, l, P, N, wo, mat, uv, x, texel, &ray, &ap));
+    }
+    std::vector<float> target(n);
+    for (unsigned j = 0; j < n; ++j) target[j] = x.uniform() * W;
+    if (W < 1e-6f) return V3{};
+    struct Pick {
+      bool found = false, ok = false;
+      V3 rad;
+      ShadowRay ray;
+      float joint = 0.f;
+    };
+    std::vector<Pick> pk(n);
+    float cum = 0.f;
+    for (const Light& l : s.lights) {
+      ShadowRay ray;
+      float ap;
+      V3 rd = GpuLightEval(s, l, P, N, wo, mat, uv, x0, texel, &ray, &ap);
+      const float w = maxc(rd);
+      const float next = cum + w;
+      for (unsigned j = 0; j < n; ++j) {
+        if (pk[j].found || !(w > 0.f) || !(target[j] < next)) continue;
+        pk[j].found = true;
+        const float joint = (w / W) * ap;
+        if (joint < 1e-3f) continue;
+        pk[j].ok = true;
+        pk[j].rad = rd;
+        pk[j].ray = ray;
+        pk[j].joint = joint;
+      }
+      cum = next;
+    }
+    V3 sum{};
+    for (unsigned j = 0; j < n; ++j) {
+      if (!pk[j].ok) continue;
+      const ShadowRay& r = pk[j].ray;
+      if (FindOcclusion(s, r.o, r.d, r.tnear, r.tfar, nullptr, &c.shadow)) continue;
+      if (texel)
+        AccumulateRadiance(pk[j].rad / (pk[j].joint * (float)n), r.d, sh);
+      else
+        sum += pk[j].rad / pk[j].joint;
+    }
+    return texel ? V3{} : sum / (float)n;
+  }
   float W = 0.f;
   int pick = -1;
   float pw = 0.f, pap = 1.f;
@@ -896,25 +978,8 @@ static V3 NeeGpu(const Scene& s, V3 P, V3 N, V3 wo, const Mat* mat, V2 uv,
   ShadowRay pray;
   for (const Light& l : s.lights) {
     ShadowRay ray;
-    float ap = 1.0f;
-    V3 rd;
-    if (l.type == SHB_LIGHT_AREA) {
-      if (area_valid(s, l)) {
-        uint32_t nt = (uint32_t)(s.geos[l.geom].idx.size() / 3);
-        uint32_t tri = nt > 1 ? x.index(nt) : 0u;  // one triangle: no draw
-        float u1 = x.uniform();
-        float u2 = x.uniform();
-        AreaSample as = SampleAreaLightAt(s, l, tri, u1, u2, true);
-        rd = LightRadiance(s, l, &as, P, N, !texel, mat, uv, wo, &ray, true);
-        ap = as.pdf;
-      } else {
-        AreaSample as{};
-        rd = LightRadiance(s, l, &as, P, N, !texel, mat, uv, wo, &ray, true);
-        ap = as.pdf;
-      }
-    } else {
-      rd = LightRadiance(s, l, nullptr, P, N, !texel, mat, uv, wo, &ray, true);
-    }
+    float ap;
+    V3 rd = GpuLightEval(s, l, P, N, wo, mat, uv, x, texel, &ray, &ap);
     float w = maxc(rd);
     float u = x.uniform();
     W += w;
@@ -947,6 +1012,7 @@ struct GpuCtx {
   uint32_t texel, sample, job;
   float* vis;
   Counters* c;
+  unsigned nls = 1;  // light samples per NEE
 };
 
 // Trace with the GPU stream: the control flow of TraceRef (baker.cpp:70-149)
@@ -979,7 +1045,7 @@ static V3 TraceGpu(const GpuCtx& cx, V3 origin, V3 dir, int depth,
   Xoshiro128ss x = site_rng(cx.seed, cx.texel, cx.sample, (uint32_t)depth,
                             branch, SITE_VERTEX, cx.job);
   V3 Ld = NeeGpu(*cx.s, hit_pos, occ.normal, -dir, &mat, occ.uv, x, false,
-                 nullptr, *cx.c);
+                 nullptr, *cx.c, cx.nls);
   color += alpha * Ld;
   float r1 = x.uniform(), r2 = x.uniform();
   RefSample smp = SampleMaterialUV(occ.normal, r1, r2);
@@ -1686,8 +1752,9 @@ int orc_bake(void* sp, const shb_surface_point* pts, int width, int height,
             TraceCtx cx{&s, bounces, num_light_samples, &vis, &c};
             Li = TraceRef(cx, origin, dw, 0, rr) * inv_pdf_uniform;
           } else {
-            NeeGpu(s, P, N, V3{}, nullptr, V2{}, x, true, nee1.data(), c);
-            GpuCtx cx{&s, bounces, seed, (uint32_t)idx, (uint32_t)smp, job, &vis, &c};
+            NeeGpu(s, P, N, V3{}, nullptr, V2{}, x, true, nee1.data(), c, num_light_samples);
+            GpuCtx cx{&s, bounces, seed, (uint32_t)idx, (uint32_t)smp, job, &vis, &c,
+                      num_light_samples};
             Li = TraceGpu(cx, origin, dw, 0, 1) * inv_pdf_uniform;
           }
           AccumulateRadiance(Li, dw, ind1.data());

@@ -136,6 +136,110 @@ __device__ __forceinline__ float dot_fma(f3 a, f3 b) {
 // returned radiance carries the scales. UNIT: every scale is 1, so the
 // scaled and unscaled values are equal bit for bit (x * 1 == x, and brdf ==
 // brdf_w) and the scale products are skipped.
+// Light l at P: unshadowed radiance with the light's scale (rad) and
+// without (radw, the selection weight's radiance), the pending shadow ray
+// (direction, tfar) and the area pdf; an area light first draws its triangle
+// (when it has several) and two barycentric uniforms from rng. VERTEX:
+// BRDF x cosine applied (light.h *Radiance<Brdf>); else the texel form.
+template <bool VERTEX, bool UNIT>
+__device__ __forceinline__ void light_eval(const SceneView& sc, int l, f3 P, f3 N, f3 brdf,
+                                           f3 brdf_w, bool front, Rng& rng, f3& rad_out,
+                                           f3& radw_out, f3& sdir_out, float& tfar_out,
+                                           float& ap_out) {
+  // every lane visits the same light: a wave-uniform index lets the light
+  // record come through the scalar cache
+  const LightDev L = uniform_load(sc.lights, (uint32_t)l);
+  f3 rad = {0.f, 0.f, 0.f}, sdir;
+  float tfar, ap = 1.0f, cos_n = 0.f;
+  const int type = L.type;
+  if (type == kLightDirectional) {
+    f3 ld = {L.dir[0], L.dir[1], L.dir[2]};
+    sdir = -ld;
+    tfar = 1.0e10f;
+    cos_n = dot(N, -ld);
+    if (!(cos_n < 0.f)) rad = f3{L.ic[0], L.ic[1], L.ic[2]};
+  } else {
+    f3 target;
+    f3 emit = {0.f, 0.f, 0.f};
+    bool area = type == kLightArea;
+    if (area) {
+      if (L.num_tris == 0) {
+        ap = 0.f;
+        target = f3{0.f, 0.f, 0.f};
+      } else {
+        // single-triangle emitters draw no triangle index
+        uint32_t tri = L.num_tris > 1 ? rng.index(L.num_tris) : 0u;
+        float u1 = rng.uniform();
+        float u2 = rng.uniform();
+        if (u1 + u2 > 1.0f) {
+          u1 = 1.0f - u1;
+          u2 = 1.0f - u2;
+        }
+        // single-triangle emitters (tri is 0) read a wave-uniform record
+        float tu = 0.f, tv = 0.f;
+        if (L.num_tris == 1)
+          area_point(uniform_load(sc.area_tris, L.first_tri), u1, u2, target, ap, tu, tv);
+        else
+          area_point(sc.area_tris[L.first_tri + tri], u1, u2, target, ap, tu, tv);
+        if (L.emit_const)
+          emit = f3{L.emit[0], L.emit[1], L.emit[2]};
+        else
+          emit = get_emission(sc, L.material, tu, tv);
+      }
+    } else {
+      target = f3{L.pos[0], L.pos[1], L.pos[2]};
+    }
+    f3 Lv = target - P;
+    float dist_sq = sqnorm_fma(Lv);
+    float dist = sqrtf(dist_sq);
+    sdir = target - P;
+    tfar = dist - 0.001f;
+    // one division per light (the reference divides per component:
+    // light.h:82-93, 127-143, 176-188) and fused sums; the oracle's
+    // GPU-stream mode follows this arithmetic
+    const float inv = 1.0f / dist;
+    const float inv2 = inv * inv;
+    Lv = Lv * inv;
+    cos_n = dot_fma(N, Lv);
+    if (type == kLightSpot) {
+      f3 ld = {L.dir[0], L.dir[1], L.dir[2]};
+      float cos_l = dot(ld, -Lv);
+      float falloff = (cos_l - L.cos_outer) / (L.cos_inner - L.cos_outer);
+      falloff = fminf(fmaxf(falloff, 0.0f), 1.0f);
+      if (!(cos_n < 0.f)) rad = f3{L.ic[0], L.ic[1], L.ic[2]} * inv2 * falloff;
+    } else if (type == kLightPoint) {
+      if (!(cos_n < 0.f)) rad = f3{L.ic[0], L.ic[1], L.ic[2]} * inv2;
+    } else {
+      if (!(cos_n < 0.f)) rad = emit * inv2;
+    }
+  }
+  // BRDF * cosine at a path vertex: scaled albedo for the contribution,
+  // unscaled for the selection weight
+  f3 radw = rad;
+  if (VERTEX && !(cos_n < 0.f)) {
+    bool ok = front && cos_n > 0.0f;
+    f3 bv = ok ? brdf : f3{0.f, 0.f, 0.f};
+    if (UNIT) {
+      rad = cmul(rad, cos_n * bv);
+      radw = rad;
+    } else {
+      f3 bw = ok ? brdf_w : f3{0.f, 0.f, 0.f};
+      radw = cmul(rad, cos_n * bw);
+      rad = cmul(rad, cos_n * bv);
+    }
+  }
+  if (!UNIT) {
+    const uint32_t l3 = 3u * (uint32_t)l;
+    rad = cmul(rad, f3{uniform_load(sc.light_scale, l3), uniform_load(sc.light_scale, l3 + 1),
+                       uniform_load(sc.light_scale, l3 + 2)});
+  }
+  rad_out = rad;
+  radw_out = radw;
+  sdir_out = sdir;
+  tfar_out = tfar;
+  ap_out = ap;
+}
+
 template <bool VERTEX, bool UNIT>
 __device__ __forceinline__ bool nee_sample(const SceneView& sc, f3 P, f3 N, f3 brdf, f3 brdf_w,
                                            bool front, Rng& rng, NeeResult& out) {
@@ -145,93 +249,9 @@ __device__ __forceinline__ bool nee_sample(const SceneView& sc, f3 P, f3 N, f3 b
   f3 prad = {0.f, 0.f, 0.f}, pdir = {0.f, 0.f, 0.f};
   const int nl = sc.num_lights;
   for (int l = 0; l < nl; ++l) {
-    // every lane visits the same light: a wave-uniform index lets the light
-    // record come through the scalar cache
-    const LightDev L = uniform_load(sc.lights, (uint32_t)l);
-    f3 rad = {0.f, 0.f, 0.f}, sdir;
-    float tfar, ap = 1.0f, cos_n = 0.f;
-    const int type = L.type;
-    if (type == kLightDirectional) {
-      f3 ld = {L.dir[0], L.dir[1], L.dir[2]};
-      sdir = -ld;
-      tfar = 1.0e10f;
-      cos_n = dot(N, -ld);
-      if (!(cos_n < 0.f)) rad = f3{L.ic[0], L.ic[1], L.ic[2]};
-    } else {
-      f3 target;
-      f3 emit = {0.f, 0.f, 0.f};
-      bool area = type == kLightArea;
-      if (area) {
-        if (L.num_tris == 0) {
-          ap = 0.f;
-          target = f3{0.f, 0.f, 0.f};
-        } else {
-          // single-triangle emitters draw no triangle index
-          uint32_t tri = L.num_tris > 1 ? rng.index(L.num_tris) : 0u;
-          float u1 = rng.uniform();
-          float u2 = rng.uniform();
-          if (u1 + u2 > 1.0f) {
-            u1 = 1.0f - u1;
-            u2 = 1.0f - u2;
-          }
-          // single-triangle emitters (tri is 0) read a wave-uniform record
-          float tu = 0.f, tv = 0.f;
-          if (L.num_tris == 1)
-            area_point(uniform_load(sc.area_tris, L.first_tri), u1, u2, target, ap, tu, tv);
-          else
-            area_point(sc.area_tris[L.first_tri + tri], u1, u2, target, ap, tu, tv);
-          if (L.emit_const)
-            emit = f3{L.emit[0], L.emit[1], L.emit[2]};
-          else
-            emit = get_emission(sc, L.material, tu, tv);
-        }
-      } else {
-        target = f3{L.pos[0], L.pos[1], L.pos[2]};
-      }
-      f3 Lv = target - P;
-      float dist_sq = sqnorm_fma(Lv);
-      float dist = sqrtf(dist_sq);
-      sdir = target - P;
-      tfar = dist - 0.001f;
-      // one division per light (the reference divides per component:
-      // light.h:82-93, 127-143, 176-188) and fused sums; the oracle's
-      // GPU-stream mode follows this arithmetic
-      const float inv = 1.0f / dist;
-      const float inv2 = inv * inv;
-      Lv = Lv * inv;
-      cos_n = dot_fma(N, Lv);
-      if (type == kLightSpot) {
-        f3 ld = {L.dir[0], L.dir[1], L.dir[2]};
-        float cos_l = dot(ld, -Lv);
-        float falloff = (cos_l - L.cos_outer) / (L.cos_inner - L.cos_outer);
-        falloff = fminf(fmaxf(falloff, 0.0f), 1.0f);
-        if (!(cos_n < 0.f)) rad = f3{L.ic[0], L.ic[1], L.ic[2]} * inv2 * falloff;
-      } else if (type == kLightPoint) {
-        if (!(cos_n < 0.f)) rad = f3{L.ic[0], L.ic[1], L.ic[2]} * inv2;
-      } else {
-        if (!(cos_n < 0.f)) rad = emit * inv2;
-      }
-    }
-    // BRDF * cosine at a path vertex: scaled albedo for the contribution,
-    // unscaled for the selection weight
-    f3 radw = rad;
-    if (VERTEX && !(cos_n < 0.f)) {
-      bool ok = front && cos_n > 0.0f;
-      f3 bv = ok ? brdf : f3{0.f, 0.f, 0.f};
-      if (UNIT) {
-        rad = cmul(rad, cos_n * bv);
-        radw = rad;
-      } else {
-        f3 bw = ok ? brdf_w : f3{0.f, 0.f, 0.f};
-        radw = cmul(rad, cos_n * bw);
-        rad = cmul(rad, cos_n * bv);
-      }
-    }
-    if (!UNIT) {
-      const uint32_t l3 = 3u * (uint32_t)l;
-      rad = cmul(rad, f3{uniform_load(sc.light_scale, l3), uniform_load(sc.light_scale, l3 + 1),
-                         uniform_load(sc.light_scale, l3 + 2)});
-    }
+    f3 rad, radw, sdir;
+    float tfar, ap;
+    light_eval<VERTEX, UNIT>(sc, l, P, N, brdf, brdf_w, front, rng, rad, radw, sdir, tfar, ap);
     float w = maxc(radw);
     float u = rng.uniform();
     W += w;
@@ -254,6 +274,60 @@ __device__ __forceinline__ bool nee_sample(const SceneView& sc, f3 P, f3 N, f3 b
   out.tfar = ptfar;
   out.light = pick;
   return true;
+}
+
+// n > 1 light samples (light.cpp:164-188 and 266-290): the reference's n
+// picks with replacement from one set of per-light samples, as an inverse
+// CDF. A first pass sums W (the area draws of every light), then n
+// uniforms give targets u_j W, and a second pass replays the same area
+// draws from a copy of the starting state and gives target j to the light
+// whose interval of the running sum holds it -- light i with probability
+// w_i / W, as discrete_distribution's. pick(j, result, joint) is called
+// for every target whose joint pdf is at least 1e-3 (none when W < 1e-6),
+// with result.L the light's radiance (not yet divided by the pdf). rng
+// ends after the targets (the material sample continues from there).
+template <bool VERTEX, bool UNIT, class F>
+__device__ __forceinline__ void nee_multi(const SceneView& sc, f3 P, f3 N, f3 brdf, f3 brdf_w,
+                                          bool front, Rng& rng, int n, F&& pick) {
+  const Rng start = rng;
+  const int nl = sc.num_lights;
+  float W = 0.f;
+  for (int l = 0; l < nl; ++l) {
+    f3 rad, radw, sdir;
+    float tfar, ap;
+    light_eval<VERTEX, UNIT>(sc, l, P, N, brdf, brdf_w, front, rng, rad, radw, sdir, tfar, ap);
+    W += maxc(radw);
+  }
+  float target[kMaxLightSamples];
+#pragma unroll
+  for (int j = 0; j < kMaxLightSamples; ++j) target[j] = j < n ? rng.uniform() * W : 0.f;
+  if (W < 1e-6f) return;
+  Rng replay = start;
+  uint32_t open = (n >= 32 ? 0xffffffffu : (1u << n) - 1u);
+  float cum = 0.f;
+  for (int l = 0; l < nl && open; ++l) {
+    f3 rad, radw, sdir;
+    float tfar, ap;
+    light_eval<VERTEX, UNIT>(sc, l, P, N, brdf, brdf_w, front, replay, rad, radw, sdir, tfar,
+                             ap);
+    const float w = maxc(radw);
+    const float next = cum + w;
+#pragma unroll
+    for (int j = 0; j < kMaxLightSamples; ++j) {
+      if (!(open & (1u << j)) || !(w > 0.f) || !(target[j] < next)) continue;
+      open &= ~(1u << j);
+      const float joint = (w / W) * ap;
+      if (joint < 1e-3f) continue;
+      NeeResult r;
+      r.L = rad;
+      r.dir = sdir;
+      r.origin = P;
+      r.tfar = tfar;
+      r.light = l;
+      pick(j, r, joint);
+    }
+    cum = next;
+  }
 }
 
 }  // namespace shb

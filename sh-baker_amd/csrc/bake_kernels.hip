@@ -79,6 +79,22 @@ struct BakeShadowEpilogue {
   }
 };
 
+// Shadow rays of a bake with num_light_samples > 1: a visible pick sets its
+// flag (vertex picks: scontrib.w at slot < vlimit = nls * Pc; texel picks:
+// nee0.w at slot - vlimit); k_nee_gather sums the vertex picks in order.
+struct BakeShadowMultiEpilogue {
+  float4* scontrib;
+  float4* nee0;
+  uint32_t vlimit;
+  __device__ void operator()(uint32_t slot, const TravState& s) const {
+    if (s.best != 0x7fffffff) return;
+    if (slot < vlimit)
+      scontrib[slot].w = 1.0f;
+    else
+      nee0[slot - vlimit].w = 1.0f;
+  }
+};
+
 // Shadow rays of the adjoint replay: a visible vertex NEE ray flags its slot
 // (its gradient is added by k_adjoint_vertex), a visible texel NEE ray
 // marks the texel term as in the forward bake.
@@ -116,11 +132,18 @@ __device__ __forceinline__ void trace_begin(const TraceArgs& a, uint32_t slot, T
   trav_init(s, r, d.w);
   uint32_t root = a.root0;
   if (a.nseg > 1) {
+    // shadow slots: vertex pick j of slot p at j * Pc + p, texel pick j at
+    // nls * Pc + j * P + p (one pick: p and Pc + p)
+    const uint32_t nls = a.nls > 1 ? a.nls : 1u;
+    const uint32_t vlimit = nls * a.Pc;
     uint32_t p = slot;
-    if (p >= a.Pc)
-      p -= a.Pc;  // texel shadow rays sit at Pc + p
-    else if (p >= a.P)
-      p = a.link[p - a.P].x;  // a branch slot: its root
+    if (p >= vlimit) {
+      p -= vlimit;
+      while (p >= a.P) p -= a.P;  // pick j: at most nls - 1 steps
+    } else {
+      while (p >= a.Pc) p -= a.Pc;
+      if (p >= a.P) p = a.link[p - a.P].x;  // a branch slot: its root
+    }
     root = a.map->root[find_segment(a.map->slot_base, a.nseg, p)];
   }
   s.gbase = root;
@@ -450,18 +473,27 @@ __global__ void k_hit_details(SceneView sc, const float4* rays, const float4* hi
 // ------------------------------------------------------------ bake stages
 // Primary ray + texel NEE for every path slot of the batch (UNIT: every
 // light and albedo scale is 1).
-template <bool UNIT>
+template <bool UNIT, bool MULTI>
 __global__ void __launch_bounds__(256, 5) k_generate(BakeArgs a) {
   const uint32_t n = a.P;
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
     uint32_t p = base + threadIdx.x;
-    bool shadow = false;
+    uint32_t shadow = 0u;
     if (p < n) {
       const SlotRef sr = slot_ref(a, p);
-      with_job(a, sr.job, [&](const JobDev& J) { shadow = generate_path<UNIT>(a, J, sr, p); });
+      with_job(a, sr.job,
+               [&](const JobDev& J) { shadow = generate_path<UNIT, MULTI>(a, J, sr, p); });
     }
-    uint32_t at = wave_append(a.shadow_count, shadow);
-    if (shadow) a.shadow_queue[at] = a.Pc + p;
+    if (MULTI) {
+      for (uint32_t j = 0; j < a.nls; ++j) {  // wave-uniform bound
+        const bool made = (shadow >> j) & 1u;
+        uint32_t at = wave_append(a.shadow_count, made);
+        if (made) a.shadow_queue[at] = a.nls * a.Pc + j * a.P + p;
+      }
+    } else {
+      uint32_t at = wave_append(a.shadow_count, shadow != 0u);
+      if (shadow) a.shadow_queue[at] = a.Pc + p;
+    }
   }
 }
 
@@ -471,7 +503,7 @@ __global__ void __launch_bounds__(256, 5) k_generate(BakeArgs a) {
 // Every form runs at 5 waves per SIMD: a register cap with a small spill
 // (24-76 bytes per lane by form) is faster than 4 spill-free waves, since
 // the light loop waits on a scalar load per light.
-template <bool ADJ, bool SPLIT, bool UNIT>
+template <bool ADJ, bool SPLIT, bool UNIT, bool MULTI>
 __global__ void __launch_bounds__(256, 5) k_shade(BakeArgs a, int depth, const uint32_t* in_queue,
                                                   const uint32_t* in_count, uint32_t* out_queue,
                                                   uint32_t* out_count) {
@@ -482,17 +514,26 @@ __global__ void __launch_bounds__(256, 5) k_shade(BakeArgs a, int depth, const u
   const uint32_t n = in_queue ? *in_count : a.P;
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
     uint32_t i = base + threadIdx.x;
-    bool shadow = false, extend = false;
+    uint32_t shadow = 0u;
+    bool extend = false;
     uint32_t p = 0, child = kNoSlot;
     if (i < n) {
       p = in_queue ? in_queue[i] : i;
       const SlotRef sr = slot_ref(a, SPLIT ? root_slot(a, p) : p);
       with_job(a, sr.job, [&](const JobDev& J) {
-        shade_path<ADJ, SPLIT, UNIT>(a, J, sr, depth, p, shadow, extend, child, &acc);
+        shade_path<ADJ, SPLIT, UNIT, MULTI>(a, J, sr, depth, p, shadow, extend, child, &acc);
       });
     }
-    uint32_t at = wave_append(a.shadow_count, shadow);
-    if (shadow) a.shadow_queue[at] = p;
+    if (MULTI) {
+      for (uint32_t j = 0; j < a.nls; ++j) {  // wave-uniform bound
+        const bool made = (shadow >> j) & 1u;
+        uint32_t at = wave_append(a.shadow_count, made);
+        if (made) a.shadow_queue[at] = j * a.Pc + p;
+      }
+    } else {
+      uint32_t at = wave_append(a.shadow_count, shadow != 0u);
+      if (shadow) a.shadow_queue[at] = p;
+    }
     uint32_t at2 = wave_append(out_count, extend);
     if (extend) out_queue[at2] = p;
     if (SPLIT) {  // transmitted rays of splits
@@ -502,6 +543,33 @@ __global__ void __launch_bounds__(256, 5) k_shade(BakeArgs a, int depth, const u
     }
   }
   if (ADJ) acc.flush();
+}
+
+// num_light_samples > 1: each slot k_shade took at this depth adds its
+// visible vertex picks, in pick order, divided by its job's sample count
+// (EvaluateLightSamples, light.cpp:164-188: sum over the samples / n).
+__global__ void k_nee_gather(BakeArgs a, const uint32_t* in_queue, const uint32_t* in_count) {
+  const uint32_t n = in_queue ? *in_count : a.P;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t p = in_queue ? in_queue[i] : i;
+    f3 sum = {0.f, 0.f, 0.f};
+    bool any = false;
+    for (uint32_t j = 0; j < a.nls; ++j) {
+      const float4 c = a.scontrib[(size_t)j * a.Pc + p];
+      if (c.w != 0.f) {
+        sum = sum + f3_of(c);
+        any = true;
+      }
+    }
+    if (!any) continue;
+    const SlotRef sr = slot_ref(a, root_slot(a, p));
+    const float nj = (float)a.jobs[sr.job].nls;
+    float4 l = a.li[p];
+    l.x += sum.x / nj;
+    l.y += sum.y / nj;
+    l.z += sum.z / nj;
+    a.li[p] = l;
+  }
 }
 
 // Adjoint of the visible vertex NEE terms of one depth: the shadow queue's
@@ -640,6 +708,19 @@ hipError_t launch_trace_bake_shadow(const TraceArgs& a, const BakeArgs& b, bool 
                : launch_trace_t<true, false>(a, e, capacity, block, st);
 }
 
+hipError_t launch_trace_bake_shadow_multi(const TraceArgs& a, const BakeArgs& b, bool count,
+                                          uint32_t capacity, int block, hipStream_t st) {
+  BakeShadowMultiEpilogue e{b.scontrib, b.nee0, b.nls * b.Pc};
+  return count ? launch_trace_t<true, true>(a, e, capacity, block, st)
+               : launch_trace_t<true, false>(a, e, capacity, block, st);
+}
+
+hipError_t launch_nee_gather(const BakeArgs& b, const uint32_t* in_q, const uint32_t* in_n,
+                             int grid, hipStream_t st) {
+  hipLaunchKernelGGL(k_nee_gather, dim3(grid), dim3(256), 0, st, b, in_q, in_n);
+  return hipGetLastError();
+}
+
 hipError_t launch_trace_bake_shadow_adj(const TraceArgs& a, const BakeArgs& b, bool count,
                                         uint32_t capacity, int block, hipStream_t st) {
   BakeShadowAdjEpilogue e{b.adj_vflag, b.nee0, b.Pc};
@@ -656,10 +737,18 @@ hipError_t launch_hit_details(const SceneView& sc, const float4* rays, const flo
 }
 
 hipError_t launch_generate(const BakeArgs& b, int grid, hipStream_t st) {
-  if (b.unit_scales)
-    hipLaunchKernelGGL(k_generate<true>, dim3(grid), dim3(256), 0, st, b);
-  else
-    hipLaunchKernelGGL(k_generate<false>, dim3(grid), dim3(256), 0, st, b);
+  const bool multi = b.nls > 1;
+  if (b.unit_scales) {
+    if (multi)
+      hipLaunchKernelGGL((k_generate<true, true>), dim3(grid), dim3(256), 0, st, b);
+    else
+      hipLaunchKernelGGL((k_generate<true, false>), dim3(grid), dim3(256), 0, st, b);
+  } else {
+    if (multi)
+      hipLaunchKernelGGL((k_generate<false, true>), dim3(grid), dim3(256), 0, st, b);
+    else
+      hipLaunchKernelGGL((k_generate<false, false>), dim3(grid), dim3(256), 0, st, b);
+  }
   return hipGetLastError();
 }
 
@@ -672,19 +761,29 @@ hipError_t launch_shade(const BakeArgs& b, int depth, const uint32_t* in_q,
                         const uint32_t* in_n, uint32_t* out_q, uint32_t* out_n, int grid,
                         hipStream_t st) {
   const size_t lds = b.adj_grad ? adj_lds(b) : 0;
-#define SHB_SHADE(A, S, U)                                                                 \
-  hipLaunchKernelGGL((k_shade<A, S, U>), dim3(grid), dim3(256), lds, st, b, depth, in_q, in_n, \
-                     out_q, out_n)
+#define SHB_SHADE(A, S, U, M)                                                               \
+  hipLaunchKernelGGL((k_shade<A, S, U, M>), dim3(grid), dim3(256), lds, st, b, depth, in_q,    \
+                     in_n, out_q, out_n)
   const int v = (b.adj_grad ? 4 : 0) | (b.S ? 2 : 0) | (b.unit_scales ? 1 : 0);
+  if (b.nls > 1) {  // several light samples: the bake forms only (no adjoint)
+    switch (v) {
+      case 0: SHB_SHADE(false, false, false, true); break;
+      case 1: SHB_SHADE(false, false, true, true); break;
+      case 2: SHB_SHADE(false, true, false, true); break;
+      case 3: SHB_SHADE(false, true, true, true); break;
+      default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+  }
   switch (v) {
-    case 0: SHB_SHADE(false, false, false); break;
-    case 1: SHB_SHADE(false, false, true); break;
-    case 2: SHB_SHADE(false, true, false); break;
-    case 3: SHB_SHADE(false, true, true); break;
-    case 4: SHB_SHADE(true, false, false); break;
-    case 5: SHB_SHADE(true, false, true); break;
-    case 6: SHB_SHADE(true, true, false); break;
-    default: SHB_SHADE(true, true, true); break;
+    case 0: SHB_SHADE(false, false, false, false); break;
+    case 1: SHB_SHADE(false, false, true, false); break;
+    case 2: SHB_SHADE(false, true, false, false); break;
+    case 3: SHB_SHADE(false, true, true, false); break;
+    case 4: SHB_SHADE(true, false, false, false); break;
+    case 5: SHB_SHADE(true, false, true, false); break;
+    case 6: SHB_SHADE(true, true, false, false); break;
+    default: SHB_SHADE(true, true, true, false); break;
   }
 #undef SHB_SHADE
   return hipGetLastError();

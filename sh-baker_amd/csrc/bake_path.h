@@ -64,11 +64,13 @@ __device__ __forceinline__ void with_job(const BakeArgs& a, uint32_t job, F&& f)
 }
 
 // Primary ray and texel NEE of path slot p (baker.cpp:245-276,
-// light.cpp:191-293). Returns true when a texel shadow ray was made (stored
-// at shadow slot Pc + p). UNIT: every light and albedo scale is 1.
-template <bool UNIT>
-__device__ __forceinline__ bool generate_path(const BakeArgs& a, const JobDev& J,
-                                              const SlotRef& sr, uint32_t p) {
+// light.cpp:191-293). Returns the mask of texel shadow rays made: bit j for
+// the ray of pick j at shadow slot nls * Pc + j * P + p (nls = 1: bit 0,
+// slot Pc + p). UNIT: every light and albedo scale is 1. MULTI: the batch
+// has a job with num_light_samples > 1 (per-pick arrays, visible flags).
+template <bool UNIT, bool MULTI>
+__device__ __forceinline__ uint32_t generate_path(const BakeArgs& a, const JobDev& J,
+                                                  const SlotRef& sr, uint32_t p) {
   uint32_t texel = J.texel_list[sr.rank];
   const float4* sp = J.surface + 3 * (size_t)texel;
   float4 q0 = sp[0], q1 = sp[1], q2 = sp[2];
@@ -103,16 +105,42 @@ __device__ __forceinline__ bool generate_path(const BakeArgs& a, const JobDev& J
   a.li[p] = make_float4(0.f, 0.f, 0.f, 0.f);
   NeeResult nr;
   f3 zero = {0.f, 0.f, 0.f};
+  if (MULTI) {
+    // one entry per pick, invisible until the shadow trace flags it
+    for (uint32_t j = 0; j < a.nls; ++j)
+      a.nee0[(size_t)j * a.P + p] = make_float4(0.f, 0.f, 0.f, 0.f);
+    uint32_t made = 0;
+    auto put = [&](int j, const f3& Li, const NeeResult& r) {
+      const size_t e = (size_t)j * a.P + p;
+      a.nee0[e] = make_float4(Li.x, Li.y, Li.z, 0.f);
+      a.nee0_dir[e] = make_float4(r.dir.x, r.dir.y, r.dir.z, as_float((uint32_t)r.light));
+      const size_t so = (size_t)a.nls * a.Pc + e;
+      a.sray[2 * so] = make_float4(P.x, P.y, P.z, kRayEps);
+      a.sray[2 * so + 1] = make_float4(r.dir.x, r.dir.y, r.dir.z, r.tfar);
+      made |= 1u << j;
+    };
+    const int n = (int)J.nls;
+    if (n > 1) {
+      // Li = L / (joint n) per visible pick (light.cpp:287-288)
+      nee_multi<false, UNIT>(J.scene, P, N, zero, zero, false, rng, n,
+                             [&](int j, const NeeResult& r, float joint) {
+                               put(j, r.L / (joint * (float)n), r);
+                             });
+    } else if (nee_sample<false, UNIT>(J.scene, P, N, zero, zero, false, rng, nr)) {
+      put(0, nr.L, nr);
+    }
+    return made;
+  }
   if (nee_sample<false, UNIT>(J.scene, P, N, zero, zero, false, rng, nr)) {
     a.nee0[p] = make_float4(nr.L.x, nr.L.y, nr.L.z, 0.f);
     a.nee0_dir[p] = make_float4(nr.dir.x, nr.dir.y, nr.dir.z, as_float((uint32_t)nr.light));
     size_t so = (size_t)a.Pc + p;
     a.sray[2 * so] = make_float4(P.x, P.y, P.z, kRayEps);
     a.sray[2 * so + 1] = make_float4(nr.dir.x, nr.dir.y, nr.dir.z, nr.tfar);
-    return true;
+    return 1u;
   }
   a.nee0[p] = make_float4(0.f, 0.f, 0.f, 0.f);
-  return false;
+  return 0u;
 }
 
 // Appends a wave's spawning lanes to the batch's branch slots (one atomic
@@ -133,22 +161,30 @@ __device__ __forceinline__ uint32_t claim_branch_slot(const BakeArgs& a) {
 }
 
 // Shading of path slot p's closest hit at depth `depth` (baker.cpp:94-146):
-// sky, alpha split, emission at depth 0, vertex NEE (shadow slot p),
-// cosine-weighted continuation. Sets shadow / extend when a ray was made and
-// `child` to the branch slot of a transmitted ray (kNoSlot: none). `sr` is
-// the root slot's texel and sample.
+// sky, alpha split, emission at depth 0, vertex NEE (shadow slot p; pick j
+// at j * Pc + p), cosine-weighted continuation. Sets `shadow` to the mask of
+// vertex shadow rays made (bit j: pick j), `extend` when a continuation ray
+// was made and `child` to the branch slot of a transmitted ray (kNoSlot:
+// none). `sr` is the root slot's texel and sample. MULTI: per-pick
+// contributions with visible flags (the batch has a job with
+// num_light_samples > 1); every entry of the slot is cleared first, so
+// k_nee_gather reads zeros for picks not made at this depth.
 // ADJ: records the vertex's material for the replay adjoint and adds the
 // gradient of the depth-0 emission term to `acc`.
 // SPLIT: the batch has spare slots (S > 0). Without them no material has a
 // texel with alpha < 255 (the host sets S from the texel pool), every alpha
 // is 1 and the split code is compiled out. UNIT: every light and albedo
 // scale is 1.
-template <bool ADJ, bool SPLIT, bool UNIT>
+template <bool ADJ, bool SPLIT, bool UNIT, bool MULTI>
 __device__ __forceinline__ void shade_path(const BakeArgs& a, const JobDev& J, const SlotRef& sr,
-                                           int depth, uint32_t p, bool& shadow, bool& extend,
+                                           int depth, uint32_t p, uint32_t& shadow, bool& extend,
                                            uint32_t& child, GradAcc* acc) {
-  shadow = extend = false;
+  shadow = 0u;
+  extend = false;
   child = kNoSlot;
+  if (MULTI)
+    for (uint32_t j = 0; j < a.nls; ++j)
+      a.scontrib[(size_t)j * a.Pc + p] = make_float4(0.f, 0.f, 0.f, 0.f);
   const SceneView& sc = J.scene;
   float4 h = a.hit[p];
   int id = (int)as_uint(h.y);
@@ -249,12 +285,31 @@ __device__ __forceinline__ void shade_path(const BakeArgs& a, const JobDev& J, c
   f3 brdf = lambert(sc, mat, tu, tv, &brdf_w);
   Rng rng = site_rng(J.seed, texel, sr.smp, (uint32_t)depth, branch, kSiteVertex, J.stream);
   NeeResult nr;
-  if (nee_sample<true, UNIT>(sc, hit_pos, nrm, brdf, brdf_w, front, rng, nr)) {
+  if (MULTI) {
+    // pick j: contribution thr x L / joint, visible flag in w (0 until the
+    // shadow trace); k_nee_gather adds the visible ones / n
+    auto put = [&](int j, const f3& Lj, const NeeResult& r) {
+      const f3 c = cmul(thr, Lj);
+      const size_t e = (size_t)j * a.Pc + p;
+      a.scontrib[e] = make_float4(c.x, c.y, c.z, 0.f);
+      a.sray[2 * e] = make_float4(hit_pos.x, hit_pos.y, hit_pos.z, kRayEps);
+      a.sray[2 * e + 1] = make_float4(r.dir.x, r.dir.y, r.dir.z, r.tfar);
+      shadow |= 1u << j;
+    };
+    const int n = (int)J.nls;
+    if (n > 1)
+      nee_multi<true, UNIT>(sc, hit_pos, nrm, brdf, brdf_w, front, rng, n,
+                            [&](int j, const NeeResult& r, float joint) {
+                              put(j, r.L / joint, r);
+                            });
+    else if (nee_sample<true, UNIT>(sc, hit_pos, nrm, brdf, brdf_w, front, rng, nr))
+      put(0, nr.L, nr);
+  } else if (nee_sample<true, UNIT>(sc, hit_pos, nrm, brdf, brdf_w, front, rng, nr)) {
     f3 c = cmul(thr, nr.L);
     a.scontrib[p] = make_float4(c.x, c.y, c.z, as_float((uint32_t)nr.light));
     a.sray[2 * (size_t)p] = make_float4(hit_pos.x, hit_pos.y, hit_pos.z, kRayEps);
     a.sray[2 * (size_t)p + 1] = make_float4(nr.dir.x, nr.dir.y, nr.dir.z, nr.tfar);
-    shadow = true;
+    shadow = 1u;
   }
   if (add.x != 0.f || add.y != 0.f || add.z != 0.f) {
     float4 l = a.li[p];
@@ -353,16 +408,20 @@ __device__ __forceinline__ void project_texel(const BakeArgs& a, const JobDev& J
   };
   for (uint32_t smp = 0; smp < J.spp; ++smp) {
     uint32_t p = slot0 + smp * stride;
-    float4 n0 = a.nee0[p];
     float Y[9];
-    if (n0.w != 0.f) {
-      float4 nd = a.nee0_dir[p];
-      sh_basis(f3_of(nd), Y);
+    // the texel NEE picks in order (one unless num_light_samples > 1)
+    for (uint32_t j = 0; j < a.nls; ++j) {
+      const size_t e = (size_t)j * a.P + p;
+      float4 n0 = a.nee0[e];
+      if (n0.w != 0.f) {
+        float4 nd = a.nee0_dir[e];
+        sh_basis(f3_of(nd), Y);
 #pragma unroll
-      for (int i = 0; i < 9; ++i) {
-        addk(3 * i + 0, n0.x * Y[i]);
-        addk(3 * i + 1, n0.y * Y[i]);
-        addk(3 * i + 2, n0.z * Y[i]);
+        for (int i = 0; i < 9; ++i) {
+          addk(3 * i + 0, n0.x * Y[i]);
+          addk(3 * i + 1, n0.y * Y[i]);
+          addk(3 * i + 2, n0.z * Y[i]);
+        }
       }
     }
     float4 l = a.li[p];

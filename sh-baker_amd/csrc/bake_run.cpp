@@ -20,8 +20,9 @@ int prepare_job_async(JobIn& j, hipStream_t stream, uint32_t* count_dev) {
     return fail(c, SHB_ERR_INVALID_ARG, "samples must be in [1, 65535]");
   if (cfg->bounces < 0 || cfg->bounces > kMaxBounces)
     return fail(c, SHB_ERR_INVALID_ARG, "bounces must be in [0, 15]");
-  if (cfg->num_light_samples != 1)
-    return fail(c, SHB_ERR_UNSUPPORTED, "num_light_samples must be 1");
+  if (cfg->num_light_samples < 1 || cfg->num_light_samples > kMaxLightSamples)
+    return fail(c, SHB_ERR_INVALID_ARG,
+                "num_light_samples must be in [1, " + std::to_string(kMaxLightSamples) + "]");
   if (c.hs.bad_material)
     return fail(c, SHB_ERR_INVALID_ARG,
                 "a geometry's material_id does not name a material (the bake shades every hit)");
@@ -191,6 +192,7 @@ int run_pass(Ctx& x, std::vector<JobIn>& jobs, bool graph, BakeRun* run, const A
     J.spp = (uint32_t)jobs[k].cfg.samples;
     J.bounces = (uint32_t)jobs[k].cfg.bounces;
     J.stream = jobs[k].stream;
+    J.nls = (uint32_t)jobs[k].cfg.num_light_samples;
   }
   SHB_TRY(x, x.d_jobs.reserve(std::max<size_t>(nj * sizeof(JobDev), 16)));
   SHB_TRY(x, hipMemcpyAsync(x.d_jobs.ptr, jt.data(), nj * sizeof(JobDev), hipMemcpyHostToDevice,
@@ -241,10 +243,18 @@ int run_pass(Ctx& x, std::vector<JobIn>& jobs, bool graph, BakeRun* run, const A
   SHB_TRY(x, hipMemcpyAsync(x.d_maps.ptr, maps.data(), maps.size() * sizeof(BatchMap),
                             hipMemcpyHostToDevice, st));
 
-  // path state: 13 float4 per slot (+1 for the adjoint's projection
-  // weights), Pc = P + S slots; queues: two extension queues of Pc, the
-  // shadow queue of Pc + P (vertex and texel rays); branch links
-  const size_t f4_per_slot = adj ? 14 : 13;
+  // light samples per NEE: the largest of the jobs' (per-pick arrays)
+  uint32_t nls = 1;
+  for (const JobIn& j : jobs) nls = std::max(nls, (uint32_t)j.cfg.num_light_samples);
+  if (adj && nls > 1)
+    return fail(x, SHB_ERR_UNSUPPORTED, "the adjoint takes one light sample per NEE");
+  // path state per slot, Pc = P + S slots: extension ray 2, hit, throughput,
+  // radiance, texel NEE value and direction nls each, primary direction,
+  // shadow rays 4 nls (vertex and texel, 2 float4 each), vertex NEE
+  // contribution nls: 6 + 7 nls float4 (13 with one light sample; +1 for the
+  // adjoint's projection weights). Queues: two extension queues of Pc, the
+  // shadow queue of nls (Pc + P) (vertex and texel rays); branch links.
+  const size_t f4_per_slot = adj ? 14 : 6 + 7 * (size_t)nls;
   SHB_TRY(x,
           x.d_paths.reserve(std::max<size_t>((size_t)Pcmax * f4_per_slot * sizeof(float4), 16)));
   uint32_t maxB_all = 0;
@@ -252,8 +262,8 @@ int run_pass(Ctx& x, std::vector<JobIn>& jobs, bool graph, BakeRun* run, const A
   // adjoint: vertex materials (u16 per depth and slot), visible flags
   const size_t hist_bytes = ((size_t)(maxB_all + 1) * Pcmax * 2 + 15) / 16 * 16;
   if (adj) SHB_TRY(x, x.d_adj.reserve(hist_bytes + (size_t)Pcmax * 4 + 16));
-  SHB_TRY(x, x.d_queues.reserve(
-                 std::max<size_t>(((size_t)Pcmax * 3 + Pmax) * sizeof(uint32_t), 16)));
+  SHB_TRY(x, x.d_queues.reserve(std::max<size_t>(
+                 ((size_t)Pcmax * 2 + (size_t)nls * (Pcmax + Pmax)) * sizeof(uint32_t), 16)));
   if (S) SHB_TRY(x, x.d_link.reserve((size_t)S * sizeof(uint4)));
   SHB_TRY(x, x.d_counters.reserve(std::max<size_t>((size_t)nb * kCntStride * 4, 64)));
   SHB_TRY(x, x.d_work.reserve(kWorkBytes));
@@ -275,6 +285,7 @@ int run_pass(Ctx& x, std::vector<JobIn>& jobs, bool graph, BakeRun* run, const A
   tbase.refill = (uint32_t)x.tuning.queue_chunk;
   tbase.num_cus = x.num_cus;
   tbase.error = x.d_work.as<uint32_t>();
+  tbase.nls = nls;
 
   // event pairs around each traversal launch; kind 0 extension, 1 shadow
   std::vector<hipEvent_t> tev;
@@ -322,15 +333,16 @@ int run_pass(Ctx& x, std::vector<JobIn>& jobs, bool graph, BakeRun* run, const A
       b.link = S ? x.d_link.as<uint4>() : nullptr;
       b.spawn_count = cnt + kCntSpawn;
       b.unit_scales = unit ? 1u : 0u;
+      b.nls = nls;
       b.ray = base;
       b.hit = base + 2 * (size_t)Pc;
       b.thr = base + 3 * (size_t)Pc;
       b.li = base + 4 * (size_t)Pc;
       b.nee0 = base + 5 * (size_t)Pc;
-      b.nee0_dir = base + 6 * (size_t)Pc;
-      b.dir0 = base + 7 * (size_t)Pc;
-      b.sray = base + 8 * (size_t)Pc;
-      b.scontrib = base + 12 * (size_t)Pc;
+      b.nee0_dir = base + (5 + (size_t)nls) * Pc;
+      b.dir0 = base + (5 + 2 * (size_t)nls) * Pc;
+      b.sray = base + (6 + 2 * (size_t)nls) * Pc;
+      b.scontrib = base + (6 + 6 * (size_t)nls) * Pc;
       uint32_t* qext[2] = {q, q + Pc};
       b.shadow_queue = q + 2 * (size_t)Pc;
       b.shadow_count = cnt + kCntSh;
@@ -392,14 +404,19 @@ int run_pass(Ctx& x, std::vector<JobIn>& jobs, bool graph, BakeRun* run, const A
           r.add(5, sa);
         } else {
           trace_ev(1);
+          const uint32_t sh_cap = nls * (Pc + P);
           if (adj)
             SHB_TRY(x,
-                    launch_trace_bake_shadow_adj(sa, b, count, Pc + P, x.tuning.block_size, st));
+                    launch_trace_bake_shadow_adj(sa, b, count, sh_cap, x.tuning.block_size, st));
+          else if (nls > 1)
+            SHB_TRY(x, launch_trace_bake_shadow_multi(sa, b, count, sh_cap, x.tuning.block_size,
+                                                      st));
           else
-            SHB_TRY(x, launch_trace_bake_shadow(sa, b, count, Pc + P, x.tuning.block_size, st));
+            SHB_TRY(x, launch_trace_bake_shadow(sa, b, count, sh_cap, x.tuning.block_size, st));
           trace_ev(1);
           ++run->launches_sh;
           SHB_TRY(x, stage_done(st, "shadow", bi, (int)d));
+          if (nls > 1) SHB_TRY(x, launch_nee_gather(b, ext_q, cnt + kCntExt + d, sgrid, st));
           if (adj) SHB_TRY(x, launch_adjoint_vertex(b, (int)d, cnt + kCntSh + d, sgrid, st));
         }
       }

@@ -21,7 +21,7 @@ struct JobDev {
   uint32_t spp;
   uint32_t bounces;
   uint32_t stream;             // random-stream id (the counter's job word)
-  uint32_t pad;
+  uint32_t nls;                // light samples per NEE (BakeConfig::num_light_samples)
 };
 
 // Path slots of one wavefront batch: segment s holds texels
@@ -82,6 +82,9 @@ struct TraceArgs {
   // persistent form: test the triangles of a step's nodes spread over the
   // wave's lanes (needs fewer than 2^26 triangles)
   uint32_t wave_tris;
+  // light samples per NEE (the batch's largest): shadow slots j * Pc + p
+  // are vertex rays of path slot p, nls * Pc + j * P + p texel rays of p
+  uint32_t nls;
 };
 
 struct BakeArgs {
@@ -102,6 +105,12 @@ struct BakeArgs {
   uint4* link;
   uint32_t* spawn_count;
   uint32_t unit_scales;       // every job's light and albedo scales are all 1
+  // light samples per NEE, the batch's largest (a job's own: JobDev::nls).
+  // Per-pick arrays hold nls entries per slot: nee0 / nee0_dir at j * P + p,
+  // scontrib at j * Pc + p; shadow slots j * Pc + p (vertex) and
+  // nls * Pc + j * P + p (texel). With nls > 1, scontrib.w is the pick's
+  // visible flag (set by the shadow trace, summed by k_nee_gather).
+  uint32_t nls;
   float4* ray;                // 2 per slot: extension ray
   float4* hit;                // 1 per slot: (t, id, u, v)
   float4* thr;                // throughput, branch id in w
@@ -186,6 +195,13 @@ hipError_t launch_adjoint_vertex(const BakeArgs& b, int depth, const uint32_t* s
                                  hipStream_t st);
 hipError_t launch_trace_bake_shadow_adj(const TraceArgs& a, const BakeArgs& b, bool count,
                                         uint32_t capacity, int block, hipStream_t st);
+// b.nls > 1: the shadow trace flags visible picks; k_nee_gather then adds
+// each shaded slot's visible vertex picks (in pick order, / its job's
+// nls) to its radiance. The slots are those k_shade took at `depth`.
+hipError_t launch_trace_bake_shadow_multi(const TraceArgs& a, const BakeArgs& b, bool count,
+                                          uint32_t capacity, int block, hipStream_t st);
+hipError_t launch_nee_gather(const BakeArgs& b, const uint32_t* in_q, const uint32_t* in_n,
+                             int grid, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t st);
 // Copies n BVH nodes, shifting child and triangle bases by node_off / tri_off
 // (concatenated trees of a batched bake).

@@ -41,6 +41,7 @@ namespace shb {
 // ------------------------------------------------------------ limits
 constexpr int kMaxBounces = 15;       // depth fits 4 bits of the RNG counter
 constexpr int kMaxSamples = 65535;    // sample fits 16 bits of the counter
+constexpr int kMaxLightSamples = 16;  // light samples per NEE (a pick bit mask)
 constexpr int kStackLds = 16;         // traversal stack entries (LDS)
 constexpr float kRayEps = 0.001f;     // baker.cpp:118, 122, light.h:40,83
 

@@ -300,7 +300,8 @@ class Baker:
         self._chk(lib.shb_bake(self.h, C.byref(cfg), _f(sh), _f(vis), C.byref(st)))
         return sh, vis, st.as_dict()
 
-    def bake_tiled(self, samples, bounces, tiles, seed=12345, flags=0, want=True):
+    def bake_tiled(self, samples, bounces, tiles, seed=12345, flags=0, want=True,
+                   num_light_samples=1):
         """shb_bake_tiled: the surface baked as tile jobs (x0, y0, w, h in
         supersampled texels) of one batched run; returns (sh, vis, stats)
         like bake (texels outside every tile are 0)."""
@@ -309,7 +310,7 @@ class Baker:
         vis = np.zeros(n, np.float32) if want else None
         arr = (ShbTile * max(1, len(tiles)))(*[ShbTile(*map(int, t)) for t in tiles])
         st = ShbBakeStats()
-        cfg = self.cfg(samples, bounces, seed, flags)
+        cfg = self.cfg(samples, bounces, seed, flags, num_light_samples)
         self._chk(lib.shb_bake_tiled(self.h, C.byref(cfg), arr, len(tiles), _f(sh), _f(vis),
                                      C.byref(st)))
         return sh, vis, st.as_dict()
@@ -404,7 +405,8 @@ class Baker:
         t = ShbTuning(**cur)
         self._chk(lib.shb_set_tuning(self.h, C.byref(t)))
 
-    def bake_adjoint(self, samples, bounces, dloss_dsh, seed=12345, flags=0):
+    def bake_adjoint(self, samples, bounces, dloss_dsh, seed=12345, flags=0,
+                     num_light_samples=1):
         """Path-replay adjoint of bake(): given dL/dSH at the supersampled
         resolution ((n, 27) like bake's sh), returns dL/d(light_scale)
         (lights, 3), dL/d(albedo_scale) (materials, 3) and the stats."""
@@ -414,7 +416,7 @@ class Baker:
         dl = np.zeros((nl, 3), np.float32)
         dm = np.zeros((nm, 3), np.float32)
         st = ShbBakeStats()
-        cfg = self.cfg(samples, bounces, seed, flags)
+        cfg = self.cfg(samples, bounces, seed, flags, num_light_samples)
         self._chk(lib.shb_bake_adjoint(self.h, C.byref(cfg), _f(g), _f(dl), _f(dm), C.byref(st)))
         return dl, dm, st.as_dict()
 
@@ -447,17 +449,19 @@ class Baker:
         self._chk(lib.shb_set_params(self.h, _f(ls), _f(al)))
 
 
-def bake_batched(bakers, samples, bounces, seeds=None, flags=0, want=True):
+def bake_batched(bakers, samples, bounces, seeds=None, flags=0, want=True,
+                 num_light_samples=1):
     """Bakes several Bakers (each with its own scene and surface, all on one
     device) in one batched run (shared wavefront batches, hipGraph replay).
-    samples / bounces / seeds: one value per job or a scalar. Returns
-    ([sh], [vis], stats) like Baker.bake per job."""
+    samples / bounces / seeds / num_light_samples: one value per job or a
+    scalar. Returns ([sh], [vis], stats) like Baker.bake per job."""
     n = len(bakers)
     as_list = lambda v: list(v) if isinstance(v, (list, tuple, np.ndarray)) else [v] * n
     samples, bounces = as_list(samples), as_list(bounces)
     seeds = as_list(12345 if seeds is None else seeds)
-    cfgs = (ShbBakeCfg * n)(*[Baker.cfg(int(samples[i]), int(bounces[i]), int(seeds[i]), flags)
-                              for i in range(n)])
+    nls = as_list(num_light_samples)
+    cfgs = (ShbBakeCfg * n)(*[Baker.cfg(int(samples[i]), int(bounces[i]), int(seeds[i]), flags,
+                                        int(nls[i])) for i in range(n)])
     ctxs = (C.c_void_p * n)(*[b.h for b in bakers])
     shs, viss = [], []
     for b in bakers:

@@ -176,7 +176,13 @@ def test_bake_errors(baker):
     assert e.value.code == -6
     fresh.rasterize(64, 64, 1, want=False)
     with pytest.raises(shb.ShbError) as e:
-        fresh.bake(4, 1, num_light_samples=2)
+        fresh.bake(4, 1, num_light_samples=17)
+    assert e.value.code == -1
+    with pytest.raises(shb.ShbError) as e:
+        fresh.bake(4, 1, num_light_samples=0)
+    assert e.value.code == -1
+    with pytest.raises(shb.ShbError) as e:  # the adjoint takes one light sample
+        fresh.bake_adjoint(4, 1, np.zeros((64 * 64, 27), np.float32), num_light_samples=2)
     assert e.value.code == -7
     with pytest.raises(shb.ShbError) as e:
         fresh.bake(0, 1)

@@ -154,3 +154,31 @@ def test_tiled_bake_rejects_bad_tiles():
             assert e.value.code in (-1, -5)
     finally:
         b.close()
+
+
+def test_batched_jobs_with_different_light_sample_counts():
+    """Jobs of one batch with 1, 3 and 2 light samples per NEE: per-pick
+    arrays sized for the largest, each job bit-identical to its own plain
+    bake (the one-sample job included)."""
+    specs = [(scenes.scaled_synthetic_scene(4000, 48, 31), 48, 1),
+             (scenes.scaled_synthetic_scene(6000, 64, 32), 64, 3),
+             (scenes.box_room_scene(40, 40), 40, 2)]
+    bakers = []
+    try:
+        for sc, res, _ in specs:
+            b = shb.Baker(0)
+            b.upload_scene(sc)
+            b.rasterize(res, res, 1, want=False)
+            bakers.append(b)
+        nls = [n for _, _, n in specs]
+        ref = [b.bake(8, 3, seed=5, num_light_samples=n)[:2] for b, n in zip(bakers, nls)]
+        sh, vis, _ = shb.bake_batched(bakers, 8, 3, 5, num_light_samples=nls)
+        for k in range(len(specs)):
+            assert np.array_equal(bits(sh[k]), bits(ref[k][0])), k
+            assert np.array_equal(bits(vis[k]), bits(ref[k][1])), k
+        # the one-sample job alone is also the plain one-sample bake
+        one = bakers[0].bake(8, 3, seed=5)[0]
+        assert np.array_equal(bits(one), bits(ref[0][0]))
+    finally:
+        for b in bakers:
+            b.close()

@@ -48,3 +48,20 @@ def test_gpu_same_stream_as_oracle_on_pinned_scenes(baker, name):
     per_texel = (np.abs(g_sh - o_sh) / scale).max(axis=1)
     assert (per_texel < 1e-3).mean() > 0.97, np.sort(per_texel)[-10:]
     assert np.abs(g_vis - o_vis).max() <= 2.0 / 16
+
+
+@pytest.mark.parametrize("name,B", [("point_floor", 1), ("spot_down_plates", 2),
+                                    ("spot_up_plates", 1), ("area_near", 1)])
+def test_gpu_several_light_samples_match_closed_form(baker, name, B):
+    """num_light_samples = 4 (light.cpp:164-188, 266-290): four picks per
+    NEE, their mean; the expectation is the same as with one."""
+    c = CASES[name]
+    print(f"closed form {name} bounces {B}, 4 light samples", flush=True)
+    baker.upload_scene(c["scene"]())
+    pts, w, h = pcf.texel_surface(c["P"], c["N"], N_GPU)
+    baker.set_surface(pts, w, h, 1)
+    sh, vis, _ = baker.bake(SPP, B, seed=2025, num_light_samples=4)
+    exp_sh, exp_vis = expected(name, B)
+    a_sh, a_vis = agreement(sh, vis, SPP, exp_sh, exp_vis)
+    print(f"  worst |GPU - closed form| / tolerance: SH {a_sh:.3f}, visibility {a_vis:.3f}")
+    assert a_sh <= 1.0 and a_vis <= 1.0

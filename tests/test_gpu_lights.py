@@ -71,3 +71,31 @@ def test_mixed_lights_within_3_sigma_of_reference_stream(baker, room):
     assert 0.6 < np.abs(z).mean() < 1.0, np.abs(z).mean()
     assert (np.abs(z) > 4.5).mean() < 2e-3
     np.testing.assert_allclose(g_vis[v].mean(), r_vis[v].mean(), atol=0.02)
+
+
+@pytest.mark.parametrize("nls", [2, 5])
+def test_several_light_samples_same_stream_as_oracle(baker, room, nls):
+    """num_light_samples > 1: the GPU's n picks per NEE (inverse CDF over
+    one set of per-light samples) against the oracle's GPU-stream
+    restatement, and within the Monte Carlo error of the reference stream
+    (std::discrete_distribution drawn n times, light.cpp:164-188)."""
+    s, o = room
+    baker.upload_scene(s)
+    pts = baker.rasterize(RES, RES, 1)
+    spp, B = 16, 3
+    g_sh, g_vis, g_st = baker.bake(spp, B, seed=808, num_light_samples=nls)
+    o_sh, o_vis, o_var, o_st = o.bake(pts, RES, RES, spp, B, mode=orc.MODE_GPU, seed=808,
+                                      num_light_samples=nls)
+    v = pts["material_id"] >= 0
+    scale = np.abs(o_sh[v]).max(axis=1, keepdims=True) + 1e-6
+    per_texel = (np.abs(g_sh[v] - o_sh[v]) / scale).max(axis=1)
+    print(f"n={nls}: share within 1e-3 {np.mean(per_texel < 1e-3):.4f}", flush=True)
+    assert (per_texel < 1e-3).mean() > 0.97, np.sort(per_texel)[-20:]
+    n_sh = o_st["rays_shadow"]
+    assert abs(g_st["rays_shadow"] - n_sh) <= 2e-3 * n_sh
+    r_sh, _, r_var, _ = o.bake(pts, RES, RES, spp, B, mode=orc.MODE_REF, num_light_samples=nls)
+    live = (r_var[v] + o_var[v]) > 0
+    z = (g_sh[v] - r_sh[v])[live] / np.sqrt(r_var[v] + o_var[v])[live]
+    print(f"  against the reference stream: mean |z| {np.abs(z).mean():.3f}")
+    assert 0.6 < np.abs(z).mean() < 1.0
+    assert (np.abs(z) > 4.5).mean() < 2e-3

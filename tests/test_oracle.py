@@ -612,3 +612,19 @@ def test_two_streams_agree_with_spot_and_area_lights():
     z = (g[v] - r[v])[live] / np.sqrt(gv[v] + rv[v])[live]
     assert 0.6 < np.abs(z).mean() < 1.0, np.abs(z).mean()
     assert (np.abs(z) > 4.5).mean() < 2e-3
+
+
+def test_two_streams_agree_with_several_light_samples():
+    """num_light_samples = 3: the GPU stream's inverse-CDF picks against
+    the reference's discrete_distribution drawn three times."""
+    s = scenes.scaled_synthetic_scene(10000, 96, 77, n_spot=4, emitter_tris=3,
+                                      textured_emission=True, emitter_y=(14.3, 14.8))
+    o = orc.Oracle(s)
+    pts = o.rasterize(96, 96, 1)
+    v = pts["material_id"] >= 0
+    g, _, gv, _ = o.bake(pts, 96, 96, 64, 3, mode=orc.MODE_GPU, seed=99, num_light_samples=3)
+    r, _, rv, _ = o.bake(pts, 96, 96, 64, 3, mode=orc.MODE_REF, num_light_samples=3)
+    live = (gv[v] + rv[v]) > 0
+    z = (g[v] - r[v])[live] / np.sqrt(gv[v] + rv[v])[live]
+    assert 0.6 < np.abs(z).mean() < 1.0, np.abs(z).mean()
+    assert (np.abs(z) > 4.5).mean() < 2e-3

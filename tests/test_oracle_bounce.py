@@ -143,3 +143,19 @@ def test_closed_form_building_blocks():
     # sRGB decode as the bake does it (the oracle's table)
     for b in (0, 10, 128, pcf.FLOOR_BYTE, 255):
         assert pcf.srgb_to_linear(b) == orc.srgb_to_linear(b)
+
+
+@pytest.mark.parametrize("n", [2, 4])
+@pytest.mark.parametrize("name,B", [("point_floor", 1), ("spot_down_plates", 2),
+                                    ("spot_up_plates", 3), ("area_near", 1)])
+def test_oracle_several_light_samples_match_closed_form(name, B, n):
+    """num_light_samples = n (light.cpp:164-188, 266-290): n picks per NEE
+    and their mean, in both streams; the expectation does not depend on n."""
+    exp_sh, exp_vis = expected(name, B)
+    c = CASES[name]
+    pts, w, h = pcf.texel_surface(c["P"], c["N"], N_TEXELS)
+    for mode in (orc.MODE_REF, orc.MODE_GPU):
+        sh, vis, _, _ = oracle_of(name).bake(pts, w, h, SPP, B, mode=mode, seed=7,
+                                             num_light_samples=n)
+        a_sh, a_vis = agreement(sh, vis, SPP, exp_sh, exp_vis)
+        assert a_sh <= 1.0 and a_vis <= 1.0, (name, B, n, mode, a_sh)
