@@ -141,14 +141,48 @@ __device__ __forceinline__ float dot_fma(f3 a, f3 b) {
 // (direction, tfar) and the area pdf; an area light first draws its triangle
 // (when it has several) and two barycentric uniforms from rng. VERTEX:
 // BRDF x cosine applied (light.h *Radiance<Brdf>); else the texel form.
+// The random numbers light L takes (an area light: its triangle when it has
+// several, then two barycentric uniforms; other lights: none).
+struct LightDraws {
+  uint32_t tri;
+  float u1, u2;
+};
+__device__ __forceinline__ LightDraws light_draws(const LightDev& L, Rng& rng) {
+  LightDraws d{0u, 0.f, 0.f};
+  if (L.type == kLightArea && L.num_tris != 0) {
+    // single-triangle emitters draw no triangle index
+    d.tri = L.num_tris > 1 ? rng.index(L.num_tris) : 0u;
+    d.u1 = rng.uniform();
+    d.u2 = rng.uniform();
+  }
+  return d;
+}
+
+template <bool VERTEX, bool UNIT>
+__device__ __forceinline__ void light_geom(const SceneView& sc, int l, const LightDev& L,
+                                           const LightDraws& dr, f3 P, f3 N, f3 brdf,
+                                           f3 brdf_w, bool front, f3& rad_out, f3& radw_out,
+                                           f3& sdir_out, float& tfar_out, float& ap_out);
+
 template <bool VERTEX, bool UNIT>
 __device__ __forceinline__ void light_eval(const SceneView& sc, int l, f3 P, f3 N, f3 brdf,
                                            f3 brdf_w, bool front, Rng& rng, f3& rad_out,
                                            f3& radw_out, f3& sdir_out, float& tfar_out,
                                            float& ap_out) {
-  // every lane visits the same light: a wave-uniform index lets the light
-  // record come through the scalar cache
-  const LightDev L = uniform_load(sc.lights, (uint32_t)l);
+  // where every lane of the wave shares the job, the light record comes
+  // through the scalar cache
+  const LightDev L = const_load(sc.lights, (uint32_t)l);
+  const LightDraws dr = light_draws(L, rng);
+  light_geom<VERTEX, UNIT>(sc, l, L, dr, P, N, brdf, brdf_w, front, rad_out, radw_out, sdir_out,
+                           tfar_out, ap_out);
+}
+
+// Light l's values from its draws (light_eval without the random numbers).
+template <bool VERTEX, bool UNIT>
+__device__ __forceinline__ void light_geom(const SceneView& sc, int l, const LightDev& L,
+                                           const LightDraws& dr, f3 P, f3 N, f3 brdf,
+                                           f3 brdf_w, bool front, f3& rad_out, f3& radw_out,
+                                           f3& sdir_out, float& tfar_out, float& ap_out) {
   f3 rad = {0.f, 0.f, 0.f}, sdir;
   float tfar, ap = 1.0f, cos_n = 0.f;
   const int type = L.type;
@@ -167,18 +201,17 @@ __device__ __forceinline__ void light_eval(const SceneView& sc, int l, f3 P, f3 
         ap = 0.f;
         target = f3{0.f, 0.f, 0.f};
       } else {
-        // single-triangle emitters draw no triangle index
-        uint32_t tri = L.num_tris > 1 ? rng.index(L.num_tris) : 0u;
-        float u1 = rng.uniform();
-        float u2 = rng.uniform();
+        const uint32_t tri = dr.tri;
+        float u1 = dr.u1;
+        float u2 = dr.u2;
         if (u1 + u2 > 1.0f) {
           u1 = 1.0f - u1;
           u2 = 1.0f - u2;
         }
-        // single-triangle emitters (tri is 0) read a wave-uniform record
+        // single-triangle emitters (tri is 0) read one record per job
         float tu = 0.f, tv = 0.f;
         if (L.num_tris == 1)
-          area_point(uniform_load(sc.area_tris, L.first_tri), u1, u2, target, ap, tu, tv);
+          area_point(const_load(sc.area_tris, L.first_tri), u1, u2, target, ap, tu, tv);
         else
           area_point(sc.area_tris[L.first_tri + tri], u1, u2, target, ap, tu, tv);
         if (L.emit_const)
@@ -230,8 +263,8 @@ __device__ __forceinline__ void light_eval(const SceneView& sc, int l, f3 P, f3 
   }
   if (!UNIT) {
     const uint32_t l3 = 3u * (uint32_t)l;
-    rad = cmul(rad, f3{uniform_load(sc.light_scale, l3), uniform_load(sc.light_scale, l3 + 1),
-                       uniform_load(sc.light_scale, l3 + 2)});
+    rad = cmul(rad, f3{const_load(sc.light_scale, l3), const_load(sc.light_scale, l3 + 1),
+                       const_load(sc.light_scale, l3 + 2)});
   }
   rad_out = rad;
   radw_out = radw;
@@ -248,12 +281,9 @@ __device__ __forceinline__ bool nee_sample(const SceneView& sc, f3 P, f3 N, f3 b
   float pw = 0.f, pap = 1.f, ptfar = 0.f;
   f3 prad = {0.f, 0.f, 0.f}, pdir = {0.f, 0.f, 0.f};
   const int nl = sc.num_lights;
-  for (int l = 0; l < nl; ++l) {
-    f3 rad, radw, sdir;
-    float tfar, ap;
-    light_eval<VERTEX, UNIT>(sc, l, P, N, brdf, brdf_w, front, rng, rad, radw, sdir, tfar, ap);
+  auto take = [&](int l, const f3& rad, const f3& radw, const f3& sdir, float tfar, float ap,
+                  float u) {
     float w = maxc(radw);
-    float u = rng.uniform();
     W += w;
     if (w > 0.f && u * W < w) {
       pick = l;
@@ -263,6 +293,34 @@ __device__ __forceinline__ bool nee_sample(const SceneView& sc, f3 P, f3 N, f3 b
       pdir = sdir;
       ptfar = tfar;
     }
+  };
+  // two lights per iteration: both lights' numbers are drawn first, in the
+  // stream's order (draws of l, its pick uniform, draws of l + 1, its
+  // uniform), so their geometry is independent work the scheduler can
+  // interleave; the reservoir then takes l before l + 1
+  int l = 0;
+  for (; l + 1 < nl; l += 2) {
+    const LightDev La = const_load(sc.lights, (uint32_t)l);
+    const LightDev Lb = const_load(sc.lights, (uint32_t)(l + 1));
+    const LightDraws da = light_draws(La, rng);
+    const float ua = rng.uniform();
+    const LightDraws db = light_draws(Lb, rng);
+    const float ub = rng.uniform();
+    f3 rada, radwa, sdira, radb, radwb, sdirb;
+    float tfara, apa, tfarb, apb;
+    light_geom<VERTEX, UNIT>(sc, l, La, da, P, N, brdf, brdf_w, front, rada, radwa, sdira, tfara,
+                             apa);
+    light_geom<VERTEX, UNIT>(sc, l + 1, Lb, db, P, N, brdf, brdf_w, front, radb, radwb, sdirb,
+                             tfarb, apb);
+    take(l, rada, radwa, sdira, tfara, apa, ua);
+    take(l + 1, radb, radwb, sdirb, tfarb, apb, ub);
+  }
+  if (l < nl) {
+    f3 rad, radw, sdir;
+    float tfar, ap;
+    light_eval<VERTEX, UNIT>(sc, l, P, N, brdf, brdf_w, front, rng, rad, radw, sdir, tfar, ap);
+    const float u = rng.uniform();
+    take(l, rad, radw, sdir, tfar, ap, u);
   }
   if (W < 1e-6f || pick < 0) return false;
   float pdf = pw / W;

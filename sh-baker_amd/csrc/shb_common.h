@@ -197,17 +197,31 @@ SHB_HD Rng site_rng(uint64_t seed, uint32_t texel, uint32_t sample,
 // uniform_load(p, i): p[i] for a wave-uniform i from memory no kernel writes
 // while it runs (scene tables), read through the constant address space so
 // it can be a scalar load.
+// const_load(p, i): p[i] from such memory where p or i may differ between
+// lanes (a wave holding texels of several jobs): the compiler makes it a
+// scalar load only where it proves both uniform, a vector load elsewhere.
 #if defined(SHB_HOST_ONLY)
 SHB_HD uint32_t uniform_index(uint32_t v) { return v; }
 SHB_HD int uniform_index(int v) { return v; }
 template <class T>
 SHB_HD T uniform_load(const T* p, uint32_t i) { return p[i]; }
+template <class T>
+SHB_HD T const_load(const T* p, uint32_t i) { return p[i]; }
 #else
 template <class T>
 __device__ __forceinline__ T uniform_load(const T* p, uint32_t i) {
 #if defined(__HIP_DEVICE_COMPILE__)
   typedef const __attribute__((address_space(4))) T* cptr;
   return ((cptr)p)[__builtin_amdgcn_readfirstlane((int)i)];
+#else
+  return p[i];
+#endif
+}
+template <class T>
+__device__ __forceinline__ T const_load(const T* p, uint32_t i) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef const __attribute__((address_space(4))) T* cptr;
+  return ((cptr)p)[i];
 #else
   return p[i];
 #endif

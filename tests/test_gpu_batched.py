@@ -182,3 +182,59 @@ def test_batched_jobs_with_different_light_sample_counts():
     finally:
         for b in bakers:
             b.close()
+
+
+def emitter_floor(kinds, seed):
+    """A 4 x 4 m floor under lights 0.6 m above it: kinds[i] triangles for
+    area light i (a fan when more than one), 0 for a point light. Scenes
+    with other kinds differ in light count, in each light's type and in
+    where an area light's triangles start in the area-triangle table."""
+    rng = np.random.default_rng(seed)
+    b = scenes.SceneBuilder()
+    mf = b.add_material((200, 200, 200, 255))
+    b.add_quads([((-2.0, 0.0, -2.0), (0.0, 0.0, 4.0), (4.0, 0.0, 0.0))], mf, grid=[(4, 4)])
+    for k in kinds:
+        c = np.array([rng.uniform(-1.5, 1.5), 0.6, rng.uniform(-1.5, 1.5)])
+        if k == 0:
+            b.add_light(type=shb.LIGHT_POINT, position=tuple(c), intensity=2.0, color=(1, 1, 1))
+            continue
+        e = float(rng.uniform(1, 5))
+        em = b.add_material((255, 255, 255, 255), emission=e)
+        if k == 1:
+            a = rng.uniform(0, 2 * np.pi)
+            p = [c + 0.4 * np.array([np.cos(a + i * 2 * np.pi / 3), 0, np.sin(a + i * 2 * np.pi / 3)])
+                 for i in range(3)]
+            if np.cross(p[1] - p[0], p[2] - p[0])[1] > 0:  # face down
+                p = [p[0], p[2], p[1]]
+            g = b.add_triangle(p, em)
+        else:
+            g = b.add_emitter_fan(c, 0.4, k, em, rng)
+        b.add_light(type=shb.LIGHT_AREA, geometry_index=g, material_index=em, intensity=e)
+    return b.build(64, 64, 2)
+
+
+@pytest.mark.parametrize("nls", [(1, 1, 1), (1, 3, 2)])
+def test_batched_jobs_with_different_light_tables(nls):
+    """Depth >= 1 waves hold paths of several jobs whose light records differ
+    lane by lane (count, type, first area triangle): every job bit-identical
+    to its plain bake, with the area lights lighting the floor."""
+    # light counts 3, 5, 3 (the pair loop's odd tail at different lights);
+    # light 1 is a one-triangle area light starting at area triangle 4 in
+    # the second scene and 3 in the third
+    kinds = [[1, 3, 0], [4, 1, 1, 0, 1], [3, 1, 5]]
+    bakers = []
+    try:
+        for i, kd in enumerate(kinds):
+            b = shb.Baker(0)
+            b.upload_scene(emitter_floor(kd, 40 + i))
+            b.rasterize(64, 64, 1, want=False)
+            bakers.append(b)
+        ref = [b.bake(8, 2, seed=11, num_light_samples=n)[:2] for b, n in zip(bakers, nls)]
+        sh, vis, _ = shb.bake_batched(bakers, 8, 2, 11, num_light_samples=list(nls))
+        for k in range(len(kinds)):
+            assert (np.abs(ref[k][0]).max(axis=1) > 0).sum() > 500, k
+            assert np.array_equal(bits(sh[k]), bits(ref[k][0])), k
+            assert np.array_equal(bits(vis[k]), bits(ref[k][1])), k
+    finally:
+        for b in bakers:
+            b.close()
