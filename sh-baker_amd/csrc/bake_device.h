@@ -273,30 +273,109 @@ __device__ __forceinline__ void light_geom(const SceneView& sc, int l, const Lig
   ap_out = ap;
 }
 
+// Light L's selection weight maxc(radw) from its draws, as light_geom
+// computes it (the same operations in the same order) but without what only
+// the picked light needs: the shadow ray, the area pdf and the scaled
+// radiance. The per-channel products are reduced with two identities that
+// hold bit for bit because x -> fl(s x) never decreases for s >= 0: the
+// texel weight max_c fl(k_c s) is fl(max_c k_c s), and at a path vertex a
+// grey light's max_c fl(R fl(c b_c)) is fl(R fl(c max_c b_c)). bw is the
+// BRDF of the weight (brdf_w, or brdf with unit scales) and maxbw its
+// largest channel.
+template <bool VERTEX>
+__device__ __forceinline__ float light_weight(const SceneView& sc, const LightDev& L,
+                                              const LightDraws& dr, f3 P, f3 N, f3 bw,
+                                              float maxbw, bool front) {
+  const int type = L.type;
+  float cos_n;
+  f3 k3 = {0.f, 0.f, 0.f};  // the radiance channels (non-grey vertex weight)
+  float kmax;               // their largest (grey lights and texel weights)
+  if (type == kLightDirectional) {
+    const f3 ld = {L.dir[0], L.dir[1], L.dir[2]};
+    cos_n = dot(N, -ld);
+    if (cos_n < 0.f) return 0.f;
+    kmax = L.wmax;
+    if (VERTEX && !L.grey) k3 = f3{L.ic[0], L.ic[1], L.ic[2]};
+  } else {
+    f3 target;
+    f3 emit = {0.f, 0.f, 0.f};
+    float emax = L.wmax;
+    if (type == kLightArea) {
+      if (L.num_tris == 0) {
+        target = f3{0.f, 0.f, 0.f};
+      } else {
+        float u1 = dr.u1, u2 = dr.u2;
+        if (u1 + u2 > 1.0f) {
+          u1 = 1.0f - u1;
+          u2 = 1.0f - u2;
+        }
+        float ap, tu = 0.f, tv = 0.f;
+        if (L.num_tris == 1)
+          area_point(const_load(sc.area_tris, L.first_tri), u1, u2, target, ap, tu, tv);
+        else
+          area_point(sc.area_tris[L.first_tri + dr.tri], u1, u2, target, ap, tu, tv);
+        if (L.emit_const) {
+          emit = f3{L.emit[0], L.emit[1], L.emit[2]};
+        } else {
+          emit = get_emission(sc, L.material, tu, tv);
+          emax = maxc(emit);
+        }
+      }
+    } else {
+      target = f3{L.pos[0], L.pos[1], L.pos[2]};
+    }
+    f3 Lv = target - P;
+    const float dist_sq = sqnorm_fma(Lv);
+    const float dist = sqrtf(dist_sq);
+    const float inv = 1.0f / dist;
+    const float inv2 = inv * inv;
+    Lv = Lv * inv;
+    cos_n = dot_fma(N, Lv);
+    if (cos_n < 0.f) return 0.f;
+    if (type == kLightSpot) {
+      const f3 ld = {L.dir[0], L.dir[1], L.dir[2]};
+      const float cos_l = dot(ld, -Lv);
+      float falloff = (cos_l - L.cos_outer) / (L.cos_inner - L.cos_outer);
+      falloff = fminf(fmaxf(falloff, 0.0f), 1.0f);
+      kmax = emax * inv2 * falloff;
+      if (VERTEX && !L.grey) k3 = f3{L.ic[0], L.ic[1], L.ic[2]} * inv2 * falloff;
+    } else if (type == kLightPoint) {
+      kmax = emax * inv2;
+      if (VERTEX && !L.grey) k3 = f3{L.ic[0], L.ic[1], L.ic[2]} * inv2;
+    } else {
+      kmax = emax * inv2;
+      if (VERTEX && !L.grey) k3 = emit * inv2;
+    }
+  }
+  if (!VERTEX) return kmax;
+  // BRDF x cosine, the BRDF zero unless the vertex faces the light
+  const bool ok = front && cos_n > 0.0f;
+  if (L.grey) return kmax * (cos_n * (ok ? maxbw : 0.f));
+  return maxc(cmul(k3, cos_n * (ok ? bw : f3{0.f, 0.f, 0.f})));
+}
+
 template <bool VERTEX, bool UNIT>
 __device__ __forceinline__ bool nee_sample(const SceneView& sc, f3 P, f3 N, f3 brdf, f3 brdf_w,
                                            bool front, Rng& rng, NeeResult& out) {
-  float W = 0.f;
+  const f3 bw = UNIT ? brdf : brdf_w;
+  const float maxbw = maxc(bw);
+  float W = 0.f, pw = 0.f;
   int pick = -1;
-  float pw = 0.f, pap = 1.f, ptfar = 0.f;
-  f3 prad = {0.f, 0.f, 0.f}, pdir = {0.f, 0.f, 0.f};
-  const int nl = sc.num_lights;
-  auto take = [&](int l, const f3& rad, const f3& radw, const f3& sdir, float tfar, float ap,
-                  float u) {
-    float w = maxc(radw);
+  LightDraws pd{0u, 0.f, 0.f};
+  // the reservoir keeps the picked light's index, weight and draws; its
+  // radiance, shadow ray and pdf are evaluated once after the loop
+  auto take = [&](int l, const LightDraws& d, float w, float u) {
     W += w;
     if (w > 0.f && u * W < w) {
       pick = l;
       pw = w;
-      pap = ap;
-      prad = rad;
-      pdir = sdir;
-      ptfar = tfar;
+      pd = d;
     }
   };
+  const int nl = sc.num_lights;
   // two lights per iteration: both lights' numbers are drawn first, in the
   // stream's order (draws of l, its pick uniform, draws of l + 1, its
-  // uniform), so their geometry is independent work the scheduler can
+  // uniform), so their weights are independent work the scheduler can
   // interleave; the reservoir then takes l before l + 1
   int l = 0;
   for (; l + 1 < nl; l += 2) {
@@ -306,30 +385,29 @@ __device__ __forceinline__ bool nee_sample(const SceneView& sc, f3 P, f3 N, f3 b
     const float ua = rng.uniform();
     const LightDraws db = light_draws(Lb, rng);
     const float ub = rng.uniform();
-    f3 rada, radwa, sdira, radb, radwb, sdirb;
-    float tfara, apa, tfarb, apb;
-    light_geom<VERTEX, UNIT>(sc, l, La, da, P, N, brdf, brdf_w, front, rada, radwa, sdira, tfara,
-                             apa);
-    light_geom<VERTEX, UNIT>(sc, l + 1, Lb, db, P, N, brdf, brdf_w, front, radb, radwb, sdirb,
-                             tfarb, apb);
-    take(l, rada, radwa, sdira, tfara, apa, ua);
-    take(l + 1, radb, radwb, sdirb, tfarb, apb, ub);
+    const float wa = light_weight<VERTEX>(sc, La, da, P, N, bw, maxbw, front);
+    const float wb = light_weight<VERTEX>(sc, Lb, db, P, N, bw, maxbw, front);
+    take(l, da, wa, ua);
+    take(l + 1, db, wb, ub);
   }
   if (l < nl) {
-    f3 rad, radw, sdir;
-    float tfar, ap;
-    light_eval<VERTEX, UNIT>(sc, l, P, N, brdf, brdf_w, front, rng, rad, radw, sdir, tfar, ap);
+    const LightDev L = const_load(sc.lights, (uint32_t)l);
+    const LightDraws d = light_draws(L, rng);
     const float u = rng.uniform();
-    take(l, rad, radw, sdir, tfar, ap, u);
+    take(l, d, light_weight<VERTEX>(sc, L, d, P, N, bw, maxbw, front), u);
   }
   if (W < 1e-6f || pick < 0) return false;
+  f3 rad, radw, sdir;
+  float tfar, ap;
+  light_geom<VERTEX, UNIT>(sc, pick, sc.lights[pick], pd, P, N, brdf, brdf_w, front, rad, radw,
+                           sdir, tfar, ap);
   float pdf = pw / W;
-  float joint = pdf * pap;
+  float joint = pdf * ap;
   if (joint < 1e-3f) return false;
-  out.L = prad / joint;
-  out.dir = pdir;
+  out.L = rad / joint;
+  out.dir = sdir;
   out.origin = P;
-  out.tfar = ptfar;
+  out.tfar = tfar;
   out.light = pick;
   return true;
 }
@@ -349,12 +427,13 @@ __device__ __forceinline__ void nee_multi(const SceneView& sc, f3 P, f3 N, f3 br
                                           bool front, Rng& rng, int n, F&& pick) {
   const Rng start = rng;
   const int nl = sc.num_lights;
+  const f3 bw = UNIT ? brdf : brdf_w;
+  const float maxbw = maxc(bw);
   float W = 0.f;
   for (int l = 0; l < nl; ++l) {
-    f3 rad, radw, sdir;
-    float tfar, ap;
-    light_eval<VERTEX, UNIT>(sc, l, P, N, brdf, brdf_w, front, rng, rad, radw, sdir, tfar, ap);
-    W += maxc(radw);
+    const LightDev L = const_load(sc.lights, (uint32_t)l);
+    const LightDraws d = light_draws(L, rng);
+    W += light_weight<VERTEX>(sc, L, d, P, N, bw, maxbw, front);
   }
   float target[kMaxLightSamples];
 #pragma unroll

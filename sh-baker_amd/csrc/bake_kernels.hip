@@ -238,12 +238,15 @@ __device__ __forceinline__ bool wave_tris(const SceneView& sc, const TravRay& r,
   const uint32_t lane = lane_id();
   const uint32_t k = active ? (uint32_t)__builtin_popcount(tmask) : 0u;
   if (COUNT) s.tris += k;
-  const uint64_t below = (1ull << lane) - 1ull;
+  // off = the sum of k over the lanes below this one, K over the wave: bit
+  // b of every lane's k in one ballot, counted below the lane by mbcnt
   uint32_t off = 0, K = 0;
 #pragma unroll
-  for (int b = 0; b < 6; ++b) {  // k <= 32
+  for (int b = 0; b < 5; ++b) {  // k <= 24: 8 leaf slots of at most 3 triangles
     const uint64_t m = __ballot(((k >> b) & 1u) != 0u);
-    off += (uint32_t)__popcll(m & below) << b;
+    const uint32_t under = __builtin_amdgcn_mbcnt_hi(
+        (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+    off += under << b;
     K += (uint32_t)__popcll(m) << b;
   }
   if (K == 0u) return false;
@@ -253,11 +256,14 @@ __device__ __forceinline__ bool wave_tris(const SceneView& sc, const TravRay& r,
     return f;
   }
   const unsigned long long init = hit_key(s.tbest, s.best);
+  // pair = owner lane << 26 | triangle (tri_base + j < 2^26 when the wave
+  // form is on)
+  const uint32_t pbase = (lane << 26) + tri_base;
   uint32_t tm = k ? tmask : 0u, q = off;
   while (tm) {
     const uint32_t j = __builtin_ctz(tm);
     tm &= tm - 1u;
-    L->pair[q++] = (lane << 26) | (tri_base + j);
+    L->pair[q++] = pbase + j;
   }
   if (k) L->key[lane] = init;
   wave_sync_lds();

@@ -268,6 +268,12 @@ int build_host_scene(const shb_scene_desc* d, HostScene* out, std::string* err) 
         for (uint32_t t = 0; t < nt; ++t) hs.tri_info[geom_first_tri[gi] + t].y = (int)li;
       }
     }
+    // the channels the selection weight scales: intensity x colour, or a
+    // constant emitter's emission (a textured emitter's vary per sample)
+    const float* wc = (sl.type == SHB_LIGHT_AREA) ? L.emit : L.ic;
+    const bool per_sample = sl.type == SHB_LIGHT_AREA && !L.emit_const;
+    L.wmax = per_sample ? 0.f : std::max(wc[0], std::max(wc[1], wc[2]));
+    L.grey = !per_sample && wc[0] == wc[1] && wc[1] == wc[2] ? 1 : 0;
     hs.lights.push_back(L);
   }
   *out = std::move(hs);

@@ -85,7 +85,12 @@ struct alignas(16) LightDev {
   uint32_t first_tri;  // area lights: first record in area_tris
   uint32_t num_tris;   // area lights: triangle count of the geometry (0: off)
   int32_t material;    // area lights: emitting material
-  int32_t pad;
+  // selection-weight helpers: the largest channel of ic (of emit for a
+  // constant emitter), and 1 when every channel of it is equal (a grey
+  // light: its vertex weight needs only the largest BRDF channel)
+  float wmax;
+  int32_t grey;
+  int32_t pad[2];
 };
 
 // Area-light triangle record (object-space vertices, light.cpp:45-49): v0
