@@ -224,13 +224,13 @@ __device__ __forceinline__ void light_geom(const SceneView& sc, int l, const Lig
     }
     f3 Lv = target - P;
     float dist_sq = sqnorm_fma(Lv);
-    float dist = sqrtf(dist_sq);
+    float dist, inv;
+    inv_dist(dist_sq, dist, inv);
     sdir = target - P;
     tfar = dist - 0.001f;
     // one division per light (the reference divides per component:
     // light.h:82-93, 127-143, 176-188) and fused sums; the oracle's
     // GPU-stream mode follows this arithmetic
-    const float inv = 1.0f / dist;
     const float inv2 = inv * inv;
     Lv = Lv * inv;
     cos_n = dot_fma(N, Lv);
@@ -326,8 +326,8 @@ __device__ __forceinline__ float light_weight(const SceneView& sc, const LightDe
     }
     f3 Lv = target - P;
     const float dist_sq = sqnorm_fma(Lv);
-    const float dist = sqrtf(dist_sq);
-    const float inv = 1.0f / dist;
+    float dist, inv;
+    inv_dist(dist_sq, dist, inv);
     const float inv2 = inv * inv;
     Lv = Lv * inv;
     cos_n = dot_fma(N, Lv);

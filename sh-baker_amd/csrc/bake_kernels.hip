@@ -479,7 +479,7 @@ __global__ void k_hit_details(SceneView sc, const float4* rays, const float4* hi
 // ------------------------------------------------------------ bake stages
 // Primary ray + texel NEE for every path slot of the batch (UNIT: every
 // light and albedo scale is 1).
-template <bool UNIT, bool MULTI>
+template <bool UNIT, bool MULTI, bool ONE>
 __global__ void __launch_bounds__(256, 5) k_generate(BakeArgs a) {
   const uint32_t n = a.P;
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
@@ -487,8 +487,8 @@ __global__ void __launch_bounds__(256, 5) k_generate(BakeArgs a) {
     uint32_t shadow = 0u;
     if (p < n) {
       const SlotRef sr = slot_ref(a, p);
-      with_job(a, sr.job,
-               [&](const JobDev& J) { shadow = generate_path<UNIT, MULTI>(a, J, sr, p); });
+      with_job<ONE>(a, sr.job,
+                    [&](const JobDev& J) { shadow = generate_path<UNIT, MULTI>(a, J, sr, p); });
     }
     if (MULTI) {
       for (uint32_t j = 0; j < a.nls; ++j) {  // wave-uniform bound
@@ -505,11 +505,15 @@ __global__ void __launch_bounds__(256, 5) k_generate(BakeArgs a) {
 
 // Hit shading at depth d (ADJ: the replay form of the adjoint, with the
 // block's gradient sums in dynamic LDS; SPLIT: batches with spare slots for
-// alpha splits; UNIT: every light and albedo scale is 1).
-// Every form runs at 5 waves per SIMD: a register cap with a small spill
-// (24-76 bytes per lane by form) is faster than 4 spill-free waves, since
-// the light loop waits on a scalar load per light.
-template <bool ADJ, bool SPLIT, bool UNIT, bool MULTI>
+// alpha splits; UNIT: every light and albedo scale is 1; ONE: the batch is
+// one job's segment, so the shading code is compiled once, for a
+// wave-uniform job record).
+// Every form runs at 5 waves per SIMD, since the light loop waits on a
+// scalar load per light: the one-job forms fit in 75-91 registers; the
+// multi-job forms, with a second copy of the shading code for waves that
+// hold several jobs, spill a little under that cap, which is still faster
+// than 4 spill-free waves.
+template <bool ADJ, bool SPLIT, bool UNIT, bool MULTI, bool ONE>
 __global__ void __launch_bounds__(256, 5) k_shade(BakeArgs a, int depth, const uint32_t* in_queue,
                                                   const uint32_t* in_count, uint32_t* out_queue,
                                                   uint32_t* out_count) {
@@ -526,7 +530,7 @@ __global__ void __launch_bounds__(256, 5) k_shade(BakeArgs a, int depth, const u
     if (i < n) {
       p = in_queue ? in_queue[i] : i;
       const SlotRef sr = slot_ref(a, SPLIT ? root_slot(a, p) : p);
-      with_job(a, sr.job, [&](const JobDev& J) {
+      with_job<ONE>(a, sr.job, [&](const JobDev& J) {
         shade_path<ADJ, SPLIT, UNIT, MULTI>(a, J, sr, depth, p, shadow, extend, child, &acc);
       });
     }
@@ -623,7 +627,7 @@ __global__ void k_project(BakeArgs a) {
     if (ADJ)
       project_texel_adjoint(a, a.jobs[job], rank, slot0, stride, &acc);
     else
-      with_job(a, job, [&](const JobDev& J) { project_texel(a, J, rank, slot0, stride); });
+      with_job<false>(a, job, [&](const JobDev& J) { project_texel(a, J, rank, slot0, stride); });
   }
   if (ADJ) acc.flush();
 }
@@ -743,18 +747,19 @@ hipError_t launch_hit_details(const SceneView& sc, const float4* rays, const flo
 }
 
 hipError_t launch_generate(const BakeArgs& b, int grid, hipStream_t st) {
-  const bool multi = b.nls > 1;
-  if (b.unit_scales) {
-    if (multi)
-      hipLaunchKernelGGL((k_generate<true, true>), dim3(grid), dim3(256), 0, st, b);
-    else
-      hipLaunchKernelGGL((k_generate<true, false>), dim3(grid), dim3(256), 0, st, b);
-  } else {
-    if (multi)
-      hipLaunchKernelGGL((k_generate<false, true>), dim3(grid), dim3(256), 0, st, b);
-    else
-      hipLaunchKernelGGL((k_generate<false, false>), dim3(grid), dim3(256), 0, st, b);
+#define SHB_GEN(U, M, O) hipLaunchKernelGGL((k_generate<U, M, O>), dim3(grid), dim3(256), 0, st, b)
+  const int v = (b.unit_scales ? 4 : 0) | (b.nls > 1 ? 2 : 0) | (b.nseg <= 1 ? 1 : 0);
+  switch (v) {
+    case 0: SHB_GEN(false, false, false); break;
+    case 1: SHB_GEN(false, false, true); break;
+    case 2: SHB_GEN(false, true, false); break;
+    case 3: SHB_GEN(false, true, true); break;
+    case 4: SHB_GEN(true, false, false); break;
+    case 5: SHB_GEN(true, false, true); break;
+    case 6: SHB_GEN(true, true, false); break;
+    default: SHB_GEN(true, true, true); break;
   }
+#undef SHB_GEN
   return hipGetLastError();
 }
 
@@ -767,29 +772,39 @@ hipError_t launch_shade(const BakeArgs& b, int depth, const uint32_t* in_q,
                         const uint32_t* in_n, uint32_t* out_q, uint32_t* out_n, int grid,
                         hipStream_t st) {
   const size_t lds = b.adj_grad ? adj_lds(b) : 0;
-#define SHB_SHADE(A, S, U, M)                                                               \
-  hipLaunchKernelGGL((k_shade<A, S, U, M>), dim3(grid), dim3(256), lds, st, b, depth, in_q,    \
+#define SHB_SHADE(A, S, U, M, O)                                                            \
+  hipLaunchKernelGGL((k_shade<A, S, U, M, O>), dim3(grid), dim3(256), lds, st, b, depth, in_q, \
                      in_n, out_q, out_n)
-  const int v = (b.adj_grad ? 4 : 0) | (b.S ? 2 : 0) | (b.unit_scales ? 1 : 0);
-  if (b.nls > 1) {  // several light samples: the bake forms only (no adjoint)
+  // the adjoint runs one job (batches of one segment)
+  if (b.adj_grad && b.nseg > 1) return hipErrorInvalidValue;
+  const int v = (b.S ? 2 : 0) | (b.unit_scales ? 1 : 0);
+  if (b.adj_grad) {
     switch (v) {
-      case 0: SHB_SHADE(false, false, false, true); break;
-      case 1: SHB_SHADE(false, false, true, true); break;
-      case 2: SHB_SHADE(false, true, false, true); break;
-      case 3: SHB_SHADE(false, true, true, true); break;
-      default: return hipErrorInvalidValue;
+      case 0: SHB_SHADE(true, false, false, false, true); break;
+      case 1: SHB_SHADE(true, false, true, false, true); break;
+      case 2: SHB_SHADE(true, true, false, false, true); break;
+      default: SHB_SHADE(true, true, true, false, true); break;
     }
     return hipGetLastError();
   }
-  switch (v) {
-    case 0: SHB_SHADE(false, false, false, false); break;
-    case 1: SHB_SHADE(false, false, true, false); break;
-    case 2: SHB_SHADE(false, true, false, false); break;
-    case 3: SHB_SHADE(false, true, true, false); break;
-    case 4: SHB_SHADE(true, false, false, false); break;
-    case 5: SHB_SHADE(true, false, true, false); break;
-    case 6: SHB_SHADE(true, true, false, false); break;
-    default: SHB_SHADE(true, true, true, false); break;
+  const int w = v | (b.nls > 1 ? 4 : 0) | (b.nseg <= 1 ? 8 : 0);
+  switch (w) {
+    case 0: SHB_SHADE(false, false, false, false, false); break;
+    case 1: SHB_SHADE(false, false, true, false, false); break;
+    case 2: SHB_SHADE(false, true, false, false, false); break;
+    case 3: SHB_SHADE(false, true, true, false, false); break;
+    case 4: SHB_SHADE(false, false, false, true, false); break;
+    case 5: SHB_SHADE(false, false, true, true, false); break;
+    case 6: SHB_SHADE(false, true, false, true, false); break;
+    case 7: SHB_SHADE(false, true, true, true, false); break;
+    case 8: SHB_SHADE(false, false, false, false, true); break;
+    case 9: SHB_SHADE(false, false, true, false, true); break;
+    case 10: SHB_SHADE(false, true, false, false, true); break;
+    case 11: SHB_SHADE(false, true, true, false, true); break;
+    case 12: SHB_SHADE(false, false, false, true, true); break;
+    case 13: SHB_SHADE(false, false, true, true, true); break;
+    case 14: SHB_SHADE(false, true, false, true, true); break;
+    default: SHB_SHADE(false, true, true, true, true); break;
   }
 #undef SHB_SHADE
   return hipGetLastError();

@@ -47,11 +47,18 @@ __device__ __forceinline__ uint32_t root_slot(const BakeArgs& a, uint32_t p) {
 
 // Calls f(job record) with the record read wave-uniformly when all active
 // lanes share `job` (the common case: segments are large and contiguous).
-template <class F>
+// ONE: the batch is one segment, so every slot is job0's and f is compiled
+// once, with the job's tables and light records in scalar registers.
+template <bool ONE, class F>
 __device__ __forceinline__ void with_job(const BakeArgs& a, uint32_t job, F&& f) {
 #if defined(SHB_HOST_ONLY)
   f(a.jobs[job]);
 #else
+  if (ONE) {
+    const JobDev J = uniform_load(a.jobs, a.job0);
+    f(J);
+    return;
+  }
   const uint32_t j0 = uniform_index(job);
   if (__ballot(job != j0) == 0ull) {
     const JobDev J = uniform_load(a.jobs, j0);

@@ -288,6 +288,43 @@ SHB_HD uint32_t as_uint(float f) {
   return x.u;
 }
 
+// 1 / sqrt(d2) as fl(1 / fl(sqrt(d2))) and the distance fl(sqrt(d2)), both
+// correctly rounded. For d2 in [2^-96, 2^126] (distances 2^-48 .. 2^63) the
+// hardware estimates (within one ulp) are corrected exactly as the general
+// IEEE sequences correct them, without those sequences' range scaling and
+// special-value steps: the same bits as sqrtf and 1.0f / x there (checked for
+// every float of the range on the GPU); other d2 take sqrtf and the division.
+#if defined(SHB_HOST_ONLY)
+SHB_HD void inv_dist(float d2, float& dist, float& inv) {
+  dist = std::sqrt(d2);
+  inv = 1.0f / dist;
+}
+#else
+__device__ __forceinline__ float sqrt_core(float x) {
+  const float s = __builtin_amdgcn_sqrtf(x);
+  const float sm = as_float(as_uint(s) - 1u);
+  const float sp = as_float(as_uint(s) + 1u);
+  const float t = fmaf(-sm, s, x) <= 0.f ? sm : s;
+  return fmaf(-sp, s, x) > 0.f ? sp : t;
+}
+__device__ __forceinline__ float rcp_core(float d) {
+  const float y0 = __builtin_amdgcn_rcpf(d);
+  const float y1 = fmaf(fmaf(-d, y0, 1.0f), y0, y0);
+  const float q1 = fmaf(fmaf(-d, y1, 1.0f), y1, y1);
+  return fmaf(fmaf(-d, q1, 1.0f), y1, q1);
+}
+constexpr float kCoreLo = 0x1p-96f, kCoreHi = 0x1p126f;
+__device__ __forceinline__ void inv_dist(float d2, float& dist, float& inv) {
+  if (d2 >= kCoreLo && d2 <= kCoreHi) {
+    dist = sqrt_core(d2);
+    inv = rcp_core(dist);
+  } else {
+    dist = sqrtf(d2);
+    inv = 1.0f / dist;
+  }
+}
+#endif
+
 }  // namespace shb
 
 #endif  // SHB_COMMON_H_
