@@ -285,8 +285,6 @@ __device__ __forceinline__ bool wave_tris(const SceneView& sc, const TravRay& r,
     ro.ky = (kk >> 2) & 3;
     ro.kz = (kk >> 4) & 3;
     const float tnear_o = __shfl(r.tnear, o);
-    const float tbest_o = __shfl(s.tbest, o);
-    const int best_o = __shfl(s.best, o);
     unsigned long long key = ~0ull;
     float t = 0.f, u = 0.f, v = 0.f;
     if (have) {
@@ -294,7 +292,9 @@ __device__ __forceinline__ bool wave_tris(const SceneView& sc, const TravRay& r,
       const float4 a = tp[0], b = tp[1], c = tp[2];
       if (tri_test(ro, a, b, c, t, u, v)) {
         const int id = (int)as_uint(a.w);
-        if (better_hit(t, id, tnear_o, tbest_o, best_o)) {
+        // the owner's key starts at its best hit so far, so the minimum
+        // over the keys keeps exactly the candidates better_hit accepts
+        if (t >= tnear_o) {
           key = hit_key(t, id);
           atomicMin(&L->key[o], key);
         }

@@ -199,22 +199,22 @@ __device__ __forceinline__ bool trav_node(const SceneView& sc, const TravRay& r,
   if (COUNT) ++s.nodes;
   float px = as_float(w0.x), py = as_float(w0.y), pz = as_float(w0.z);
   uint32_t ew = w0.w;
-  float sx = as_float((uint32_t)((int)(int8_t)(ew & 0xffu) + 127) << 23);
-  float sy = as_float((uint32_t)((int)(int8_t)((ew >> 8) & 0xffu) + 127) << 23);
-  float sz = as_float((uint32_t)((int)(int8_t)((ew >> 16) & 0xffu) + 127) << 23);
   uint32_t imask = ew >> 24;
-  float ax = sx * r.ix, bx = (px - r.ox) * r.ix;
-  float ay = sy * r.iy, by = (py - r.oy) * r.iy;
-  float az = sz * r.iz, bz = (pz - r.oz) * r.iz;
+  // a = 2^e / d: the exact product of the node's power-of-two scale with 1/d
+  float ax = ldexpf(r.ix, (int)(int8_t)(ew & 0xffu)), bx = (px - r.ox) * r.ix;
+  float ay = ldexpf(r.iy, (int)(int8_t)((ew >> 8) & 0xffu)), by = (py - r.oy) * r.iy;
+  float az = ldexpf(r.iz, (int)(int8_t)((ew >> 16) & 0xffu)), bz = (pz - r.oz) * r.iz;
   // Per-axis absolute bound on the rounding of fmaf(q, a, b) against the
   // exact plane distance (p + q s - o) / d: 16 ulps of (|b| + 256 |a|).
   // Near planes move down and far planes up by it, so no box holding a hit
   // in [tnear, tbest] is culled and closest-hit ties resolve as in a
   // brute-force search. Axes the ray is parallel to (d = 0, 1/d replaced by
-  // +-1e20) keep an exact sign test.
-  float ex = r.zx ? 0.f : (fabsf(bx) + 256.0f * fabsf(ax)) * 0x1p-20f;
-  float ey = r.zy ? 0.f : (fabsf(by) + 256.0f * fabsf(ay)) * 0x1p-20f;
-  float ez = r.zz ? 0.f : (fabsf(bz) + 256.0f * fabsf(az)) * 0x1p-20f;
+  // +-1e20) keep an exact sign test. Computed as |a| 2^-12 + |b| 2^-20 in
+  // one fma: the power-of-two scalings are exact, so this is the same float
+  // as (|b| + 256 |a|) 2^-20.
+  float ex = r.zx ? 0.f : fmaf(fabsf(ax), 0x1p-12f, fabsf(bx) * 0x1p-20f);
+  float ey = r.zy ? 0.f : fmaf(fabsf(ay), 0x1p-12f, fabsf(by) * 0x1p-20f);
+  float ez = r.zz ? 0.f : fmaf(fabsf(az), 0x1p-12f, fabsf(bz) * 0x1p-20f);
   const float bxn = bx - ex, bxf = bx + ex;
   const float byn = by - ey, byf = by + ey;
   const float bzn = bz - ez, bzf = bz + ez;
@@ -242,8 +242,10 @@ __device__ __forceinline__ bool trav_node(const SceneView& sc, const TravRay& r,
     float tfz = fmaf((float)((fzw >> sh) & 0xffu), az, bzf);
     float tmin = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, r.tnear));
     float tmax = fminf(fminf(fminf(tfx, tfy), tfz), tmax_ray);
+    // an empty slot (meta 0) adds nothing even when its box is taken: it
+    // is neither internal nor a leaf with triangles
     uint32_t m = (mw >> sh) & 0xffu;
-    if (m != 0u && tmin <= tmax) {
+    if (tmin <= tmax) {
       if (m & 0x80u)
         hits |= 1u << (c ^ r.sstar);
       else
