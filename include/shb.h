@@ -227,7 +227,9 @@ const char* shb_last_error(const shb_ctx* ctx);
 const char* shb_version(void);
 
 /* Uploads the scene and builds the BVH on the host (binned SAH, collapsed to
- * the compressed 8-wide node layout).
+ * the compressed 8-wide node layout). SHB_ERR_LIMIT when the BVH would be
+ * deeper than the traversal stack or hold 2^24 nodes or more (about 160 M
+ * triangles); a batched bake's scenes hold fewer than 2^24 nodes together.
  * Replaces BuildBVH / ReleaseBVH (src/scene.cpp:209-264, src/scene.h:129-132)
  * and the scene arguments of BakeSHLightMap (src/baker.h:25-28). */
 int shb_upload_scene(shb_ctx* ctx, const shb_scene_desc* scene);

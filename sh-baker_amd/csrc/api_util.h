@@ -58,7 +58,7 @@ inline TraceArgs trace_args(Ctx& c) {
   a.P = 0xffffffffu;
   a.Pc = 0xffffffffu;
   a.link = nullptr;
-  a.wave_tris = c.bvh.tris.size() / 3 < (1u << 26) ? 1u : 0u;
+  a.wave_tris = c.bvh.tris.size() / 3 < kWaveTrisMaxRecords ? 1u : 0u;
   return a;
 }
 

@@ -288,7 +288,9 @@ __device__ __forceinline__ bool wave_tris(const SceneView& sc, const TravRay& r,
     unsigned long long key = ~0ull;
     float t = 0.f, u = 0.f, v = 0.f;
     if (have) {
-      const float4* tp = sc.tris + 3 * (size_t)(pr & 0x3ffffffu);
+      // fewer than 2^24 triangle records when the wave form is on
+      const float4* tp = reinterpret_cast<const float4*>(
+          reinterpret_cast<const char*>(sc.tris) + __umul24(pr & 0x3ffffffu, 48u));
       const float4 a = tp[0], b = tp[1], c = tp[2];
       if (tri_test(ro, a, b, c, t, u, v)) {
         const int id = (int)as_uint(a.w);

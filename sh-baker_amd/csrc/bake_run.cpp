@@ -151,6 +151,11 @@ int run_pass(Ctx& x, std::vector<JobIn>& jobs, bool graph, BakeRun* run, const A
     total_tris += c->bvh.tris.size() / 3;
     depth = std::max(depth, c->bvh.max_depth);
   }
+  if (total_nodes >= kMaxTreeNodes)
+    return fail(x, SHB_ERR_LIMIT,
+                "the batch's scenes hold " + std::to_string(total_nodes) +
+                    " BVH nodes together: at most " + std::to_string(kMaxTreeNodes - 1) +
+                    "; bake fewer scenes per batch");
   if (nu == 1) {
     tnodes = scenes_u[0]->d_nodes.as<CWNode>();
     ttris = scenes_u[0]->d_tris.as<float4>();
@@ -279,7 +284,7 @@ int run_pass(Ctx& x, std::vector<JobIn>& jobs, bool graph, BakeRun* run, const A
   tbase.scene.tris = ttris;
   tbase.scene.has_nodes = total_nodes ? 1 : 0;
   tbase.has_nodes = total_nodes ? 1 : 0;
-  tbase.wave_tris = total_tris < (1u << 26) ? 1u : 0u;
+  tbase.wave_tris = total_tris < kWaveTrisMaxRecords ? 1u : 0u;
   tbase.max_steps = (uint32_t)total_nodes + 1;
   tbase.stack_entries = (uint32_t)std::max(1, std::min(depth - 1, kStackLds));
   tbase.refill = (uint32_t)x.tuning.queue_chunk;

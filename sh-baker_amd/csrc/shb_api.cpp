@@ -33,6 +33,10 @@ int upload_bvh(Ctx& c) {
                 "BVH depth " + std::to_string(c.bvh.max_depth) + " exceeds the traversal stack (" +
                     std::to_string(kMaxBvhDepth) + " levels) even with median splits: use a "
                     "wider BVH (tuning.bvh_width)");
+  if (c.bvh.nodes.size() >= kMaxTreeNodes)
+    return fail(c, SHB_ERR_LIMIT,
+                "BVH of " + std::to_string(c.bvh.nodes.size()) + " nodes: at most " +
+                    std::to_string(kMaxTreeNodes - 1) + " (about 160 M triangles)");
   SHB_TRY(c, upload(c.d_nodes, c.bvh.nodes, c.stream));
   SHB_TRY(c, upload(c.d_tris, c.bvh.tris, c.stream));
   return SHB_OK;

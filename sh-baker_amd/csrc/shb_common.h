@@ -43,6 +43,11 @@ constexpr int kMaxBounces = 15;       // depth fits 4 bits of the RNG counter
 constexpr int kMaxSamples = 65535;    // sample fits 16 bits of the counter
 constexpr int kMaxLightSamples = 16;  // light samples per NEE (a pick bit mask)
 constexpr int kStackLds = 16;         // traversal stack entries (LDS)
+// nodes of one traversed tree (a scene's, or the joined trees of a batch):
+// node addresses are formed with 24-bit multiplies; above 2^24 triangle
+// records the wave-spread triangle tests are off (same reason)
+constexpr uint32_t kMaxTreeNodes = 1u << 24;
+constexpr uint32_t kWaveTrisMaxRecords = 1u << 24;
 constexpr float kRayEps = 0.001f;     // baker.cpp:118, 122, light.h:40,83
 
 // ------------------------------------------------------------ BVH node

@@ -194,7 +194,10 @@ __device__ __forceinline__ bool trav_node(const SceneView& sc, const TravRay& r,
     ++s.sp;
   }
   // ---- decode node ni
-  const uint4* np = reinterpret_cast<const uint4*>(sc.nodes + ni);
+  // a 32-bit byte offset from one 24-bit multiply: trees hold fewer than
+  // 2^24 nodes (kMaxTreeNodes, checked where trees are uploaded and joined)
+  const uint4* np = reinterpret_cast<const uint4*>(reinterpret_cast<const char*>(sc.nodes) +
+                                                   __umul24(ni, 80u));
   uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3], w4 = np[4];
   if (COUNT) ++s.nodes;
   float px = as_float(w0.x), py = as_float(w0.y), pz = as_float(w0.z);
@@ -246,10 +249,11 @@ __device__ __forceinline__ bool trav_node(const SceneView& sc, const TravRay& r,
     // is neither internal nor a leaf with triangles
     uint32_t m = (mw >> sh) & 0xffu;
     if (tmin <= tmax) {
-      if (m & 0x80u)
-        hits |= 1u << (c ^ r.sstar);
-      else
-        tmask |= ((1u << (m >> 5)) - 1u) << (m & 31u);
+      // internal children (meta 0x80 | rank) set their octant bit; leaves
+      // (count << 5 | offset, count 1..3) their triangles; the leaf count
+      // field of an internal child, (0x80 >> 5) & 3, is 0
+      hits |= (m >> 7) << (c ^ r.sstar);
+      tmask |= ((1u << ((m >> 5) & 3u)) - 1u) << (m & 31u);
     }
   }
   s.gbase = w1.x;
