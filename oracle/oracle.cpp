@@ -1706,7 +1706,11 @@ int orc_bake(void* sp, const shb_surface_point* pts, int width, int height,
   auto worker = [&]() {
     Counters c;
     uint64_t nvalid = 0;
-    std::vector<float> sh(27), sq(27), nee1(27), ind1(27);
+    std::vector<float> sh(27), nee1(27), ind1(27);
+    // per-sample sums for the variance, in double apart from the estimate
+    // itself (a float sum of squares loses the variance of nearly constant
+    // samples to cancellation)
+    std::vector<double> s1(27), s2(27);
     for (;;) {
       size_t begin = next.fetch_add(256);
       if (begin >= total) break;
@@ -1721,7 +1725,8 @@ int orc_bake(void* sp, const shb_surface_point* pts, int width, int height,
         if ((int)(idx / width) % row_stride != 0) continue;
         ++nvalid;
         std::fill(sh.begin(), sh.end(), 0.f);
-        std::fill(sq.begin(), sq.end(), 0.f);
+        std::fill(s1.begin(), s1.end(), 0.0);
+        std::fill(s2.begin(), s2.end(), 0.0);
         float vis = 0.f;
         V3 P{p.position[0], p.position[1], p.position[2]};
         V3 N{p.normal[0], p.normal[1], p.normal[2]};
@@ -1763,8 +1768,9 @@ int orc_bake(void* sp, const shb_surface_point* pts, int width, int height,
           for (int k = 0; k < 27; ++k) {
             sh[k] += nee1[k];
             sh[k] += ind1[k];
-            float one = nee1[k] + ind1[k];
-            sq[k] += one * one;
+            const double one = (double)nee1[k] + (double)ind1[k];
+            s1[k] += one;
+            s2[k] += one * one;
           }
         }
         float inv = 1.0f / samples;  // baker.cpp:292-294
@@ -1772,8 +1778,8 @@ int orc_bake(void* sp, const shb_surface_point* pts, int width, int height,
         if (vis_out) vis_out[idx] = vis * inv;
         if (var_out && samples > 1) {
           for (int k = 0; k < 27; ++k) {
-            double m = (double)sh[k] / samples;
-            double v = ((double)sq[k] / samples - m * m) * samples / (samples - 1);
+            const double m = s1[k] / samples;
+            const double v = (s2[k] / samples - m * m) * samples / (samples - 1);
             var_out[idx * 27 + k] = (float)(std::max(0.0, v) / samples);
           }
         }

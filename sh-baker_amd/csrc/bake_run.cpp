@@ -65,8 +65,23 @@ struct Seg {
   uint32_t job, begin, T, spp;
 };
 
+// Job j's texels continue segment g (of job i) in memory: same context and
+// settings, and j's texel-list entries start where g's end (the tiles of one
+// surface). g may then run on into j's entries, which read as ranks of job
+// i past its own valid count; per texel nothing changes (random streams are
+// keyed by the texel, outputs are the context's), and the batch keeps one
+// segment, so it runs the one-job kernel forms.
+bool continues(const JobIn& i, const Seg& g, const JobIn& j) {
+  return i.c == j.c && i.stream == j.stream && i.cfg.samples == j.cfg.samples &&
+         i.cfg.bounces == j.cfg.bounces && i.cfg.num_light_samples == j.cfg.num_light_samples &&
+         i.cfg.flags == j.cfg.flags && i.cfg.seed == j.cfg.seed &&
+         (uint64_t)i.list_offset + g.begin + g.T == (uint64_t)j.list_offset;
+}
+
 // Greedy packing of every job's valid texels into batches of at most
 // batch_paths path slots (a texel's samples never split across batches).
+// A job whose texels continue the batch's last segment (continues())
+// extends that segment instead of opening one.
 std::vector<std::vector<Seg>> plan_batches(const std::vector<JobIn>& jobs, uint64_t cap) {
   std::vector<std::vector<Seg>> out;
   std::vector<Seg> cur;
@@ -80,7 +95,11 @@ std::vector<std::vector<Seg>> plan_batches(const std::vector<JobIn>& jobs, uint6
     const uint32_t spp = (uint32_t)jobs[j].cfg.samples;
     uint32_t left = jobs[j].valid, begin = 0;
     while (left > 0) {
-      if (cur.size() == (size_t)kMaxJobs) flush();
+      // job j's texels continue the last segment: extend it (its T stays
+      // below 2^32: at most cap / spp)
+      const bool extend =
+          begin == 0 && !cur.empty() && continues(jobs[cur.back().job], cur.back(), jobs[j]);
+      if (cur.size() == (size_t)kMaxJobs && !extend) flush();
       uint64_t room = cap > used ? cap - used : 0;
       uint32_t t = (uint32_t)std::min<uint64_t>(left, room / spp);
       if (t == 0) {
@@ -89,6 +108,13 @@ std::vector<std::vector<Seg>> plan_batches(const std::vector<JobIn>& jobs, uint6
           continue;
         }
         t = 1;  // one texel's samples exceed the batch size
+      }
+      if (extend) {
+        cur.back().T += t;
+        used += (uint64_t)t * spp;
+        begin += t;
+        left -= t;
+        continue;
       }
       cur.push_back({j, begin, t, spp});
       used += (uint64_t)t * spp;
