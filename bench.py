@@ -412,9 +412,12 @@ def per_texel_agreement(g_sh, r_sh, r_var, g_var, pts, width, stride):
     z = (g_sh[v].astype(np.float64) - r_sh[v])[live] / np.sqrt(var[live])
     az = np.abs(z)
     dead = ~live
+    gd, rd = g_sh[v][dead], r_sh[v][dead]
+    uneq = ~np.isclose(gd, rd, rtol=1e-5, atol=0)
     return {"texels": int(v.sum()), "coefficients": int(live.sum()),
             "coefficients_zero_variance": int(dead.sum()),
-            "zero_variance_equal": bool(np.allclose(g_sh[v][dead], r_sh[v][dead], rtol=1e-5, atol=0)),
+            "zero_variance_unequal": int(uneq.sum()),
+            "zero_variance_max_abs_diff": float(np.abs(gd - rd).max()) if gd.size else 0.0,
             "texels_black_in_both": int(((np.abs(g_sh[v]).max(1) == 0) &
                                          (np.abs(r_sh[v]).max(1) == 0)).sum()),
             "share_within_3_sigma": round(float((az <= 3.0).mean()), 5),

@@ -229,6 +229,33 @@ __device__ __forceinline__ void wave_sync_lds() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Inclusive prefix sum of x over the wave's 64 lanes: shifts by 1, 2, 4 and
+// 8 lanes within each row of 16 (DPP row_shr; lanes shifted in from outside
+// the row add 0), then the last lane of row 0 added to row 1 and of row 2 to
+// row 3 (row_bcast:15), then lane 31 to rows 2 and 3 (row_bcast:31). Six
+// DPP adds (VALU only, no memory access); each reads the register the
+// previous instruction wrote, so two wait states (s_nop 1) sit before it.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
+  uint32_t y;
+  asm volatile(
+      "s_nop 1\n\t"
+      "v_add_u32_dpp %0, %1, %1 row_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+      "s_nop 1\n\t"
+      "v_add_u32_dpp %0, %0, %0 row_shr:2 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+      "s_nop 1\n\t"
+      "v_add_u32_dpp %0, %0, %0 row_shr:4 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+      "s_nop 1\n\t"
+      "v_add_u32_dpp %0, %0, %0 row_shr:8 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+      "s_nop 1\n\t"
+      "v_add_u32_dpp %0, %0, %0 row_bcast:15 row_mask:0xa bank_mask:0xf\n\t"
+      "s_nop 1\n\t"
+      "v_add_u32_dpp %0, %0, %0 row_bcast:31 row_mask:0xc bank_mask:0xf\n\t"
+      "s_nop 1"
+      : "=&v"(y)
+      : "v"(x));
+  return y;
+}
+
 // Every lane of the wave calls this (active: the lane decoded a node this
 // step). Returns ANY && a hit was taken, as trav_tris does.
 template <bool ANY, bool COUNT>
@@ -238,17 +265,11 @@ __device__ __forceinline__ bool wave_tris(const SceneView& sc, const TravRay& r,
   const uint32_t lane = lane_id();
   const uint32_t k = active ? (uint32_t)__builtin_popcount(tmask) : 0u;
   if (COUNT) s.tris += k;
-  // off = the sum of k over the lanes below this one, K over the wave: bit
-  // b of every lane's k in one ballot, counted below the lane by mbcnt
-  uint32_t off = 0, K = 0;
-#pragma unroll
-  for (int b = 0; b < 5; ++b) {  // k <= 24: 8 leaf slots of at most 3 triangles
-    const uint64_t m = __ballot(((k >> b) & 1u) != 0u);
-    const uint32_t under = __builtin_amdgcn_mbcnt_hi(
-        (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-    off += under << b;
-    K += (uint32_t)__popcll(m) << b;
-  }
+  // off = the sum of k over the lanes below this one, K over the wave
+  // (k <= 24: 8 leaf slots of at most 3 triangles)
+  const uint32_t incl = wave_incl_scan(k);
+  const uint32_t off = incl - k;
+  const uint32_t K = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
   if (K == 0u) return false;
   if (K > kWavePairs) {
     bool f = false;
@@ -267,23 +288,17 @@ __device__ __forceinline__ bool wave_tris(const SceneView& sc, const TravRay& r,
   }
   if (k) L->key[lane] = init;
   wave_sync_lds();
-  const int kxyz = r.kx | (r.ky << 2) | (r.kz << 4);
+  // the owner's origin in the component order of its major axis
+  float rx, ry, rz;
+  rot3(r.kz, r.ox, r.oy, r.oz, rx, ry, rz);
   for (uint32_t base = 0; base < K; base += 64u) {
     const uint32_t qi = base + lane;
     const bool have = qi < K;
     const uint32_t pr = have ? L->pair[qi] : 0u;
     const int o = (int)(pr >> 26);
-    TravRay ro;
-    ro.ox = __shfl(r.ox, o);
-    ro.oy = __shfl(r.oy, o);
-    ro.oz = __shfl(r.oz, o);
-    ro.Sx = __shfl(r.Sx, o);
-    ro.Sy = __shfl(r.Sy, o);
-    ro.Sz = __shfl(r.Sz, o);
-    const int kk = __shfl(kxyz, o);
-    ro.kx = kk & 3;
-    ro.ky = (kk >> 2) & 3;
-    ro.kz = (kk >> 4) & 3;
+    const float ox = __shfl(rx, o), oy = __shfl(ry, o), oz = __shfl(rz, o);
+    const float Sx = __shfl(r.Sx, o), Sy = __shfl(r.Sy, o), Sz = __shfl(r.Sz, o);
+    const int kz = __shfl(r.kz, o);
     const float tnear_o = __shfl(r.tnear, o);
     unsigned long long key = ~0ull;
     float t = 0.f, u = 0.f, v = 0.f;
@@ -292,7 +307,7 @@ __device__ __forceinline__ bool wave_tris(const SceneView& sc, const TravRay& r,
       const float4* tp = reinterpret_cast<const float4*>(
           reinterpret_cast<const char*>(sc.tris) + __umul24(pr & 0x3ffffffu, 48u));
       const float4 a = tp[0], b = tp[1], c = tp[2];
-      if (tri_test(ro, a, b, c, t, u, v)) {
+      if (tri_test(kz, ox, oy, oz, Sx, Sy, Sz, a, b, c, t, u, v)) {
         const int id = (int)as_uint(a.w);
         // the owner's key starts at its best hit so far, so the minimum
         // over the keys keeps exactly the candidates better_hit accepts
@@ -657,10 +672,12 @@ __global__ void k_relocate_nodes(const CWNode* src, CWNode* dst, uint32_t n, uin
                                  uint32_t tri_off) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  CWNode nd = src[i];
+  // both arrays at the device node stride
+  CWNode nd = *reinterpret_cast<const CWNode*>(reinterpret_cast<const char*>(src) +
+                                               (size_t)i * kNodeBytes);
   nd.child_base += node_off;
   nd.tri_base += tri_off;
-  dst[i] = nd;
+  *reinterpret_cast<CWNode*>(reinterpret_cast<char*>(dst) + (size_t)i * kNodeBytes) = nd;
 }
 
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n) {

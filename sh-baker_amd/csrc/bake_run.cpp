@@ -187,7 +187,7 @@ int run_pass(Ctx& x, std::vector<JobIn>& jobs, bool graph, BakeRun* run, const A
     ttris = scenes_u[0]->d_tris.as<float4>();
     if (!scenes_u[0]->bvh.nodes.empty()) uroot[0] = 0;
   } else {
-    SHB_TRY(x, x.d_cnodes.reserve(std::max<size_t>(total_nodes * sizeof(CWNode), 16)));
+    SHB_TRY(x, x.d_cnodes.reserve(std::max<size_t>(total_nodes * kNodeBytes, 16)));
     SHB_TRY(x, x.d_ctris.reserve(std::max<size_t>(total_tris * 3 * sizeof(float4), 16)));
     size_t no = 0, to = 0;
     for (uint32_t u = 0; u < nu; ++u) {
@@ -195,8 +195,10 @@ int run_pass(Ctx& x, std::vector<JobIn>& jobs, bool graph, BakeRun* run, const A
       const size_t nn = c.bvh.nodes.size(), nt = c.bvh.tris.size() / 3;
       if (nn) {
         uroot[u] = (uint32_t)no;
-        SHB_TRY(x, launch_relocate_nodes(c.d_nodes.as<CWNode>(), x.d_cnodes.as<CWNode>() + no,
-                                         (uint32_t)nn, (uint32_t)no, (uint32_t)to, st));
+        SHB_TRY(x, launch_relocate_nodes(
+                       c.d_nodes.as<CWNode>(),
+                       reinterpret_cast<CWNode*>(x.d_cnodes.as<uint8_t>() + no * kNodeBytes),
+                       (uint32_t)nn, (uint32_t)no, (uint32_t)to, st));
         SHB_TRY(x, hipMemcpyAsync(x.d_ctris.as<float4>() + 3 * to, c.d_tris.ptr,
                                   nt * 3 * sizeof(float4), hipMemcpyDeviceToDevice, st));
       }

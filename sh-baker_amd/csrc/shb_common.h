@@ -75,6 +75,14 @@ struct alignas(16) CWNode {
 };
 static_assert(sizeof(CWNode) == 80, "CWNode must be 80 bytes");
 
+// Bytes between consecutive nodes of the device node array (the host tree
+// and its export are packed at 80).
+#ifndef SHB_NODE_BYTES
+#define SHB_NODE_BYTES 80
+#endif
+constexpr uint32_t kNodeBytes = SHB_NODE_BYTES;
+static_assert(kNodeBytes >= 80 && kNodeBytes % 16 == 0, "node stride");
+
 // ------------------------------------------------------------ lights
 enum : int32_t { kLightPoint = 0, kLightDirectional = 1, kLightSpot = 2, kLightArea = 3 };
 

@@ -23,16 +23,26 @@ struct TravRay {
   float dx, dy, dz;
   float ix, iy, iz;     // 1/d with zero components replaced by +-1e20
   float tnear;
-  int kx, ky, kz;
-  float Sx, Sy, Sz;
+  int kz;               // major axis of d; kx = kz + 1, ky = kz + 2 (mod 3)
+  float Sx, Sy, Sz;     // d[kx] / d[kz], d[ky] / d[kz], 1 / d[kz]
   uint32_t sstar;       // nearest octant slot
   uint32_t negx, negy, negz;
   uint32_t zx, zy, zz;  // direction component below 1e-20 in magnitude
 };
 
+// 1.0f / d, correctly rounded: the corrected hardware reciprocal where it
+// equals the IEEE division bit for bit (|d| in [2^-48, 2^63], checked for
+// every float of that range; the sign is copied, as the division's is),
+// the division elsewhere.
+__device__ __forceinline__ float recip(float d) {
+  const float a = fabsf(d);
+  if (a >= 0x1p-48f && a <= 0x1p63f) return copysignf(rcp_core(a), d);
+  return 1.0f / d;
+}
+
 __device__ __forceinline__ float safe_inv(float d) {
   float ad = fabsf(d);
-  float r = 1.0f / d;
+  float r = recip(d);
   return ad < 1e-20f ? (d < 0.f || (d == 0.f && signbit(d)) ? -1e20f : 1e20f) : r;
 }
 
@@ -71,34 +81,41 @@ __device__ __forceinline__ void trav_setup(TravRay& r, float4 o, float4 d) {
   int kx = kz + 1 == 3 ? 0 : kz + 1;
   int ky = kx + 1 == 3 ? 0 : kx + 1;
   float dkz = sel3(kz, d.x, d.y, d.z);
-  if (dkz < 0.f) {
-    int t = kx;
-    kx = ky;
-    ky = t;
-  }
-  r.kx = kx;
-  r.ky = ky;
   r.kz = kz;
   r.Sx = sel3(kx, d.x, d.y, d.z) / dkz;
   r.Sy = sel3(ky, d.x, d.y, d.z) / dkz;
-  r.Sz = 1.0f / dkz;
+  r.Sz = recip(dkz);
+}
+
+// The components (v[kx], v[ky], v[kz]) of v for major axis kz.
+__device__ __forceinline__ void rot3(int kz, float x, float y, float z, float& rx, float& ry,
+                                     float& rz) {
+  rx = kz == 0 ? y : (kz == 1 ? z : x);
+  ry = kz == 0 ? z : (kz == 1 ? x : y);
+  rz = kz == 0 ? x : (kz == 1 ? y : z);
 }
 
 // Watertight test (see the host restatement for the identical arithmetic).
-__device__ __forceinline__ bool tri_test(const TravRay& r, float4 a, float4 b,
-                                         float4 c, float& t, float& u,
-                                         float& v) {
-  float Axv = a.x - r.ox, Ayv = a.y - r.oy, Azv = a.z - r.oz;
-  float Bxv = b.x - r.ox, Byv = b.y - r.oy, Bzv = b.z - r.oz;
-  float Cxv = c.x - r.ox, Cyv = c.y - r.oy, Czv = c.z - r.oz;
-  float Akz = sel3(r.kz, Axv, Ayv, Azv), Bkz = sel3(r.kz, Bxv, Byv, Bzv),
-        Ckz = sel3(r.kz, Cxv, Cyv, Czv);
-  float Ax = fmaf(-r.Sx, Akz, sel3(r.kx, Axv, Ayv, Azv));
-  float Ay = fmaf(-r.Sy, Akz, sel3(r.ky, Axv, Ayv, Azv));
-  float Bx = fmaf(-r.Sx, Bkz, sel3(r.kx, Bxv, Byv, Bzv));
-  float By = fmaf(-r.Sy, Bkz, sel3(r.ky, Bxv, Byv, Bzv));
-  float Cx = fmaf(-r.Sx, Ckz, sel3(r.kx, Cxv, Cyv, Czv));
-  float Cy = fmaf(-r.Sy, Ckz, sel3(r.ky, Cxv, Cyv, Czv));
+// (ox, oy, oz): the ray origin in the order (kx, ky, kz) = (kz + 1, kz + 2,
+// kz) mod 3. The paper swaps kx and ky when d[kz] < 0 to keep the winding
+// for backface culling; this test is double sided, and the swap negates U,
+// V, W, det and T exactly (every operation involved is sign symmetric under
+// round-to-nearest), so t, u and v -- and which triangles pass -- are the
+// same bits with or without it; the host restatement keeps the swap.
+__device__ __forceinline__ bool tri_test(int kz, float ox, float oy, float oz, float Sx,
+                                         float Sy, float Sz, float4 a0, float4 b0, float4 c0,
+                                         float& t, float& u, float& v) {
+  float4 a, b, c;
+  rot3(kz, a0.x, a0.y, a0.z, a.x, a.y, a.z);
+  rot3(kz, b0.x, b0.y, b0.z, b.x, b.y, b.z);
+  rot3(kz, c0.x, c0.y, c0.z, c.x, c.y, c.z);
+  const float Akz = a.z - oz, Bkz = b.z - oz, Ckz = c.z - oz;
+  float Ax = fmaf(-Sx, Akz, a.x - ox);
+  float Ay = fmaf(-Sy, Akz, a.y - oy);
+  float Bx = fmaf(-Sx, Bkz, b.x - ox);
+  float By = fmaf(-Sy, Bkz, b.y - oy);
+  float Cx = fmaf(-Sx, Ckz, c.x - ox);
+  float Cy = fmaf(-Sy, Ckz, c.y - oy);
   float U = Cx * By - Cy * Bx;
   float V = Ax * Cy - Ay * Cx;
   float W = Bx * Ay - By * Ax;
@@ -110,9 +127,9 @@ __device__ __forceinline__ bool tri_test(const TravRay& r, float4 a, float4 b,
   if ((U < 0.f || V < 0.f || W < 0.f) && (U > 0.f || V > 0.f || W > 0.f)) return false;
   float det = U + V + W;
   if (det == 0.f) return false;
-  float Az = r.Sz * Akz, Bz = r.Sz * Bkz, Cz = r.Sz * Ckz;
+  float Az = Sz * Akz, Bz = Sz * Bkz, Cz = Sz * Ckz;
   float T = fmaf(U, Az, fmaf(V, Bz, W * Cz));
-  float inv = 1.0f / det;
+  float inv = recip(det);
   t = T * inv;
   u = V * inv;
   v = W * inv;
@@ -197,7 +214,7 @@ __device__ __forceinline__ bool trav_node(const SceneView& sc, const TravRay& r,
   // a 32-bit byte offset from one 24-bit multiply: trees hold fewer than
   // 2^24 nodes (kMaxTreeNodes, checked where trees are uploaded and joined)
   const uint4* np = reinterpret_cast<const uint4*>(reinterpret_cast<const char*>(sc.nodes) +
-                                                   __umul24(ni, 80u));
+                                                   __umul24(ni, kNodeBytes));
   uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3], w4 = np[4];
   if (COUNT) ++s.nodes;
   float px = as_float(w0.x), py = as_float(w0.y), pz = as_float(w0.z);
@@ -273,6 +290,8 @@ template <bool ANY, bool COUNT>
 __device__ __forceinline__ bool trav_tris(const SceneView& sc, const TravRay& r, TravState& s,
                                           uint32_t tri_base, uint32_t tmask) {
   bool found = false;
+  float rx, ry, rz;
+  rot3(r.kz, r.ox, r.oy, r.oz, rx, ry, rz);
   while (tmask) {
     uint32_t j = __builtin_ctz(tmask);
     tmask &= tmask - 1u;
@@ -280,7 +299,7 @@ __device__ __forceinline__ bool trav_tris(const SceneView& sc, const TravRay& r,
     float4 a = tp[0], b = tp[1], c = tp[2];
     if (COUNT) ++s.tris;
     float t, u, v;
-    if (tri_test(r, a, b, c, t, u, v)) {
+    if (tri_test(r.kz, rx, ry, rz, r.Sx, r.Sy, r.Sz, a, b, c, t, u, v)) {
       int id = (int)as_uint(a.w);
       if (better_hit(t, id, r.tnear, s.tbest, s.best)) {
         s.tbest = t;

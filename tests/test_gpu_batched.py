@@ -136,6 +136,17 @@ def test_tiled_bake_bit_identical_to_plain():
         # one tile: the whole surface
         sh, _, _ = b.bake_tiled(8, 3, [(0, 0, 96, 96)], seed=77)
         assert np.array_equal(bits(sh), bits(ref))
+        # small batches: tiles that follow each other in the texel list
+        # share segments, which now end and restart inside tiles
+        old = b.tuning()
+        try:
+            for bp in (1024, 1736, 4096):
+                b.set_tuning(batch_paths=bp)
+                sh, vis, st = b.bake_tiled(8, 3, grid, seed=77)
+                assert np.array_equal(bits(sh), bits(ref)) and np.array_equal(bits(vis), bits(rvis))
+                assert st["rays_extension"] == rst["rays_extension"]
+        finally:
+            b.set_tuning(**old)
     finally:
         b.close()
 
