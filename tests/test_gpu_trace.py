@@ -57,6 +57,32 @@ def test_closest_hit_bit_exact_vs_oracle(baker, which):
     assert (hg["prim_id"] >= 0).mean() > 0.3
 
 
+@pytest.mark.parametrize("scale", [1e-9, 1.0, 1e10])
+def test_closest_hit_bit_exact_at_extreme_scales(baker, scale):
+    """The triangle test's 1/det comes from the corrected hardware
+    reciprocal for |det| in [2^-48, 2^63] and from the IEEE division
+    outside it: the box room scaled by 1e-9 (determinants ~1e-16) and by
+    1e10 (~1e20) takes the division, scale 1 the reciprocal; hits stay
+    bit-exact against the oracle's FP32 restatement either way."""
+    s = scenes.box_room_scene()
+    s.positions = (s.positions.astype(np.float64) * scale).astype(np.float32)
+    baker.upload_scene(s)
+    rays = seeded_rays(s, 20000, 5)
+    rays["tnear"] = (rays["tnear"].astype(np.float64) * scale).astype(np.float32)
+    rays["tfar"] = (rays["tfar"].astype(np.float64) * scale).astype(np.float32)
+    hg = baker.trace_rays(rays)
+    ho = orc.Oracle(s).trace(rays, mode=0)
+    compare_exact(hg, ho)
+    assert (hg["prim_id"] >= 0).mean() > 0.3
+    # the determinants of the hits really lie where the case says
+    P = (s.positions[s.indices].astype(np.float64)).reshape(-1, 3, 3)
+    area2 = np.linalg.norm(np.cross(P[:, 1] - P[:, 0], P[:, 2] - P[:, 0]), axis=1)
+    if scale < 1:
+        assert area2.max() < 2.0 ** -48
+    elif scale > 1e9:
+        assert area2.min() > 2.0 ** 63
+
+
 def test_closest_hit_vs_fp64_bruteforce(baker):
     s = scenes.scaled_synthetic_scene(6000, 128, 3)
     baker.upload_scene(s)
