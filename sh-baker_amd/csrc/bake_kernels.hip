@@ -525,10 +525,11 @@ __global__ void __launch_bounds__(256, 5) k_generate(BakeArgs a) {
 // alpha splits; UNIT: every light and albedo scale is 1; ONE: the batch is
 // one job's segment, so the shading code is compiled once, for a
 // wave-uniform job record).
-// Every form runs at 5 waves per SIMD, since the light loop waits on a
-// scalar load per light: the one-job forms fit in 75-91 registers; the
+// Every form is capped at 96 registers (at least 5 waves per SIMD), since
+// the light loop waits on a scalar load per light: the bake's one-job
+// forms take 75-81 (6 waves), the other one-job forms up to 91; the
 // multi-job forms, with a second copy of the shading code for waves that
-// hold several jobs, spill a little under that cap, which is still faster
+// hold several jobs, spill a little under the cap, which is still faster
 // than 4 spill-free waves.
 template <bool ADJ, bool SPLIT, bool UNIT, bool MULTI, bool ONE>
 __global__ void __launch_bounds__(256, 5) k_shade(BakeArgs a, int depth, const uint32_t* in_queue,
