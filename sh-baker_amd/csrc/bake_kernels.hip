@@ -117,8 +117,9 @@ struct LdsStack {
   int tid;
   int stride;
   int entries;
-  __device__ uint2 get(int sp) const { return mem[sp * stride + tid]; }
-  __device__ void set(int sp, uint2 v) const { mem[sp * stride + tid] = v; }
+  // sp < 17 and stride <= 1024: a 24-bit multiply (full rate) suffices
+  __device__ uint2 get(int sp) const { return mem[__umul24(sp, stride) + tid]; }
+  __device__ void set(int sp, uint2 v) const { mem[__umul24(sp, stride) + tid] = v; }
   __device__ int cap() const { return entries; }
 };
 

@@ -37,16 +37,14 @@ int upload_bvh(Ctx& c) {
     return fail(c, SHB_ERR_LIMIT,
                 "BVH of " + std::to_string(c.bvh.nodes.size()) + " nodes: at most " +
                     std::to_string(kMaxTreeNodes - 1) + " (about 160 M triangles)");
-  if (kNodeBytes == sizeof(CWNode)) {
-    SHB_TRY(c, upload(c.d_nodes, c.bvh.nodes, c.stream));
-  } else {
-    // the device stride: each node followed by padding
-    std::vector<uint8_t> nb(c.bvh.nodes.size() * (size_t)kNodeBytes, 0);
-    for (size_t i = 0; i < c.bvh.nodes.size(); ++i)
-      std::memcpy(nb.data() + i * kNodeBytes, &c.bvh.nodes[i], sizeof(CWNode));
-    SHB_TRY(c, upload(c.d_nodes, nb, c.stream));
-    SHB_TRY(c, hipStreamSynchronize(c.stream));  // nb is a temporary
+  // the device form of the tree: device meta bytes, nodes kNodeBytes apart
+  std::vector<uint8_t> nb(c.bvh.nodes.size() * (size_t)kNodeBytes, 0);
+  for (size_t i = 0; i < c.bvh.nodes.size(); ++i) {
+    const CWNode d = device_node(c.bvh.nodes[i]);
+    std::memcpy(nb.data() + i * kNodeBytes, &d, sizeof(CWNode));
   }
+  SHB_TRY(c, upload(c.d_nodes, nb, c.stream));
+  SHB_TRY(c, hipStreamSynchronize(c.stream));  // nb is a temporary
   SHB_TRY(c, upload(c.d_tris, c.bvh.tris, c.stream));
   return SHB_OK;
 }

@@ -75,6 +75,24 @@ struct alignas(16) CWNode {
 };
 static_assert(sizeof(CWNode) == 80, "CWNode must be 80 bytes");
 
+// The device form of a node: the same layout, but a leaf slot's meta byte
+// holds its triangle bits instead of its count -- (2^count - 1) << 5 |
+// offset, so the traversal ORs (meta >> 5) << offset into its triangle mask
+// directly -- and an internal slot's meta byte is 0 (imask names internal
+// slots and ranks them). The host tree and its export keep the encoding
+// above.
+inline CWNode device_node(const CWNode& n) {
+  CWNode d = n;
+  for (int s = 0; s < 8; ++s) {
+    const uint32_t m = n.meta[s];
+    if (m & 0x80u)
+      d.meta[s] = 0;
+    else if (m != 0u)
+      d.meta[s] = (uint8_t)((((1u << (m >> 5)) - 1u) << 5) | (m & 31u));
+  }
+  return d;
+}
+
 // Bytes between consecutive nodes of the device node array (the host tree
 // and its export are packed at 80).
 #ifndef SHB_NODE_BYTES

@@ -253,7 +253,7 @@ __device__ __forceinline__ bool trav_node(const SceneView& sc, const TravRay& r,
     uint32_t nxw = c < 4 ? nx0 : nx1, fxw = c < 4 ? fx0 : fx1;
     uint32_t nyw = c < 4 ? ny0 : ny1, fyw = c < 4 ? fy0 : fy1;
     uint32_t nzw = c < 4 ? nz0 : nz1, fzw = c < 4 ? fz0 : fz1;
-    uint32_t mw = c < 4 ? w1.z : w1.w;
+    const uint32_t mw = c < 4 ? w1.z : w1.w;
     float tnx = fmaf((float)((nxw >> sh) & 0xffu), ax, bxn);
     float tfx = fmaf((float)((fxw >> sh) & 0xffu), ax, bxf);
     float tny = fmaf((float)((nyw >> sh) & 0xffu), ay, byn);
@@ -262,15 +262,14 @@ __device__ __forceinline__ bool trav_node(const SceneView& sc, const TravRay& r,
     float tfz = fmaf((float)((fzw >> sh) & 0xffu), az, bzf);
     float tmin = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, r.tnear));
     float tmax = fminf(fminf(fminf(tfx, tfy), tfz), tmax_ray);
-    // an empty slot (meta 0) adds nothing even when its box is taken: it
-    // is neither internal nor a leaf with triangles
-    uint32_t m = (mw >> sh) & 0xffu;
+    // an empty slot adds nothing even when its box is taken: its imask bit
+    // and its device meta byte are 0
     if (tmin <= tmax) {
-      // internal children (meta 0x80 | rank) set their octant bit; leaves
-      // (count << 5 | offset, count 1..3) their triangles; the leaf count
-      // field of an internal child, (0x80 >> 5) & 3, is 0
-      hits |= (m >> 7) << (c ^ r.sstar);
-      tmask |= ((1u << ((m >> 5) & 3u)) - 1u) << (m & 31u);
+      // internal children (imask) set their octant bit; leaves OR in their
+      // triangle bits (device meta: bits << 5 | offset; 0 for internal
+      // children)
+      hits |= ((imask >> c) & 1u) << (c ^ r.sstar);
+      tmask |= ((mw >> (sh + 5)) & 7u) << ((mw >> sh) & 31u);
     }
   }
   s.gbase = w1.x;
