@@ -236,6 +236,8 @@ __device__ __forceinline__ void wave_sync_lds() {
 // row 3 (row_bcast:15), then lane 31 to rows 2 and 3 (row_bcast:31). Six
 // DPP adds (VALU only, no memory access); each reads the register the
 // previous instruction wrote, so two wait states (s_nop 1) sit before it.
+// Every lane of the wave must be active (DPP reads inactive lanes' stale
+// registers): wave_tris is reached by the whole wave.
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
   uint32_t y;
   asm volatile(
