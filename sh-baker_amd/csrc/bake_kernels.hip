@@ -36,6 +36,51 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
   return base + (uint32_t)__popcll(m & ((1ull << lane_id()) - 1ull));
 }
 
+// Appends the predicated lanes of a 256-thread block to NQ queues with one
+// atomic per queue and block: a quarter of wave_append's atomics, which all
+// land on one cache line of counters (the bake's shading stages made one
+// to two per wave of every path). Queue q's entries of the block are
+// contiguous, wave by wave and lane by lane. Every thread of the block must
+// call it with the same NQ (it synchronises the block); `lds` holds 5 * NQ
+// words. at[q]: the slot of a lane with pred[q].
+template <int NQ>
+__device__ __forceinline__ void block_append(uint32_t* const (&counter)[NQ],
+                                             const bool (&pred)[NQ], uint32_t (&at)[NQ],
+                                             uint32_t* lds) {
+  // the wave index as a scalar, and lane prefixes counted from the ballot
+  // (mbcnt), so nothing derived from the lane id is kept live across the
+  // caller's code
+  const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  uint64_t m[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    m[q] = __ballot(pred[q]);
+    if (threadIdx.x == (w << 6)) lds[4 * q + w] = (uint32_t)__popcll(m[q]);
+  }
+  __syncthreads();
+  if (threadIdx.x < (uint32_t)NQ) {
+    const uint32_t q = threadIdx.x;
+    uint32_t* c = counter[0];
+#pragma unroll
+    for (int k = 1; k < NQ; ++k)
+      if (q == (uint32_t)k) c = counter[k];
+    const uint32_t n0 = lds[4 * q], n1 = lds[4 * q + 1], n2 = lds[4 * q + 2], n3 = lds[4 * q + 3];
+    lds[4 * q] = 0u;
+    lds[4 * q + 1] = n0;
+    lds[4 * q + 2] = n0 + n1;
+    lds[4 * q + 3] = n0 + n1 + n2;
+    const uint32_t total = n0 + n1 + n2 + n3;
+    lds[4 * NQ + q] = total ? atomicAdd(c, total) : 0u;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NQ; ++q)
+    at[q] = lds[4 * NQ + q] + lds[4 * q + w] +
+            __builtin_amdgcn_mbcnt_hi((uint32_t)(m[q] >> 32),
+                                      __builtin_amdgcn_mbcnt_lo((uint32_t)m[q], 0u));
+  __syncthreads();  // the words are rewritten by the next call
+}
+
 // ------------------------------------------------------------------ trace
 struct HitEpilogue {
   float4* hits;
@@ -501,6 +546,7 @@ __global__ void k_hit_details(SceneView sc, const float4* rays, const float4* hi
 // light and albedo scale is 1).
 template <bool UNIT, bool MULTI, bool ONE>
 __global__ void __launch_bounds__(256, 5) k_generate(BakeArgs a) {
+  __shared__ uint32_t qw[5];
   const uint32_t n = a.P;
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
     uint32_t p = base + threadIdx.x;
@@ -510,15 +556,19 @@ __global__ void __launch_bounds__(256, 5) k_generate(BakeArgs a) {
       with_job<ONE>(a, sr.job,
                     [&](const JobDev& J) { shadow = generate_path<UNIT, MULTI>(a, J, sr, p); });
     }
+    uint32_t* const sq[1] = {a.shadow_count};
     if (MULTI) {
-      for (uint32_t j = 0; j < a.nls; ++j) {  // wave-uniform bound
-        const bool made = (shadow >> j) & 1u;
-        uint32_t at = wave_append(a.shadow_count, made);
-        if (made) a.shadow_queue[at] = a.nls * a.Pc + j * a.P + p;
+      for (uint32_t j = 0; j < a.nls; ++j) {  // block-uniform bound
+        const bool made[1] = {((shadow >> j) & 1u) != 0u};
+        uint32_t at[1];
+        block_append<1>(sq, made, at, qw);
+        if (made[0]) a.shadow_queue[at[0]] = a.nls * a.Pc + j * a.P + p;
       }
     } else {
-      uint32_t at = wave_append(a.shadow_count, shadow != 0u);
-      if (shadow) a.shadow_queue[at] = a.Pc + p;
+      const bool made[1] = {shadow != 0u};
+      uint32_t at[1];
+      block_append<1>(sq, made, at, qw);
+      if (made[0]) a.shadow_queue[at[0]] = a.Pc + p;
     }
   }
 }
@@ -539,6 +589,7 @@ __global__ void __launch_bounds__(256, 5) k_shade(BakeArgs a, int depth, const u
                                                   const uint32_t* in_count, uint32_t* out_queue,
                                                   uint32_t* out_count) {
   extern __shared__ float grad_lds[];
+  __shared__ uint32_t qw[15];
   GradAcc acc;
   if (ADJ)
     acc.init(a.adj_nparam <= kAdjLdsParams ? grad_lds : nullptr, a.adj_grad, a.adj_nparam);
@@ -555,22 +606,32 @@ __global__ void __launch_bounds__(256, 5) k_shade(BakeArgs a, int depth, const u
         shade_path<ADJ, SPLIT, UNIT, MULTI>(a, J, sr, depth, p, shadow, extend, child, &acc);
       });
     }
+    // shadow rays (one queue per pick j when MULTI), the continuation and,
+    // with SPLIT, the transmitted ray of a split (the same queue)
     if (MULTI) {
-      for (uint32_t j = 0; j < a.nls; ++j) {  // wave-uniform bound
-        const bool made = (shadow >> j) & 1u;
-        uint32_t at = wave_append(a.shadow_count, made);
-        if (made) a.shadow_queue[at] = j * a.Pc + p;
+      uint32_t* const sq[1] = {a.shadow_count};
+      for (uint32_t j = 0; j < a.nls; ++j) {  // block-uniform bound
+        const bool made[1] = {((shadow >> j) & 1u) != 0u};
+        uint32_t at[1];
+        block_append<1>(sq, made, at, qw);
+        if (made[0]) a.shadow_queue[at[0]] = j * a.Pc + p;
       }
-    } else {
-      uint32_t at = wave_append(a.shadow_count, shadow != 0u);
-      if (shadow) a.shadow_queue[at] = p;
     }
-    uint32_t at2 = wave_append(out_count, extend);
-    if (extend) out_queue[at2] = p;
-    if (SPLIT) {  // transmitted rays of splits
-      const bool spawned = child != kNoSlot;
-      uint32_t at3 = wave_append(out_count, spawned);
-      if (spawned) out_queue[at3] = child;
+    if (SPLIT) {
+      uint32_t* const qs[3] = {a.shadow_count, out_count, out_count};
+      const bool pr[3] = {!MULTI && shadow != 0u, extend, child != kNoSlot};
+      uint32_t at[3];
+      block_append<3>(qs, pr, at, qw);
+      if (pr[0]) a.shadow_queue[at[0]] = p;
+      if (pr[1]) out_queue[at[1]] = p;
+      if (pr[2]) out_queue[at[2]] = child;
+    } else {
+      uint32_t* const qs[2] = {a.shadow_count, out_count};
+      const bool pr[2] = {!MULTI && shadow != 0u, extend};
+      uint32_t at[2];
+      block_append<2>(qs, pr, at, qw);
+      if (pr[0]) a.shadow_queue[at[0]] = p;
+      if (pr[1]) out_queue[at[1]] = p;
     }
   }
   if (ADJ) acc.flush();
