@@ -354,6 +354,36 @@ __device__ __forceinline__ float light_weight(const SceneView& sc, const LightDe
   return maxc(cmul(k3, cos_n * (ok ? bw : f3{0.f, 0.f, 0.f})));
 }
 
+// light_weight of a simple area light (one triangle, constant grey
+// emitter: LightDev::simple_run > 0) from its draws: the same operations in
+// the same order, without the branches on the light's kind.
+template <bool VERTEX>
+__device__ __forceinline__ float simple_area_weight(const AreaTri& at, float wmax, float u1,
+                                                    float u2, f3 P, f3 N, float maxbw,
+                                                    bool front) {
+  if (u1 + u2 > 1.0f) {
+    u1 = 1.0f - u1;
+    u2 = 1.0f - u2;
+  }
+  const f3 target = {fmaf(u2, at.e2[0], fmaf(u1, at.e1[0], at.v0[0])),
+                     fmaf(u2, at.e2[1], fmaf(u1, at.e1[1], at.v0[1])),
+                     fmaf(u2, at.e2[2], fmaf(u1, at.e1[2], at.v0[2]))};
+  f3 Lv = target - P;
+  const float dist_sq = sqnorm_fma(Lv);
+  float dist, inv;
+  inv_dist(dist_sq, dist, inv);
+  const float inv2 = inv * inv;
+  Lv = Lv * inv;
+  const float cos_n = dot_fma(N, Lv);
+  const float kmax = wmax * inv2;
+  float w = kmax;
+  if (VERTEX) {
+    const bool ok = front && cos_n > 0.0f;
+    w = kmax * (cos_n * (ok ? maxbw : 0.f));
+  }
+  return cos_n < 0.f ? 0.f : w;
+}
+
 template <bool VERTEX, bool UNIT>
 __device__ __forceinline__ bool nee_sample(const SceneView& sc, f3 P, f3 N, f3 brdf, f3 brdf_w,
                                            bool front, Rng& rng, NeeResult& out) {
@@ -373,28 +403,46 @@ __device__ __forceinline__ bool nee_sample(const SceneView& sc, f3 P, f3 N, f3 b
     }
   };
   const int nl = sc.num_lights;
-  // two lights per iteration: both lights' numbers are drawn first, in the
-  // stream's order (draws of l, its pick uniform, draws of l + 1, its
-  // uniform), so their weights are independent work the scheduler can
-  // interleave; the reservoir then takes l before l + 1
+  // Lights in scene order. A run of simple area lights goes two per
+  // iteration without branches on the kind: both lights' numbers are drawn
+  // first, in the stream's order (u1, u2 and the pick uniform of l, then of
+  // l + 1), so their weights are independent work the scheduler can
+  // interleave; the reservoir then takes l before l + 1. Other lights go
+  // one at a time.
   int l = 0;
-  for (; l + 1 < nl; l += 2) {
-    const LightDev La = const_load(sc.lights, (uint32_t)l);
-    const LightDev Lb = const_load(sc.lights, (uint32_t)(l + 1));
-    const LightDraws da = light_draws(La, rng);
-    const float ua = rng.uniform();
-    const LightDraws db = light_draws(Lb, rng);
-    const float ub = rng.uniform();
-    const float wa = light_weight<VERTEX>(sc, La, da, P, N, bw, maxbw, front);
-    const float wb = light_weight<VERTEX>(sc, Lb, db, P, N, bw, maxbw, front);
-    take(l, da, wa, ua);
-    take(l + 1, db, wb, ub);
-  }
-  if (l < nl) {
-    const LightDev L = const_load(sc.lights, (uint32_t)l);
-    const LightDraws d = light_draws(L, rng);
-    const float u = rng.uniform();
-    take(l, d, light_weight<VERTEX>(sc, L, d, P, N, bw, maxbw, front), u);
+  while (l < nl) {
+    const int run = (int)const_load(sc.lights, (uint32_t)l).simple_run;
+    if (run > 0) {
+      const int end = l + run;
+      for (; l + 1 < end; l += 2) {
+        const LightDev La = const_load(sc.lights, (uint32_t)l);
+        const LightDev Lb = const_load(sc.lights, (uint32_t)(l + 1));
+        const AreaTri ta = const_load(sc.area_tris, La.first_tri);
+        const AreaTri tb = const_load(sc.area_tris, Lb.first_tri);
+        const LightDraws da{0u, rng.uniform(), rng.uniform()};
+        const float ua = rng.uniform();
+        const LightDraws db{0u, rng.uniform(), rng.uniform()};
+        const float ub = rng.uniform();
+        const float wa = simple_area_weight<VERTEX>(ta, La.wmax, da.u1, da.u2, P, N, maxbw, front);
+        const float wb = simple_area_weight<VERTEX>(tb, Lb.wmax, db.u1, db.u2, P, N, maxbw, front);
+        take(l, da, wa, ua);
+        take(l + 1, db, wb, ub);
+      }
+      if (l < end) {
+        const LightDev L = const_load(sc.lights, (uint32_t)l);
+        const AreaTri t = const_load(sc.area_tris, L.first_tri);
+        const LightDraws d{0u, rng.uniform(), rng.uniform()};
+        const float u = rng.uniform();
+        take(l, d, simple_area_weight<VERTEX>(t, L.wmax, d.u1, d.u2, P, N, maxbw, front), u);
+        ++l;
+      }
+    } else {
+      const LightDev L = const_load(sc.lights, (uint32_t)l);
+      const LightDraws d = light_draws(L, rng);
+      const float u = rng.uniform();
+      take(l, d, light_weight<VERTEX>(sc, L, d, P, N, bw, maxbw, front), u);
+      ++l;
+    }
   }
   if (W < 1e-6f || pick < 0) return false;
   f3 rad, radw, sdir;

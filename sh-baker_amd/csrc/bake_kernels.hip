@@ -585,7 +585,7 @@ __global__ void __launch_bounds__(256, 5) k_generate(BakeArgs a) {
 // hold several jobs, spill a little under the cap, which is still faster
 // than 4 spill-free waves.
 template <bool ADJ, bool SPLIT, bool UNIT, bool MULTI, bool ONE>
-__global__ void __launch_bounds__(256, 5) k_shade(BakeArgs a, int depth, const uint32_t* in_queue,
+__global__ void __launch_bounds__(256, (UNIT && ONE && !MULTI && !SPLIT && !ADJ) ? 6 : 5) k_shade(BakeArgs a, int depth, const uint32_t* in_queue,
                                                   const uint32_t* in_count, uint32_t* out_queue,
                                                   uint32_t* out_count) {
   extern __shared__ float grad_lds[];

@@ -276,6 +276,13 @@ int build_host_scene(const shb_scene_desc* d, HostScene* out, std::string* err) 
     L.grey = !per_sample && wc[0] == wc[1] && wc[1] == wc[2] ? 1 : 0;
     hs.lights.push_back(L);
   }
+  // runs of simple area lights, counted from the back
+  for (size_t i = hs.lights.size(); i-- > 0;) {
+    LightDev& L = hs.lights[i];
+    const bool simple = L.type == kLightArea && L.num_tris == 1 && L.emit_const && L.grey;
+    const uint32_t next = i + 1 < hs.lights.size() ? hs.lights[i + 1].simple_run : 0u;
+    L.simple_run = simple ? next + 1u : 0u;
+  }
   *out = std::move(hs);
   return SHB_OK;
 }

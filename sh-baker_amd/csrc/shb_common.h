@@ -121,7 +121,11 @@ struct alignas(16) LightDev {
   // light: its vertex weight needs only the largest BRDF channel)
   float wmax;
   int32_t grey;
-  int32_t pad[2];
+  // lights from this one on (in scene order) that are single-triangle
+  // area lights with a constant grey emitter; 0 when this one is not. The
+  // NEE loop takes such runs in a branch-free form.
+  uint32_t simple_run;
+  int32_t pad;
 };
 
 // Area-light triangle record (object-space vertices, light.cpp:45-49): v0
