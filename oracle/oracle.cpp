@@ -889,8 +889,8 @@ static Xoshiro128ss site_rng(uint64_t seed, uint32_t texel, uint32_t sample,
 }
 
 // Unshadowed radiance, shadow ray and area pdf of light l at P in the GPU
-// stream: an area light first draws its triangle (when it has several) and
-// its two barycentric uniforms.
+// stream: an area light first draws its triangle (when it has several), then
+// one number whose high and low 16 bits are its barycentric uniforms.
 static V3 GpuLightEval(const Scene& s, const Light& l, V3 P, V3 N, V3 wo, const Mat* mat,
                        V2 uv, Xoshiro128ss& x, bool texel, ShadowRay* ray, float* ap) {
   *ap = 1.0f;
@@ -900,8 +900,8 @@ static V3 GpuLightEval(const Scene& s, const Light& l, V3 P, V3 N, V3 wo, const 
   if (area_valid(s, l)) {
     uint32_t nt = (uint32_t)(s.geos[l.geom].idx.size() / 3);
     uint32_t tri = nt > 1 ? x.index(nt) : 0u;  // one triangle: no draw
-    float u1 = x.uniform();
-    float u2 = x.uniform();
+    float u1, u2;
+    x.uniform_pair16(u1, u2);
     as = SampleAreaLightAt(s, l, tri, u1, u2, true);
   }
   V3 rd = LightRadiance(s, l, &as, P, N, !texel, mat, uv, wo, ray, true);

@@ -118,6 +118,13 @@ struct Xoshiro128ss {
   }
   // Uniform float in [0, 1) with 24 bits.
   float uniform() { return (float)(next() >> 8) * 0x1p-24f; }
+  // Two uniforms in [0, 1) with 16 bits each from one draw (high half, low
+  // half): the GPU stream's barycentric pair of an area light.
+  void uniform_pair16(float& a, float& b) {
+    const uint32_t x = next();
+    a = (float)(x >> 16) * 0x1p-16f;
+    b = (float)(x & 0xffffu) * 0x1p-16f;
+  }
   // Uniform integer in [0, n) by 32x32 -> 64 multiply-shift.
   uint32_t index(uint32_t n) {
     return (uint32_t)(((uint64_t)next() * (uint64_t)n) >> 32);

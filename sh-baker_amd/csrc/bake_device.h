@@ -139,21 +139,25 @@ __device__ __forceinline__ float dot_fma(f3 a, f3 b) {
 // Light l at P: unshadowed radiance with the light's scale (rad) and
 // without (radw, the selection weight's radiance), the pending shadow ray
 // (direction, tfar) and the area pdf; an area light first draws its triangle
-// (when it has several) and two barycentric uniforms from rng. VERTEX:
+// (when it has several) and its barycentric pair from rng. VERTEX:
 // BRDF x cosine applied (light.h *Radiance<Brdf>); else the texel form.
 // The random numbers light L takes (an area light: its triangle when it has
-// several, then two barycentric uniforms; other lights: none).
+// several, then its barycentric pair from one draw; other lights: none).
 struct LightDraws {
   uint32_t tri;
   float u1, u2;
 };
+__device__ __forceinline__ LightDraws simple_draws(Rng& rng) {
+  LightDraws d{0u, 0.f, 0.f};
+  rng.uniform_pair16(d.u1, d.u2);
+  return d;
+}
 __device__ __forceinline__ LightDraws light_draws(const LightDev& L, Rng& rng) {
   LightDraws d{0u, 0.f, 0.f};
   if (L.type == kLightArea && L.num_tris != 0) {
     // single-triangle emitters draw no triangle index
     d.tri = L.num_tris > 1 ? rng.index(L.num_tris) : 0u;
-    d.u1 = rng.uniform();
-    d.u2 = rng.uniform();
+    rng.uniform_pair16(d.u1, d.u2);
   }
   return d;
 }
@@ -405,8 +409,8 @@ __device__ __forceinline__ bool nee_sample(const SceneView& sc, f3 P, f3 N, f3 b
   const int nl = sc.num_lights;
   // Lights in scene order. A run of simple area lights goes two per
   // iteration without branches on the kind: both lights' numbers are drawn
-  // first, in the stream's order (u1, u2 and the pick uniform of l, then of
-  // l + 1), so their weights are independent work the scheduler can
+  // first, in the stream's order (the barycentric pair and the pick uniform
+  // of l, then of l + 1), so their weights are independent work the scheduler can
   // interleave; the reservoir then takes l before l + 1. Other lights go
   // one at a time.
   int l = 0;
@@ -419,9 +423,9 @@ __device__ __forceinline__ bool nee_sample(const SceneView& sc, f3 P, f3 N, f3 b
         const LightDev Lb = const_load(sc.lights, (uint32_t)(l + 1));
         const AreaTri ta = const_load(sc.area_tris, La.first_tri);
         const AreaTri tb = const_load(sc.area_tris, Lb.first_tri);
-        const LightDraws da{0u, rng.uniform(), rng.uniform()};
+        const LightDraws da = simple_draws(rng);
         const float ua = rng.uniform();
-        const LightDraws db{0u, rng.uniform(), rng.uniform()};
+        const LightDraws db = simple_draws(rng);
         const float ub = rng.uniform();
         const float wa = simple_area_weight<VERTEX>(ta, La.wmax, da.u1, da.u2, P, N, maxbw, front);
         const float wb = simple_area_weight<VERTEX>(tb, Lb.wmax, db.u1, db.u2, P, N, maxbw, front);
@@ -431,7 +435,7 @@ __device__ __forceinline__ bool nee_sample(const SceneView& sc, f3 P, f3 N, f3 b
       if (l < end) {
         const LightDev L = const_load(sc.lights, (uint32_t)l);
         const AreaTri t = const_load(sc.area_tris, L.first_tri);
-        const LightDraws d{0u, rng.uniform(), rng.uniform()};
+        const LightDraws d = simple_draws(rng);
         const float u = rng.uniform();
         take(l, d, simple_area_weight<VERTEX>(t, L.wmax, d.u1, d.u2, P, N, maxbw, front), u);
         ++l;

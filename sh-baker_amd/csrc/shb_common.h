@@ -215,6 +215,13 @@ struct Rng {
     return result;
   }
   SHB_HD float uniform() { return (float)(next() >> 8) * 0x1p-24f; }
+  // Two uniforms in [0, 1) with 16 bits each from one draw (high half,
+  // low half): an area light's barycentric pair.
+  SHB_HD void uniform_pair16(float& a, float& b) {
+    const uint32_t x = next();
+    a = (float)(x >> 16) * 0x1p-16f;
+    b = (float)(x & 0xffffu) * 0x1p-16f;
+  }
   SHB_HD uint32_t index(uint32_t n) {
     return (uint32_t)(((uint64_t)next() * (uint64_t)n) >> 32);
   }
