@@ -890,10 +890,15 @@ static Xoshiro128ss site_rng(uint64_t seed, uint32_t texel, uint32_t sample,
 
 // Unshadowed radiance, shadow ray and area pdf of light l at P in the GPU
 // stream: an area light first draws its triangle (when it has several), then
-// one number whose high and low 16 bits are its barycentric uniforms.
+// one number whose high and low 16 bits are its barycentric uniforms. With
+// pick_u (the one-sample NEE) that step also gives the light's pick uniform
+// (its xoshiro128+ output) and *picked is set; otherwise the caller draws
+// the pick uniform itself.
 static V3 GpuLightEval(const Scene& s, const Light& l, V3 P, V3 N, V3 wo, const Mat* mat,
-                       V2 uv, Xoshiro128ss& x, bool texel, ShadowRay* ray, float* ap) {
+                       V2 uv, Xoshiro128ss& x, bool texel, ShadowRay* ray, float* ap,
+                       float* pick_u = nullptr, bool* picked = nullptr) {
   *ap = 1.0f;
+  if (picked) *picked = false;
   if (l.type != SHB_LIGHT_AREA)
     return LightRadiance(s, l, nullptr, P, N, !texel, mat, uv, wo, ray, true);
   AreaSample as{};
@@ -901,7 +906,12 @@ static V3 GpuLightEval(const Scene& s, const Light& l, V3 P, V3 N, V3 wo, const 
     uint32_t nt = (uint32_t)(s.geos[l.geom].idx.size() / 3);
     uint32_t tri = nt > 1 ? x.index(nt) : 0u;  // one triangle: no draw
     float u1, u2;
-    x.uniform_pair16(u1, u2);
+    if (pick_u) {
+      x.uniform_pair16_and_pick(u1, u2, *pick_u);
+      *picked = true;
+    } else {
+      x.uniform_pair16(u1, u2);
+    }
     as = SampleAreaLightAt(s, l, tri, u1, u2, true);
   }
   V3 rd = LightRadiance(s, l, &as, P, N, !texel, mat, uv, wo, ray, true);
@@ -911,7 +921,9 @@ static V3 GpuLightEval(const Scene& s, const Light& l, V3 P, V3 N, V3 wo, const 
 
 // NEE in the GPU stream (light.cpp:82-189 at a vertex, 191-293 at a texel).
 // One sample: a reservoir pick over lights: for each light in order, its
-// area draws then u; W += w; pick light if u*W < w -- light i with
+// area draws and u (an area light with triangles: u from its pair's step,
+// GpuLightEval; other lights: one uniform); W += w; pick light if u*W < w --
+// light i with
 // probability w_i / W, as discrete_distribution's.
 // n > 1 samples: the reference's n picks with replacement from one set of
 // per-light samples, as an inverse CDF: a first pass sums W, then n
@@ -979,9 +991,11 @@ static V3 NeeGpu(const Scene& s, V3 P, V3 N, V3 wo, const Mat* mat, V2 uv,
   for (const Light& l : s.lights) {
     ShadowRay ray;
     float ap;
-    V3 rd = GpuLightEval(s, l, P, N, wo, mat, uv, x, texel, &ray, &ap);
+    float u;
+    bool picked;
+    V3 rd = GpuLightEval(s, l, P, N, wo, mat, uv, x, texel, &ray, &ap, &u, &picked);
     float w = maxc(rd);
-    float u = x.uniform();
+    if (!picked) u = x.uniform();
     W += w;
     if (w > 0.f && u * W < w) {
       pick = (int)(&l - s.lights.data());

@@ -125,6 +125,14 @@ struct Xoshiro128ss {
     a = (float)(x >> 16) * 0x1p-16f;
     b = (float)(x & 0xffffu) * 0x1p-16f;
   }
+  // The pair and, from the same step, the 24-bit uniform of xoshiro128+'s
+  // output s[0] + s[3] (before the update): the GPU stream's barycentric
+  // pair and pick uniform of an area light in the one-sample NEE.
+  void uniform_pair16_and_pick(float& a, float& b, float& u) {
+    const uint32_t plus = s[0] + s[3];
+    uniform_pair16(a, b);
+    u = (float)(plus >> 8) * 0x1p-24f;
+  }
   // Uniform integer in [0, n) by 32x32 -> 64 multiply-shift.
   uint32_t index(uint32_t n) {
     return (uint32_t)(((uint64_t)next() * (uint64_t)n) >> 32);

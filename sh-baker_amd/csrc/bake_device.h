@@ -147,9 +147,22 @@ struct LightDraws {
   uint32_t tri;
   float u1, u2;
 };
-__device__ __forceinline__ LightDraws simple_draws(Rng& rng) {
+// A simple area light's draws and pick uniform (one state update).
+__device__ __forceinline__ LightDraws simple_draws(Rng& rng, float& u) {
   LightDraws d{0u, 0.f, 0.f};
-  rng.uniform_pair16(d.u1, d.u2);
+  rng.uniform_pair16_and_pick(d.u1, d.u2, u);
+  return d;
+}
+// light_draws followed by the light's pick uniform of the one-sample NEE:
+// an area light with triangles takes the uniform from its pair's step.
+__device__ __forceinline__ LightDraws light_draws_pick(const LightDev& L, Rng& rng, float& u) {
+  LightDraws d{0u, 0.f, 0.f};
+  if (L.type == kLightArea && L.num_tris != 0) {
+    d.tri = L.num_tris > 1 ? rng.index(L.num_tris) : 0u;
+    rng.uniform_pair16_and_pick(d.u1, d.u2, u);
+  } else {
+    u = rng.uniform();
+  }
   return d;
 }
 __device__ __forceinline__ LightDraws light_draws(const LightDev& L, Rng& rng) {
@@ -409,10 +422,10 @@ __device__ __forceinline__ bool nee_sample(const SceneView& sc, f3 P, f3 N, f3 b
   const int nl = sc.num_lights;
   // Lights in scene order. A run of simple area lights goes two per
   // iteration without branches on the kind: both lights' numbers are drawn
-  // first, in the stream's order (the barycentric pair and the pick uniform
-  // of l, then of l + 1), so their weights are independent work the scheduler can
-  // interleave; the reservoir then takes l before l + 1. Other lights go
-  // one at a time.
+  // first, in the stream's order (the barycentric pair and pick uniform of
+  // l from one step, then those of l + 1), so their weights are
+  // independent work the scheduler can interleave; the reservoir then
+  // takes l before l + 1. Other lights go one at a time.
   int l = 0;
   while (l < nl) {
     const int run = (int)const_load(sc.lights, (uint32_t)l).simple_run;
@@ -423,10 +436,9 @@ __device__ __forceinline__ bool nee_sample(const SceneView& sc, f3 P, f3 N, f3 b
         const LightDev Lb = const_load(sc.lights, (uint32_t)(l + 1));
         const AreaTri ta = const_load(sc.area_tris, La.first_tri);
         const AreaTri tb = const_load(sc.area_tris, Lb.first_tri);
-        const LightDraws da = simple_draws(rng);
-        const float ua = rng.uniform();
-        const LightDraws db = simple_draws(rng);
-        const float ub = rng.uniform();
+        float ua, ub;
+        const LightDraws da = simple_draws(rng, ua);
+        const LightDraws db = simple_draws(rng, ub);
         const float wa = simple_area_weight<VERTEX>(ta, La.wmax, da.u1, da.u2, P, N, maxbw, front);
         const float wb = simple_area_weight<VERTEX>(tb, Lb.wmax, db.u1, db.u2, P, N, maxbw, front);
         take(l, da, wa, ua);
@@ -435,15 +447,15 @@ __device__ __forceinline__ bool nee_sample(const SceneView& sc, f3 P, f3 N, f3 b
       if (l < end) {
         const LightDev L = const_load(sc.lights, (uint32_t)l);
         const AreaTri t = const_load(sc.area_tris, L.first_tri);
-        const LightDraws d = simple_draws(rng);
-        const float u = rng.uniform();
+        float u;
+        const LightDraws d = simple_draws(rng, u);
         take(l, d, simple_area_weight<VERTEX>(t, L.wmax, d.u1, d.u2, P, N, maxbw, front), u);
         ++l;
       }
     } else {
       const LightDev L = const_load(sc.lights, (uint32_t)l);
-      const LightDraws d = light_draws(L, rng);
-      const float u = rng.uniform();
+      float u;
+      const LightDraws d = light_draws_pick(L, rng, u);
       take(l, d, light_weight<VERTEX>(sc, L, d, P, N, bw, maxbw, front), u);
       ++l;
     }

@@ -222,6 +222,14 @@ struct Rng {
     a = (float)(x >> 16) * 0x1p-16f;
     b = (float)(x & 0xffffu) * 0x1p-16f;
   }
+  // The pair above and, from the same step, a 24-bit uniform from the
+  // state's other scrambler (xoshiro128+: s0 + s3, before the update): an
+  // area light's barycentric pair and pick uniform for one state update.
+  SHB_HD void uniform_pair16_and_pick(float& a, float& b, float& u) {
+    const uint32_t plus = s0 + s3;
+    uniform_pair16(a, b);
+    u = (float)(plus >> 8) * 0x1p-24f;
+  }
   SHB_HD uint32_t index(uint32_t n) {
     return (uint32_t)(((uint64_t)next() * (uint64_t)n) >> 32);
   }
