@@ -515,12 +515,18 @@ def main():
     # every entry point returns after its stream has drained, so the timed
     # region is bracketed by device synchronisation on both sides
     times, rays, trace_s, trace_sh_s, launches = [], 0, 0.0, 0.0, [0, 0]
+    rays_split = {"extension": 0, "shadow_queries": 0, "shadow_traced": 0,
+                  "shadow_resolved_by_emitter_test": 0}
     t_start = time.perf_counter()
     for i in range(args.steps):
         ts = time.perf_counter()
         _, _, _, st = step()
         times.append(time.perf_counter() - ts)
         rays += st["rays_extension"] + st["rays_shadow"]
+        rays_split["extension"] += st["rays_extension"]
+        rays_split["shadow_queries"] += st["rays_shadow"]
+        rays_split["shadow_traced"] += st["rays_shadow"] - st["rays_shadow_resolved"]
+        rays_split["shadow_resolved_by_emitter_test"] += st["rays_shadow_resolved"]
         trace_s += st["seconds_trace"]
         trace_sh_s += st["seconds_trace_shadow"]
         launches[0] += st["launches_extension"]
@@ -540,7 +546,9 @@ def main():
     roofline, kernels, valu_roofline = None, None, None
     if not args.no_count:
         _, _, stc = b.bake(args.spp, args.bounces, seed=12345, flags=shb.BAKE_COUNT_VISITS)
-        n_sh = stc["rays_shadow"]
+        # the shadow traversal's rays: the queries minus those its own
+        # emitter's triangle test answered in the shading kernels
+        n_sh = stc["rays_shadow"] - stc["rays_shadow_resolved"]
         n_ext = stc["rays_extension"]
         nodes_sh, tris_sh = stc["node_visits_shadow"], stc["tri_tests_shadow"]
         nodes_ext, tris_ext = stc["node_visits"] - nodes_sh, stc["tri_tests"] - tris_sh
@@ -664,7 +672,8 @@ def main():
                    "lights": len(scene.lights), "lightmap": f"{args.res}x{args.res}",
                    "supersample_scale": args.ss, "spp": args.spp, "bounces": args.bounces,
                    "dilation": args.dilation, "valid_texels": valid,
-                   "rays_per_step": rays // args.steps},
+                   "rays_per_step": rays // args.steps,
+                   "rays_per_step_by_kind": {k: v // args.steps for k, v in rays_split.items()}},
         "roofline": roofline,
         "roofline_kernels": kernels,
         "valu_roofline": valu_roofline,

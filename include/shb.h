@@ -138,10 +138,14 @@ typedef struct shb_surface_point {
 enum {
   SHB_BAKE_COUNT_VISITS = 1u << 0, /* also count BVH node visits / triangle
                                       tests (slower; for measurement only) */
-  SHB_BAKE_USE_GRAPH = 1u << 1     /* capture the launch sequence in a hipGraph
+  SHB_BAKE_USE_GRAPH = 1u << 1,    /* capture the launch sequence in a hipGraph
                                       and replay it while the plan, buffers and
                                       tuning are unchanged (per-launch trace
                                       timings are then not reported)          */
+  SHB_BAKE_TRACE_ALL_SHADOW = 1u << 2 /* trace every shadow query, also those
+                                      the emitter test would answer (see
+                                      rays_shadow_resolved): the same results,
+                                      for checks and measurement              */
 };
 
 /* Reference `BakeConfig` (src/baker.h:10-14) plus the seed of the
@@ -159,7 +163,9 @@ typedef struct shb_bake_stats {
   uint64_t valid_texels;    /* texels with material_id >= 0                 */
   uint64_t paths;           /* valid_texels * samples                       */
   uint64_t rays_extension;  /* closest-hit queries traced                   */
-  uint64_t rays_shadow;     /* any-hit queries traced                       */
+  uint64_t rays_shadow;     /* any-hit queries (every shadow ray of the
+                               reference's FindOcclusion calls), including
+                               rays_shadow_resolved                         */
   uint64_t node_visits;     /* only with SHB_BAKE_COUNT_VISITS              */
   uint64_t tri_tests;       /* only with SHB_BAKE_COUNT_VISITS              */
   double seconds_total;     /* wall time of the call                        */
@@ -177,6 +183,12 @@ typedef struct shb_bake_stats {
   uint64_t branches;
   uint32_t passes;
   uint32_t reserved;
+  /* shadow queries of area lights answered without traversal: the ray
+   * reaches past the sampled emitter point (tfar = |d| - 0.001 with t in
+   * units of |d|, light.h:176-188), and one watertight test accepts the
+   * emitter's own triangle well inside the ray's range, so the any-hit query
+   * is occluded; rays_shadow - rays_shadow_resolved rays were traced       */
+  uint64_t rays_shadow_resolved;
 } shb_bake_stats;
 
 /* Launch parameters of the wavefront kernels (chosen by shb_autotune).

@@ -78,9 +78,10 @@ inline int check_overflow(Ctx& c, size_t cap) {
 
 // Counter layout per batch (uint32): extension-queue counts by depth, shadow
 // counts by depth, queue heads of the persistent extension and shadow
-// traversals by depth, branch slots asked for by alpha < 1 splits.
+// traversals by depth, branch slots asked for by alpha < 1 splits, shadow
+// queries answered without traversal by depth.
 constexpr int kCntExt = 0, kCntSh = 20, kCntHeadExt = 40, kCntHeadSh = 56, kCntSpawn = 72,
-              kCntStride = 76;
+              kCntRes = 76, kCntStride = 96;
 
 // SHB_DEBUG_STAGES=1: synchronise after every bake stage and log its time to
 // stderr (diagnostics; off by default).
@@ -111,7 +112,8 @@ inline hipError_t stage_done(hipStream_t st, const char* name, uint32_t batch, i
 constexpr size_t kWorkBytes = 16 + 2 * 64 * 16;
 
 struct BakeRun {
-  uint64_t rays_ext = 0, rays_sh = 0, valid = 0, node_visits = 0, tri_tests = 0;
+  uint64_t rays_ext = 0, rays_sh = 0, rays_sh_res = 0, valid = 0, node_visits = 0,
+           tri_tests = 0;
   uint64_t node_visits_sh = 0, tri_tests_sh = 0;
   double dev_s = 0, trace_s = 0, trace_sh_s = 0;
   uint32_t launches_ext = 0, launches_sh = 0;
@@ -125,7 +127,8 @@ inline void fill_stats(shb_bake_stats* s, const BakeRun& r, uint32_t spp, double
   s->valid_texels = r.valid;
   s->paths = r.valid * spp;
   s->rays_extension = r.rays_ext;
-  s->rays_shadow = r.rays_sh;
+  s->rays_shadow = r.rays_sh + r.rays_sh_res;
+  s->rays_shadow_resolved = r.rays_sh_res;
   s->node_visits = r.node_visits;
   s->tri_tests = r.tri_tests;
   s->seconds_total = now_s() - t0;

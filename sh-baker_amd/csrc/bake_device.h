@@ -5,6 +5,7 @@
 #define SHB_BAKE_DEVICE_H_
 
 #include "shb_common.h"
+#include "traverse.h"
 
 namespace shb {
 
@@ -95,6 +96,7 @@ struct NeeResult {
   f3 origin;
   float tfar;
   int light;
+  uint32_t tri;   // an area light's sampled triangle (for emitter_occludes)
 };
 
 // Point on an area-light triangle at folded barycentrics (u1, u2)
@@ -128,8 +130,10 @@ __device__ __forceinline__ float dot_fma(f3 a, f3 b) {
 // pending shadow ray per light (light.h:34-203; area lights sampled as in
 // light.cpp:16-78), weight = maxCoeff, selection by a streaming reservoir:
 // for each light draw u, W += w, keep light i when u*W < w -- the same
-// w_i / sum(w) selection law as std::discrete_distribution. Returns false
-// when nothing is selected, sum(w) < 1e-6 or joint pdf < 1e-3.
+// w_i / sum(w) selection law as std::discrete_distribution. Returns kNeeNone
+// when nothing is selected, sum(w) < 1e-6 or joint pdf < 1e-3, else
+// kNeeShadow with the pending shadow ray in out (the caller may answer it
+// with emitter_occludes instead of tracing it).
 // Selection weights use the radiance without the light / albedo scales of
 // shb_set_params, so the selection law does not depend on the fitted
 // parameters (with every scale 1 this is the reference's law exactly); the
@@ -401,9 +405,45 @@ __device__ __forceinline__ float simple_area_weight(const AreaTri& at, float wma
   return cos_n < 0.f ? 0.f : w;
 }
 
+// Shadow query of an area light answered without traversal. The ray runs
+// from P toward a point of one of the light's triangles, with t in units of
+// |dir| (the point sits at t = 1), and reaches tfar = |dir| - 0.001
+// (light.h:176-188): past the emitter whenever the light is farther than
+// 1.001. The any-hit query (occlusion.cpp:88-107) is then occluded as soon as
+// the watertight test accepts that triangle's own BVH record inside [tnear,
+// tfar]: the traversal would report it or an earlier hit. emitter_occludes
+// is the traversal's test (same ray setup, same triangle record, same
+// arithmetic) and only accepts t in [0.5, 1.005] with tfar >= 1.01, well
+// inside the range, where the traversal's widened box tests cannot cull the
+// triangle; any other ray is traced.
+// True when the shadow ray (origin o, tnear in o.w; direction d, tfar in
+// d.w) toward triangle `tri` of light L is known to be occluded by that
+// triangle (its copy of the BVH record, AreaTri::rec).
+__device__ __forceinline__ bool emitter_occludes(const SceneView& sc, const LightDev& L,
+                                                 uint32_t tri, float4 o, float4 d) {
+  if (!sc.emitter_test || L.type != kLightArea || L.num_tris == 0 || !(d.w >= 1.01f) ||
+      !(o.w <= 0.5f) || !ray_is_finite(o, d))
+    return false;
+  const AreaTri& at = sc.area_tris[L.first_tri + tri];
+  if (at.leaf == kNoLeaf) return false;
+  TravRay r;
+  trav_setup(r, o, d);
+  float rx, ry, rz;
+  rot3(r.kz, r.ox, r.oy, r.oz, rx, ry, rz);
+  float t, u, v;
+  if (!tri_test(r.kz, rx, ry, rz, r.Sx, r.Sy, r.Sz, at.rec[0], at.rec[1], at.rec[2], t, u, v))
+    return false;
+  return t >= 0.5f && t <= 1.005f;
+}
+
+// nee_sample's outcomes: nothing picked (or pdf too small), a shadow ray, or
+// (the caller's verdict) a pick whose shadow query emitter_occludes answered.
+enum : uint32_t { kNeeNone = 0, kNeeShadow = 1, kNeeOccluded = 2 };
+
 template <bool VERTEX, bool UNIT>
-__device__ __forceinline__ bool nee_sample(const SceneView& sc, f3 P, f3 N, f3 brdf, f3 brdf_w,
-                                           bool front, Rng& rng, NeeResult& out) {
+__device__ __forceinline__ uint32_t nee_sample(const SceneView& sc, f3 P, f3 N, f3 brdf,
+                                               f3 brdf_w, bool front, Rng& rng,
+                                               NeeResult& out) {
   const f3 bw = UNIT ? brdf : brdf_w;
   const float maxbw = maxc(bw);
   float W = 0.f, pw = 0.f;
@@ -460,20 +500,21 @@ __device__ __forceinline__ bool nee_sample(const SceneView& sc, f3 P, f3 N, f3 b
       ++l;
     }
   }
-  if (W < 1e-6f || pick < 0) return false;
+  if (W < 1e-6f || pick < 0) return kNeeNone;
   f3 rad, radw, sdir;
   float tfar, ap;
   light_geom<VERTEX, UNIT>(sc, pick, sc.lights[pick], pd, P, N, brdf, brdf_w, front, rad, radw,
                            sdir, tfar, ap);
   float pdf = pw / W;
   float joint = pdf * ap;
-  if (joint < 1e-3f) return false;
+  if (joint < 1e-3f) return kNeeNone;
+  out.tri = pd.tri;
   out.L = rad / joint;
   out.dir = sdir;
   out.origin = P;
   out.tfar = tfar;
   out.light = pick;
-  return true;
+  return kNeeShadow;
 }
 
 // n > 1 light samples (light.cpp:164-188 and 266-290): the reference's n
@@ -484,11 +525,14 @@ __device__ __forceinline__ bool nee_sample(const SceneView& sc, f3 P, f3 N, f3 b
 // whose interval of the running sum holds it -- light i with probability
 // w_i / W, as discrete_distribution's. pick(j, result, joint) is called
 // for every target whose joint pdf is at least 1e-3 (none when W < 1e-6),
-// with result.L the light's radiance (not yet divided by the pdf). rng
+// with result.L the light's radiance (not yet divided by the pdf), except
+// targets whose light's shadow query ends on its own emitter
+// (emitter_occludes): those are returned as a mask of occluded picks. rng
 // ends after the targets (the material sample continues from there).
 template <bool VERTEX, bool UNIT, class F>
-__device__ __forceinline__ void nee_multi(const SceneView& sc, f3 P, f3 N, f3 brdf, f3 brdf_w,
-                                          bool front, Rng& rng, int n, F&& pick) {
+__device__ __forceinline__ uint32_t nee_multi(const SceneView& sc, f3 P, f3 N, f3 brdf,
+                                              f3 brdf_w, bool front, Rng& rng, int n,
+                                              F&& pick) {
   const Rng start = rng;
   const int nl = sc.num_lights;
   const f3 bw = UNIT ? brdf : brdf_w;
@@ -502,33 +546,47 @@ __device__ __forceinline__ void nee_multi(const SceneView& sc, f3 P, f3 N, f3 br
   float target[kMaxLightSamples];
 #pragma unroll
   for (int j = 0; j < kMaxLightSamples; ++j) target[j] = j < n ? rng.uniform() * W : 0.f;
-  if (W < 1e-6f) return;
+  if (W < 1e-6f) return 0u;
   Rng replay = start;
   uint32_t open = (n >= 32 ? 0xffffffffu : (1u << n) - 1u);
+  uint32_t occluded = 0u;
   float cum = 0.f;
   for (int l = 0; l < nl && open; ++l) {
     f3 rad, radw, sdir;
     float tfar, ap;
-    light_eval<VERTEX, UNIT>(sc, l, P, N, brdf, brdf_w, front, replay, rad, radw, sdir, tfar,
+    const LightDev L = const_load(sc.lights, (uint32_t)l);
+    const LightDraws dr = light_draws(L, replay);
+    light_geom<VERTEX, UNIT>(sc, l, L, dr, P, N, brdf, brdf_w, front, rad, radw, sdir, tfar,
                              ap);
     const float w = maxc(radw);
     const float next = cum + w;
+    // the targets in this light's interval (they share its sample and pdf)
+    uint32_t here = 0u;
 #pragma unroll
-    for (int j = 0; j < kMaxLightSamples; ++j) {
-      if (!(open & (1u << j)) || !(w > 0.f) || !(target[j] < next)) continue;
-      open &= ~(1u << j);
-      const float joint = (w / W) * ap;
-      if (joint < 1e-3f) continue;
-      NeeResult r;
-      r.L = rad;
-      r.dir = sdir;
-      r.origin = P;
-      r.tfar = tfar;
-      r.light = l;
-      pick(j, r, joint);
+    for (int j = 0; j < kMaxLightSamples; ++j)
+      if ((open & (1u << j)) && w > 0.f && target[j] < next) here |= 1u << j;
+    open &= ~here;
+    const float joint = (w / W) * ap;
+    if (here && !(joint < 1e-3f)) {
+      if (emitter_occludes(sc, L, dr.tri, make_float4(P.x, P.y, P.z, kRayEps),
+                           make_float4(sdir.x, sdir.y, sdir.z, tfar))) {
+        occluded |= here;
+      } else {
+        NeeResult r;
+        r.L = rad;
+        r.dir = sdir;
+        r.origin = P;
+        r.tfar = tfar;
+        r.light = l;
+        r.tri = dr.tri;
+#pragma unroll
+        for (int j = 0; j < kMaxLightSamples; ++j)
+          if (here & (1u << j)) pick(j, r, joint);
+      }
     }
     cum = next;
   }
+  return occluded;
 }
 
 }  // namespace shb

@@ -546,7 +546,7 @@ __global__ void k_hit_details(SceneView sc, const float4* rays, const float4* hi
 // light and albedo scale is 1).
 template <bool UNIT, bool MULTI, bool ONE>
 __global__ void __launch_bounds__(256, 5) k_generate(BakeArgs a) {
-  __shared__ uint32_t qw[5];
+  __shared__ uint32_t qw[10];
   const uint32_t n = a.P;
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
     uint32_t p = base + threadIdx.x;
@@ -556,18 +556,19 @@ __global__ void __launch_bounds__(256, 5) k_generate(BakeArgs a) {
       with_job<ONE>(a, sr.job,
                     [&](const JobDev& J) { shadow = generate_path<UNIT, MULTI>(a, J, sr, p); });
     }
-    uint32_t* const sq[1] = {a.shadow_count};
+    // shadow rays, and the shadow queries answered without one (counted)
+    uint32_t* const sq[2] = {a.shadow_count, a.resolved_count};
     if (MULTI) {
       for (uint32_t j = 0; j < a.nls; ++j) {  // block-uniform bound
-        const bool made[1] = {((shadow >> j) & 1u) != 0u};
-        uint32_t at[1];
-        block_append<1>(sq, made, at, qw);
+        const bool made[2] = {((shadow >> j) & 1u) != 0u, ((shadow >> (16 + j)) & 1u) != 0u};
+        uint32_t at[2];
+        block_append<2>(sq, made, at, qw);
         if (made[0]) a.shadow_queue[at[0]] = a.nls * a.Pc + j * a.P + p;
       }
     } else {
-      const bool made[1] = {shadow != 0u};
-      uint32_t at[1];
-      block_append<1>(sq, made, at, qw);
+      const bool made[2] = {(shadow & 0xffffu) != 0u, (shadow >> 16) != 0u};
+      uint32_t at[2];
+      block_append<2>(sq, made, at, qw);
       if (made[0]) a.shadow_queue[at[0]] = a.Pc + p;
     }
   }
@@ -589,7 +590,7 @@ __global__ void __launch_bounds__(256, (UNIT && ONE && !MULTI && !SPLIT && !ADJ)
                                                   const uint32_t* in_count, uint32_t* out_queue,
                                                   uint32_t* out_count) {
   extern __shared__ float grad_lds[];
-  __shared__ uint32_t qw[15];
+  __shared__ uint32_t qw[20];
   GradAcc acc;
   if (ADJ)
     acc.init(a.adj_nparam <= kAdjLdsParams ? grad_lds : nullptr, a.adj_grad, a.adj_nparam);
@@ -604,32 +605,47 @@ __global__ void __launch_bounds__(256, (UNIT && ONE && !MULTI && !SPLIT && !ADJ)
       const SlotRef sr = slot_ref(a, SPLIT ? root_slot(a, p) : p);
       with_job<ONE>(a, sr.job, [&](const JobDev& J) {
         shade_path<ADJ, SPLIT, UNIT, MULTI>(a, J, sr, depth, p, shadow, extend, child, &acc);
+        // the stored shadow ray, read back with its light (scontrib.w) now
+        // that the path's values are dead, answered by the emitter's triangle
+        // test where it can be (single-triangle emitters: the sampled
+        // triangle is not stored)
+        if (!MULTI && shadow == 1u) {
+          const float4 d = a.sray[2 * (size_t)p + 1];
+          if (d.w >= 1.01f) {
+            const LightDev& L = J.scene.lights[as_uint(a.scontrib[p].w)];
+            if (L.num_tris == 1 && emitter_occludes(J.scene, L, 0u, a.sray[2 * (size_t)p], d))
+              shadow = 1u << 16;
+          }
+        }
       });
     }
     // shadow rays (one queue per pick j when MULTI), the continuation and,
-    // with SPLIT, the transmitted ray of a split (the same queue)
+    // with SPLIT, the transmitted ray of a split (the same queue); shadow
+    // queries answered without a ray are counted
     if (MULTI) {
-      uint32_t* const sq[1] = {a.shadow_count};
+      uint32_t* const sq[2] = {a.shadow_count, a.resolved_count};
       for (uint32_t j = 0; j < a.nls; ++j) {  // block-uniform bound
-        const bool made[1] = {((shadow >> j) & 1u) != 0u};
-        uint32_t at[1];
-        block_append<1>(sq, made, at, qw);
+        const bool made[2] = {((shadow >> j) & 1u) != 0u, ((shadow >> (16 + j)) & 1u) != 0u};
+        uint32_t at[2];
+        block_append<2>(sq, made, at, qw);
         if (made[0]) a.shadow_queue[at[0]] = j * a.Pc + p;
       }
     }
+    const bool made = !MULTI && (shadow & 0xffffu) != 0u;
+    const bool resolved = !MULTI && (shadow >> 16) != 0u;
     if (SPLIT) {
-      uint32_t* const qs[3] = {a.shadow_count, out_count, out_count};
-      const bool pr[3] = {!MULTI && shadow != 0u, extend, child != kNoSlot};
-      uint32_t at[3];
-      block_append<3>(qs, pr, at, qw);
+      uint32_t* const qs[4] = {a.shadow_count, out_count, out_count, a.resolved_count};
+      const bool pr[4] = {made, extend, child != kNoSlot, resolved};
+      uint32_t at[4];
+      block_append<4>(qs, pr, at, qw);
       if (pr[0]) a.shadow_queue[at[0]] = p;
       if (pr[1]) out_queue[at[1]] = p;
       if (pr[2]) out_queue[at[2]] = child;
     } else {
-      uint32_t* const qs[2] = {a.shadow_count, out_count};
-      const bool pr[2] = {!MULTI && shadow != 0u, extend};
-      uint32_t at[2];
-      block_append<2>(qs, pr, at, qw);
+      uint32_t* const qs[3] = {a.shadow_count, out_count, a.resolved_count};
+      const bool pr[3] = {made, extend, resolved};
+      uint32_t at[3];
+      block_append<3>(qs, pr, at, qw);
       if (pr[0]) a.shadow_queue[at[0]] = p;
       if (pr[1]) out_queue[at[1]] = p;
     }

@@ -73,7 +73,8 @@ __device__ __forceinline__ void with_job(const BakeArgs& a, uint32_t job, F&& f)
 // Primary ray and texel NEE of path slot p (baker.cpp:245-276,
 // light.cpp:191-293). Returns the mask of texel shadow rays made: bit j for
 // the ray of pick j at shadow slot nls * Pc + j * P + p (nls = 1: bit 0,
-// slot Pc + p). UNIT: every light and albedo scale is 1. MULTI: the batch
+// slot Pc + p); bit 16 + j when pick j's shadow query was answered occluded
+// without a ray (emitter_occludes). UNIT: every light and albedo scale is 1. MULTI: the batch
 // has a job with num_light_samples > 1 (per-pick arrays, visible flags).
 template <bool UNIT, bool MULTI>
 __device__ __forceinline__ uint32_t generate_path(const BakeArgs& a, const JobDev& J,
@@ -127,18 +128,30 @@ __device__ __forceinline__ uint32_t generate_path(const BakeArgs& a, const JobDe
       made |= 1u << j;
     };
     const int n = (int)J.nls;
+    uint32_t occluded = 0u;
     if (n > 1) {
       // Li = L / (joint n) per visible pick (light.cpp:287-288)
-      nee_multi<false, UNIT>(J.scene, P, N, zero, zero, false, rng, n,
-                             [&](int j, const NeeResult& r, float joint) {
-                               put(j, r.L / (joint * (float)n), r);
-                             });
-    } else if (nee_sample<false, UNIT>(J.scene, P, N, zero, zero, false, rng, nr)) {
-      put(0, nr.L, nr);
+      occluded = nee_multi<false, UNIT>(J.scene, P, N, zero, zero, false, rng, n,
+                                        [&](int j, const NeeResult& r, float joint) {
+                                          put(j, r.L / (joint * (float)n), r);
+                                        });
+    } else if (nee_sample<false, UNIT>(J.scene, P, N, zero, zero, false, rng, nr) == kNeeShadow) {
+      if (emitter_occludes(J.scene, J.scene.lights[nr.light], nr.tri,
+                           make_float4(P.x, P.y, P.z, kRayEps),
+                           make_float4(nr.dir.x, nr.dir.y, nr.dir.z, nr.tfar)))
+        occluded = 1u;
+      else
+        put(0, nr.L, nr);
     }
-    return made;
+    return made | occluded << 16;
   }
-  if (nee_sample<false, UNIT>(J.scene, P, N, zero, zero, false, rng, nr)) {
+  uint32_t ns = nee_sample<false, UNIT>(J.scene, P, N, zero, zero, false, rng, nr);
+  if (ns == kNeeShadow &&
+      emitter_occludes(J.scene, J.scene.lights[nr.light], nr.tri,
+                       make_float4(P.x, P.y, P.z, kRayEps),
+                       make_float4(nr.dir.x, nr.dir.y, nr.dir.z, nr.tfar)))
+    ns = kNeeOccluded;
+  if (ns == kNeeShadow) {
     a.nee0[p] = make_float4(nr.L.x, nr.L.y, nr.L.z, 0.f);
     a.nee0_dir[p] = make_float4(nr.dir.x, nr.dir.y, nr.dir.z, as_float((uint32_t)nr.light));
     size_t so = (size_t)a.Pc + p;
@@ -147,7 +160,7 @@ __device__ __forceinline__ uint32_t generate_path(const BakeArgs& a, const JobDe
     return 1u;
   }
   a.nee0[p] = make_float4(0.f, 0.f, 0.f, 0.f);
-  return 0u;
+  return ns == kNeeOccluded ? 1u << 16 : 0u;
 }
 
 // Appends a wave's spawning lanes to the batch's branch slots (one atomic
@@ -170,7 +183,9 @@ __device__ __forceinline__ uint32_t claim_branch_slot(const BakeArgs& a) {
 // Shading of path slot p's closest hit at depth `depth` (baker.cpp:94-146):
 // sky, alpha split, emission at depth 0, vertex NEE (shadow slot p; pick j
 // at j * Pc + p), cosine-weighted continuation. Sets `shadow` to the mask of
-// vertex shadow rays made (bit j: pick j), `extend` when a continuation ray
+// vertex shadow rays made (bit j: pick j; bit 16 + j: pick j's query answered
+// occluded without a ray; with one light sample the caller tests the stored
+// ray with emitter_occludes instead), `extend` when a continuation ray
 // was made and `child` to the branch slot of a transmitted ray (kNoSlot:
 // none). `sr` is the root slot's texel and sample. MULTI: per-pick
 // contributions with visible flags (the batch has a job with
@@ -304,14 +319,26 @@ __device__ __forceinline__ void shade_path(const BakeArgs& a, const JobDev& J, c
       shadow |= 1u << j;
     };
     const int n = (int)J.nls;
-    if (n > 1)
-      nee_multi<true, UNIT>(sc, hit_pos, nrm, brdf, brdf_w, front, rng, n,
-                            [&](int j, const NeeResult& r, float joint) {
-                              put(j, r.L / joint, r);
-                            });
-    else if (nee_sample<true, UNIT>(sc, hit_pos, nrm, brdf, brdf_w, front, rng, nr))
-      put(0, nr.L, nr);
-  } else if (nee_sample<true, UNIT>(sc, hit_pos, nrm, brdf, brdf_w, front, rng, nr)) {
+    if (n > 1) {
+      const uint32_t occluded =
+          nee_multi<true, UNIT>(sc, hit_pos, nrm, brdf, brdf_w, front, rng, n,
+                                [&](int j, const NeeResult& r, float joint) {
+                                  put(j, r.L / joint, r);
+                                });
+      shadow |= occluded << 16;
+    } else if (nee_sample<true, UNIT>(sc, hit_pos, nrm, brdf, brdf_w, front, rng, nr) ==
+               kNeeShadow) {
+      if (emitter_occludes(sc, sc.lights[nr.light], nr.tri,
+                           make_float4(hit_pos.x, hit_pos.y, hit_pos.z, kRayEps),
+                           make_float4(nr.dir.x, nr.dir.y, nr.dir.z, nr.tfar)))
+        shadow |= 1u << 16;
+      else
+        put(0, nr.L, nr);
+    }
+  } else if (nee_sample<true, UNIT>(sc, hit_pos, nrm, brdf, brdf_w, front, rng, nr) ==
+             kNeeShadow) {
+    // the shadow ray is stored; the caller may still answer it with
+    // emitter_occludes once the path's other values are dead
     f3 c = cmul(thr, nr.L);
     a.scontrib[p] = make_float4(c.x, c.y, c.z, as_float((uint32_t)nr.light));
     a.sray[2 * (size_t)p] = make_float4(hit_pos.x, hit_pos.y, hit_pos.z, kRayEps);

@@ -217,6 +217,7 @@ int run_pass(Ctx& x, std::vector<JobIn>& jobs, bool graph, BakeRun* run, const A
     JobDev& J = jt[k];
     std::memset(&J, 0, sizeof(J));
     J.scene = c.view();
+    J.scene.emitter_test = (jobs[k].cfg.flags & SHB_BAKE_TRACE_ALL_SHADOW) ? 0 : 1;
     J.surface = c.d_surface.as<float4>();
     J.texel_list = c.d_list.as<uint32_t>() + jobs[k].list_offset;
     J.sh_out = c.d_sh.as<float>();
@@ -382,6 +383,7 @@ int run_pass(Ctx& x, std::vector<JobIn>& jobs, bool graph, BakeRun* run, const A
       uint32_t* qext[2] = {q, q + Pc};
       b.shadow_queue = q + 2 * (size_t)Pc;
       b.shadow_count = cnt + kCntSh;
+      b.resolved_count = cnt + kCntRes;
       if (adj) {
         b.adj_dsh = adj->dsh;
         b.adj_grad = adj->grad;
@@ -422,6 +424,7 @@ int run_pass(Ctx& x, std::vector<JobIn>& jobs, bool graph, BakeRun* run, const A
           SHB_TRY(x, stage_done(st, "extend", bi, (int)d));
         }
         b.shadow_count = cnt + kCntSh + d;
+        b.resolved_count = cnt + kCntRes + d;
         if (r.dry) {
           r.add(4, b);
           r.add(4, d);
@@ -552,6 +555,7 @@ int run_pass(Ctx& x, std::vector<JobIn>& jobs, bool graph, BakeRun* run, const A
     run->rays_ext += maps[bi].slot_base[maps[bi].nseg];
     for (uint32_t d = 1; d <= maxB[bi]; ++d) run->rays_ext += k[kCntExt + d];
     for (uint32_t d = 0; d <= maxB[bi]; ++d) run->rays_sh += k[kCntSh + d];
+    for (uint32_t d = 0; d <= maxB[bi]; ++d) run->rays_sh_res += k[kCntRes + d];
   }
   std::vector<uint32_t> hw(kWorkBytes / 4);
   SHB_TRY(x, hipMemcpy(hw.data(), x.d_work.ptr, kWorkBytes, hipMemcpyDeviceToHost));

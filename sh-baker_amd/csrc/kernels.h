@@ -122,6 +122,7 @@ struct BakeArgs {
   float4* scontrib;           // vertex NEE contribution per slot
   uint32_t* shadow_queue;
   uint32_t* shadow_count;
+  uint32_t* resolved_count;   // shadow queries of this depth answered untraced
   // path-replay adjoint (single job): dL/dSH per supersampled texel
   // (27 floats), gradient sums (3 per light, then 3 per material), the
   // per-slot projection weights, the materials of each path's vertices

@@ -252,7 +252,9 @@ int build_host_scene(const shb_scene_desc* d, HostScene* out, std::string* err) 
           float area = std::sqrt(h3sq(h3cross(h3sub(v[0], v[1]), h3sub(v[0], v[2])))) / 2.f;
           at.pdf = std::max(1e-6f, 1.f / (float)nt * 1.f / area);
           at.has_uv = hasUV ? 1.f : 0.f;
+          at.leaf = kNoLeaf;  // set from the BVH at upload
           hs.area_tris.push_back(at);
+          hs.area_tri_global.push_back((uint32_t)(geom_first_tri[gi] + (int)t));
         }
         const MaterialDev& md = hs.mats[mi];
         if (md.tex_c == 0 || (md.tex_w == 1 && md.tex_h == 1)) {

@@ -26,6 +26,7 @@ struct HostScene {
   std::vector<uint8_t> texels;
   std::vector<LightDev> lights;
   std::vector<AreaTri> area_tris;
+  std::vector<uint32_t> area_tri_global;  // global triangle id of each area_tris entry
   bool raster_ready = true;          // every geometry has luv, normals, tangents
   bool has_transparency = false;     // some RGBA texel with alpha < 255
   bool bad_material = false;         // a geometry's material_id is not a material

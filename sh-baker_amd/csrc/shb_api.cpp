@@ -46,6 +46,22 @@ int upload_bvh(Ctx& c) {
   SHB_TRY(c, upload(c.d_nodes, nb, c.stream));
   SHB_TRY(c, hipStreamSynchronize(c.stream));  // nb is a temporary
   SHB_TRY(c, upload(c.d_tris, c.bvh.tris, c.stream));
+  // each area-light triangle's record in the leaf order of this tree (the
+  // shading kernels answer shadow queries that end on it with one test)
+  std::vector<uint32_t> leaf_of(c.hs.num_tris, kNoLeaf);
+  for (size_t i = 0; 3 * i < c.bvh.tris.size(); ++i) {
+    const uint32_t id = as_uint(c.bvh.tris[3 * i].w);
+    if (id < c.hs.num_tris) leaf_of[id] = (uint32_t)i;
+  }
+  for (size_t k = 0; k < c.hs.area_tris.size(); ++k) {
+    AreaTri& at = c.hs.area_tris[k];
+    const uint32_t g = c.hs.area_tri_global[k];
+    at.leaf = (!c.bvh.nodes.empty() && g < c.hs.num_tris) ? leaf_of[g] : kNoLeaf;
+    for (int j = 0; j < 3; ++j)
+      at.rec[j] = at.leaf == kNoLeaf ? make_float4(0.f, 0.f, 0.f, 0.f)
+                                     : c.bvh.tris[3 * (size_t)at.leaf + j];
+  }
+  SHB_TRY(c, upload(c.d_area, c.hs.area_tris, c.stream));
   return SHB_OK;
 }
 
@@ -146,7 +162,6 @@ int shb_upload_scene(shb_ctx* h, const shb_scene_desc* scene) {
   SHB_TRY(c, upload(c.d_texels, hs.texels, c.stream));
   SHB_TRY(c, upload(c.d_lut, lut, c.stream));
   SHB_TRY(c, upload(c.d_lights, hs.lights, c.stream));
-  SHB_TRY(c, upload(c.d_area, hs.area_tris, c.stream));
   std::vector<float> ones_l(3 * std::max<size_t>(hs.lights.size(), 1), 1.f);
   std::vector<float> ones_m(3 * std::max<size_t>(hs.mats.size(), 1), 1.f);
   SHB_TRY(c, upload(c.d_lscale, ones_l, c.stream));

@@ -48,6 +48,7 @@ constexpr int kStackLds = 16;         // traversal stack entries (LDS)
 // records the wave-spread triangle tests are off (same reason)
 constexpr uint32_t kMaxTreeNodes = 1u << 24;
 constexpr uint32_t kWaveTrisMaxRecords = 1u << 24;
+constexpr uint32_t kNoLeaf = 0xffffffffu;  // AreaTri::leaf of a triangle outside the BVH
 constexpr float kRayEps = 0.001f;     // baker.cpp:118, 122, light.h:40,83
 
 // ------------------------------------------------------------ BVH node
@@ -130,16 +131,20 @@ struct alignas(16) LightDev {
 
 // Area-light triangle record (object-space vertices, light.cpp:45-49): v0
 // and the edges e1 = v1 - v0, e2 = v2 - v0 (float differences), so a sample
-// point is v0 + u1 e1 + u2 e2 with two fused multiply-adds per axis.
+// point is v0 + u1 e1 + u2 e2 with two fused multiply-adds per axis. rec is
+// a copy of the emitting triangle's record in the BVH's triangle array (the
+// traversal's vertices, bit for bit), set at upload; leaf is that record's
+// index there, kNoLeaf when the triangle has none (rec is then unused).
 struct alignas(16) AreaTri {
   float v0[3];
   float pdf;           // max(1e-6, 1/ntri * 1/area)  (light.cpp:75-76)
   float e1[3];
   float has_uv;
   float e2[3];
-  float pad;
+  uint32_t leaf;
   float uv[6];
   float pad2[2];
+  float4 rec[3];
 };
 
 // ------------------------------------------------------------ materials
@@ -173,7 +178,8 @@ struct SceneView {
   int32_t num_lights;
   int32_t num_tris;
   int32_t has_nodes;
-  int32_t pad;
+  int32_t emitter_test;    // 1: shadow queries ending on their emitter are
+                           // answered by its triangle test (emitter_occludes)
 };
 
 // ------------------------------------------------------------ RNG

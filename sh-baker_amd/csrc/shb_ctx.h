@@ -102,7 +102,7 @@ struct Ctx {
     v.num_lights = (int32_t)hs.lights.size();
     v.num_tris = (int32_t)hs.num_tris;
     v.has_nodes = bvh.nodes.empty() ? 0 : 1;
-    v.pad = 0;
+    v.emitter_test = 1;
     return v;
   }
 };

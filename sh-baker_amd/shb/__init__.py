@@ -28,6 +28,7 @@ LIGHT_POINT, LIGHT_DIRECTIONAL, LIGHT_SPOT, LIGHT_AREA = 0, 1, 2, 3
 GEOM_HAS_NORMALS, GEOM_HAS_TEXTURE_UVS, GEOM_HAS_LIGHTMAP_UVS, GEOM_HAS_TANGENTS = 1, 2, 4, 8
 BAKE_COUNT_VISITS = 1
 BAKE_USE_GRAPH = 2
+BAKE_TRACE_ALL_SHADOW = 4
 SAVE_COMBINED, SAVE_SPLIT, SAVE_LUMINANCE_PACKED = 0, 1, 2
 
 
@@ -98,7 +99,8 @@ class ShbBakeStats(C.Structure):
                 ("node_visits_shadow", C.c_uint64), ("tri_tests_shadow", C.c_uint64),
                 ("seconds_trace_shadow", C.c_double),
                 ("launches_extension", C.c_uint32), ("launches_shadow", C.c_uint32),
-                ("branches", C.c_uint64), ("passes", C.c_uint32), ("reserved", C.c_uint32)]
+                ("branches", C.c_uint64), ("passes", C.c_uint32), ("reserved", C.c_uint32),
+                ("rays_shadow_resolved", C.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
