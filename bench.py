@@ -661,6 +661,10 @@ def main():
         "warmup": args.warmup,
         "ms_per_step": round(ms_per_step, 1),
         "bake_seconds": round(bake_s, 4),
+        # the same rate over the rays the traversal kernels traced (shadow
+        # queries answered by the emitter's triangle test left out)
+        "mrays_s_traced_only": round((rays_split["extension"] + rays_split["shadow_traced"]) /
+                                     elapsed / 1e6, 2),
         "bake_seconds_incl_bvh_build": round(bake_s + upload_s, 4),
         "bake_seconds_outputs_to_host": round(host_s, 4),
         "higher_is_better": True,
