@@ -1154,7 +1154,8 @@ using namespace orc;
 // meta[8] (0 empty; 0x80 | rank: internal child child_base + rank; else a
 // leaf of meta >> 5 triangles at tri_base + (meta & 31)), then the 8-bit
 // child planes lo x, lo y, lo z, hi x, hi y, hi z (8 bytes each); child box
-// plane = p + q * 2^e. Groups of children are visited nearest octant first:
+// plane = p + q * 2^e. Groups of children are visited nearest octant first
+// (farthest first for any-hit queries, as on the device):
 // bit k of a group's hit mask stands for slot k ^ sstar (sstar = the slot
 // opposite the ray's direction octant); the slab test widens each plane
 // distance by 16 ulps of (|b| + 256 |a|) so rounding never culls a box
@@ -1203,7 +1204,11 @@ static void trace(const uint8_t* nodes, uint32_t nn, const float* tris, const sh
   const bool par[3] = {std::fabs(d.x) < 1e-20f, std::fabs(d.y) < 1e-20f, std::fabs(d.z) < 1e-20f};
   const bool neg[3] = {d.x < 0.f, d.y < 0.f, d.z < 0.f};
   const float org[3] = {o.x, o.y, o.z};
-  const uint32_t sstar = ((uint32_t)neg[0] | ((uint32_t)neg[1] << 1) | ((uint32_t)neg[2] << 2)) ^ 7u;
+  // nearest octant first for closest hits; an any-hit query (whose answer
+  // does not depend on the order) takes the children far to near, as the
+  // device traversal does
+  const uint32_t sstar = ((uint32_t)neg[0] | ((uint32_t)neg[1] << 1) | ((uint32_t)neg[2] << 2)) ^
+                         (any_hit ? 0u : 7u);
   float tbest = ray.tfar, bu = 0.f, bv = 0.f;
   int best = INT_MAX;
   std::vector<Group> stack;

@@ -170,11 +170,17 @@ struct LdsStack {
 
 // Ray setup shared by the traversal kernels: the ray starts at its job's
 // BVH root (a batch of several jobs traverses their concatenated trees).
+// Any-hit rays visit each node's children far to near: the answer does not
+// depend on the order, and the bake's shadow rays mostly end on geometry far
+// along the ray (a point light's ray runs on (|d| - 0.001) |d| past the
+// shading point, beyond the light, into the walls).
+template <bool ANY>
 __device__ __forceinline__ void trace_begin(const TraceArgs& a, uint32_t slot, TravRay& r,
                                             TravState& s) {
   const float4 o = a.rays[2 * (size_t)slot];
   const float4 d = a.rays[2 * (size_t)slot + 1];
   trav_setup(r, o, d);
+  if (ANY) r.sstar ^= 7u;
   trav_init(s, r, d.w);
   uint32_t root = a.root0;
   if (a.nseg > 1) {
@@ -237,7 +243,7 @@ __global__ void __launch_bounds__(256) k_trace(TraceArgs a, Epi epi) {
   const uint32_t slot = a.queue ? a.queue[i] : i;
   TravRay r;
   TravState s;
-  trace_begin(a, slot, r, s);
+  trace_begin<ANY>(a, slot, r, s);
   LdsStack stack{stack_mem, (int)threadIdx.x, (int)blockDim.x, (int)a.stack_entries};
   bool overflow = false;
   bool capped = false;
@@ -448,7 +454,7 @@ __global__ void __launch_bounds__(256, 6) k_trace_persistent(TraceArgs a, Epi ep
         }
         if (got) {
           slot = a.queue ? a.queue[qi] : qi;
-          trace_begin(a, slot, r, s);
+          trace_begin<ANY>(a, slot, r, s);
           has = true;
           overflow = false;
           steps = 0;
